@@ -1,0 +1,129 @@
+// C-ABI entry points of libgpb.so.  Each wraps a C++ implementation in an
+// exception guard that turns failures into a status code and a message on the
+// context (the reference throws int64_t(1) after cudaDeviceReset instead,
+// damascene/include/exception.h:16-28).
+#include <cstring>
+
+#include "gpb_capi.h"
+#include "gpb_device.hpp"
+#include "gpb_pipeline.hpp"
+
+using namespace gpb;
+
+namespace {
+
+template <typename F>
+int guarded(gpb_ctx* ctx, F&& fn) {
+  try {
+    if (ctx) GPB_HIP(hipSetDevice(ctx->c.device));
+    fn();
+    return GPB_OK;
+  } catch (const ArgError& e) {
+    if (ctx) ctx->c.err = e.what();
+    return GPB_ERR_ARG;
+  } catch (const HipError& e) {
+    if (ctx) ctx->c.err = e.what();
+    return GPB_ERR_HIP;
+  } catch (const NoConvergence& e) {
+    if (ctx) ctx->c.err = e.what();
+    return GPB_ERR_NOCONV;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->c.err = e.what();
+    return GPB_ERR_INTERNAL;
+  }
+}
+
+void check_image(int width, int height) {
+  if (width <= 0 || height <= 0) throw ArgError("image dimensions must be positive");
+  if (size_t(width) * size_t(height) > (size_t(1) << 31)) throw ArgError("image too large (more than 2^31 pixels)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpb_ctx_create(int device, gpb_ctx** out) {
+  if (!out) return GPB_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GPB_ERR_HIP;
+  auto* ctx = new gpb_ctx();
+  ctx->c.device = device;
+  const int rc = guarded(ctx, [&] {
+    GPB_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+    upload_tables(ctx->c);
+  });
+  if (rc != GPB_OK) {
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return GPB_OK;
+}
+
+void gpb_ctx_destroy(gpb_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.scratch.clear();
+  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char* gpb_ctx_last_error(const gpb_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int gpb_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void gpb_pack_rgb(const uint8_t* rgb, size_t npixels, uint32_t* packed) {
+  for (size_t i = 0; i < npixels; i++) packed[i] = pack_rgb(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+}
+
+int gpb_synth_voronoi(int width, int height, int regions, double noise_sigma, uint64_t seed, int classes,
+                      uint8_t* rgb, int32_t* region) {
+  return guarded(nullptr, [&] {
+    check_image(width, height);
+    if (!rgb || regions <= 0 || noise_sigma < 0 || (classes != 1 && classes != 3)) throw ArgError("bad synth arguments");
+    SynthParams p;
+    p.width = width;
+    p.height = height;
+    p.regions = regions;
+    p.noise_sigma = noise_sigma;
+    p.seed = seed;
+    p.classes = classes;
+    synth_voronoi(p, rgb, region);
+  });
+}
+
+int gpb_make_codebook(const uint32_t* packed, int width, int height, uint64_t seed, int iters, int max_samples,
+                      float* codebook) {
+  return guarded(nullptr, [&] {
+    check_image(width, height);
+    if (!packed || !codebook || iters < 0 || max_samples < kTextonK) throw ArgError("bad codebook arguments");
+    if (size_t(width) * height < size_t(kTextonK)) throw ArgError("image smaller than the codebook");
+    make_codebook(packed, width, height, seed, iters, max_samples, codebook);
+  });
+}
+
+int gpb_texton(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, uint8_t* labels) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!packed || !codebook || !labels) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    uint8_t* d_lab = c.ws<uint8_t>("io.labels", N);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
+    launch_texton(c, d_rgb, width, height, d_cb, d_lab);
+    GPB_HIP(hipMemcpyAsync(labels, d_lab, N, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// Device-side plumbing shared by the gPb HIP sources: error propagation,
+// owning device buffers, and the context object behind the C-ABI handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpb_common.hpp"
+
+namespace gpb {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define GPB_HIP(call)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess)                                                                               \
+      throw ::gpb::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_) + " (" __FILE__ ":" + \
+                            std::to_string(__LINE__) + ")");                                            \
+  } while (0)
+
+#define GPB_LAUNCH_CHECK() GPB_HIP(hipGetLastError())
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) GPB_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  // Grow-only reservation: keeps the allocation when it is already large enough.
+  void reserve(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Per-stage timing collected with HIP events on the context stream.
+struct StageTimer {
+  hipEvent_t ev[16][2];
+  int count = 0;
+  bool enabled = false;
+};
+
+struct LanczosStats {
+  int iterations = 0;       // operator applications (columns)
+  int restarts = 0;
+  int basis = 0;            // maximum basis size m
+  int block = 1;            // block width b
+  int converged = 0;        // number of converged pairs
+  double max_residual = 0;  // max relative generalized residual of returned pairs
+  int fallback = 0;         // warm start rejected, plain solve used
+};
+
+struct Context;
+void upload_tables(Context& c);
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool tables_ready = false;
+  // constant tables
+  DevBuf<int> schmid_off;          // member LDS offsets (texton tile)
+  DevBuf<int> schmid_start;        // class starts
+  DevBuf<float> schmid_w;          // [f][class]
+  int schmid_classes = 0;
+  // workspace (grow-only)
+  std::map<std::string, std::shared_ptr<void>> scratch;
+  LanczosStats last_lanczos;
+  double stage_ms[8] = {0};
+
+  template <typename T>
+  T* ws(const std::string& name, size_t count) {
+    auto it = scratch.find(name);
+    if (it != scratch.end()) {
+      auto* b = static_cast<DevBuf<T>*>(it->second.get());
+      b->reserve(count);
+      return b->p;
+    }
+    auto b = std::make_shared<DevBuf<T>>(count);
+    T* p = b->p;
+    scratch[name] = std::static_pointer_cast<void>(b);
+    return p;
+  }
+};
+
+inline unsigned div_up(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
+
+}  // namespace gpb
+
+struct gpb_ctx {
+  gpb::Context c;
+};
