@@ -66,6 +66,30 @@ int gpb_make_codebook(const uint32_t* packed, int width, int height, uint64_t se
  * (src/SPFeatures.cpp:277-360).  Host pointers. */
 int gpb_texton(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, uint8_t* labels);
 
+/* ---- stage (b): local cues and mPb ------------------------------------------ */
+
+/* Gamma 2.5, RGB->Lab (D65), normalise, quantise to 25 bins; bins[3][N]
+ * (L, a, b planes) and nbins[3] = max(bin)+1 per plane.  Replaces
+ * rgb_gamma_correct / rgb_to_lab / lab_normalize / quantize_values
+ * (globalPb/src/math/libraries/lib_image.cc:123-245, 2315-2344) as called at
+ * src/mex_pb_parts_final_selected.cpp:115-134. */
+int gpb_lab_quantize(gpb_ctx* ctx, int width, int height, const uint32_t* packed, uint8_t* bins, int* nbins);
+
+/* The 12 half-disc chi-squared cues x 8 orientations and their mPb weighted
+ * sum.  cues[12][8][N] (bg r3,5,10; cga, cgb, tg r5,10,20; may be NULL) and
+ * mpb_all[8][N].  Replaces det_mPb (src/det_mPb.cpp:9-22) ->
+ * mex_pb_parts_final_selected (src/mex_pb_parts_final_selected.cpp:76-227) ->
+ * lib_image::hist_gradient_2D (lib_image.cc:3575-3686), and the weighted sum
+ * of multiscalePb (src/multiscalePb.cpp:216-245). */
+int gpb_local_cues(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const uint8_t* textons, float* cues,
+                   float* mpb_all);
+
+/* Max over orientations, oriented nonmax suppression (tolerance pi/8) and the
+ * clip to [0,1]: mpb_all[8][N] -> nmax[N].  Replaces nonmax_channels
+ * (src/nonmax_channels.cpp:8-80), nonmax_oriented_2D (lib_image.cc:4083-4200)
+ * and the clip of src/multiscalePb.cpp:251-261. */
+int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float* nmax);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,3 +127,64 @@ int gpb_texton(gpb_ctx* ctx, int width, int height, const uint32_t* packed, cons
 }
 
 }  // extern "C"
+
+// ---- stage (b) ------------------------------------------------------------------
+extern "C" {
+
+int gpb_lab_quantize(gpb_ctx* ctx, int width, int height, const uint32_t* packed, uint8_t* bins, int* nbins) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!packed || !bins) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    uint8_t* d_q = c.ws<uint8_t>("io.q", 3 * N);
+    int* d_nb = c.ws<int>("io.nb", 3);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    launch_lab_quantize(c, d_rgb, N, d_q, d_nb);
+    GPB_HIP(hipMemcpyAsync(bins, d_q, 3 * N, hipMemcpyDeviceToHost, c.stream));
+    if (nbins) GPB_HIP(hipMemcpyAsync(nbins, d_nb, 3 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gpb_local_cues(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const uint8_t* textons, float* cues,
+                   float* mpb_all) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!packed || !textons || !mpb_all) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    uint8_t* d_t = c.ws<uint8_t>("io.labels", N);
+    float* d_m = c.ws<float>("io.mpb_all", 8 * N);
+    float* d_g = c.ws<float>("io.gcue", 8 * N);
+    float* d_c = cues ? c.ws<float>("io.cues", 96 * N) : nullptr;
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_t, textons, N, hipMemcpyHostToDevice, c.stream));
+    launch_local_cues(c, d_rgb, d_t, width, height, d_m, d_g, d_c);
+    GPB_HIP(hipMemcpyAsync(mpb_all, d_m, 8 * N * 4, hipMemcpyDeviceToHost, c.stream));
+    if (cues) GPB_HIP(hipMemcpyAsync(cues, d_c, 96 * N * 4, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float* nmax) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!mpb_all || !nmax) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    float* d_m = c.ws<float>("io.mpb_all", 8 * N);
+    float* d_n = c.ws<float>("io.nmax", N);
+    GPB_HIP(hipMemcpyAsync(d_m, mpb_all, 8 * N * 4, hipMemcpyHostToDevice, c.stream));
+    launch_nonmax(c, d_m, width, height, d_n);
+    GPB_HIP(hipMemcpyAsync(nmax, d_n, N * 4, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+}  // extern "C"

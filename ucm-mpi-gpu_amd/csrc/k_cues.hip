@@ -1,0 +1,453 @@
+// Stage (b): colour front end, half-disc chi-squared gradients, mPb combine
+// and oriented nonmax.
+//
+// References: mex_pb_parts_final_selected.cpp:76-227 (gamma, Lab, quantise,
+// 12 hist_gradient_2D calls), lib_image.cc:3120-3231 (the histogram
+// gradient), multiscalePb.cpp:216-261 (weighted sum, nonmax, clip),
+// nonmax_channels.cpp:8-80 and lib_image.cc:4083-4200 (oriented nonmax).
+//
+// The chi-squared kernel keeps one wavefront = one 8x8 pixel tile.  Each lane
+// owns a private histogram in LDS laid out [slice/4][bin][lane] with four 8-bit
+// slice counters per dword, so every increment is a conflict-free ds_add_u32
+// whose address depends on the lane's own label only.  Disc offsets are walked
+// in increasing radius, so the three radii of a channel are snapshots of one
+// pass.  Per snapshot the 16 slice histograms are folded into the half-disc
+// chi-squared of all 8 orientations using linearity of the bin smoothing:
+// with a_s = 1 / (c . h_s) (c = column sums of the truncated smoothing
+// matrix G), sum = G * sum_s a_s h_s and diff_o = G * V_o, where
+// V_{o+1} = V_o + 2 a_o h_o - 2 a_{o+8} h_{o+8}.
+#include <array>
+#include <mutex>
+
+#include "gpb_pipeline.hpp"
+
+namespace gpb {
+
+// ---- colour front end ------------------------------------------------------
+// Gamma 2.5, RGB->XYZ->Lab (D65), normalise and quantise to 25 bins in FP64
+// (lib_image.cc:123-138, 172-185, 216-245, 143-167, 2315-2344).  The gamma
+// curve is a 256-entry table of pow(c/255, 2.5).  Also records max(bin)+1 per
+// channel, the histogram length that hist_gradient_2D uses (lib_image.cc:3674).
+__global__ __launch_bounds__(256) void lab_quantize_kernel(const uint32_t* __restrict__ rgb, size_t n,
+                                                           const double* __restrict__ gamma_lut,
+                                                           uint8_t* __restrict__ qL, uint8_t* __restrict__ qa,
+                                                           uint8_t* __restrict__ qb, int* __restrict__ nbins) {
+#pragma clang fp contract(off)
+  __shared__ int smax[3];
+  if (threadIdx.x < 3) smax[threadIdx.x] = 0;
+  __syncthreads();
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < n) {
+    const uint32_t px = rgb[p];
+    const double r = gamma_lut[px & 0xff], g = gamma_lut[(px >> 8) & 0xff], b = gamma_lut[(px >> 16) & 0xff];
+    const double X = (0.412453 * r) + (0.357580 * g) + (0.180423 * b);
+    const double Y = (0.212671 * r) + (0.715160 * g) + (0.072169 * b);
+    const double Z = (0.019334 * r) + (0.119193 * g) + (0.950227 * b);
+    const double x = X / 0.950456, y = Y / 1.000000, z = Z / 1.088754;
+    const double th = 0.008856;
+    const double fx = (x > th) ? pow(x, 1.0 / 3.0) : (7.787 * x + (16.0 / 116.0));
+    const double fy = (y > th) ? pow(y, 1.0 / 3.0) : (7.787 * y + (16.0 / 116.0));
+    const double fz = (z > th) ? pow(z, 1.0 / 3.0) : (7.787 * z + (16.0 / 116.0));
+    const double L = (y > th) ? (116 * pow(y, 1.0 / 3.0) - 16) : (903.3 * y);
+    const double A = 500 * (fx - fy), B = 200 * (fy - fz);
+    double v[3] = {L / 100.0, (A - (-73)) / (95 - (-73)), (B - (-73)) / (95 - (-73))};
+    uint8_t* outs[3] = {qL, qa, qb};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double t = v[c];
+      if (t < 0) t = 0;
+      else if (t > 1) t = 1;
+      int bin = int(floor(t * double(kLabBins)));
+      if (bin == kLabBins) bin = kLabBins - 1;
+      outs[c][p] = uint8_t(bin);
+      atomicMax(&smax[c], bin + 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicMax(&nbins[threadIdx.x], smax[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void label_max_kernel(const uint8_t* __restrict__ lab, size_t n, int* __restrict__ nb) {
+  __shared__ int smax;
+  if (threadIdx.x == 0) smax = 0;
+  __syncthreads();
+  int m = 0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += size_t(gridDim.x) * blockDim.x)
+    m = max(m, int(lab[p]) + 1);
+  atomicMax(&smax, m);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(nb, smax);
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---- half-disc chi-squared ---------------------------------------------------
+constexpr int kCueTile = 8;                         // 8x8 pixels per wavefront
+constexpr int kCueRmax = 20;
+constexpr int kCueTileW = kCueTile + 2 * kCueRmax;  // 48
+constexpr int kCueLanes = kCueTile * kCueTile;      // 64
+constexpr int kNbMax = 32;                          // largest histogram (textons)
+
+struct CueParams {
+  const uint8_t* plane[4];  // L, a, b, texton labels (H x W, unmirrored)
+  const int* nbins;         // [4] histogram lengths
+  const int* off;           // disc offsets (LDS tile offset) sorted by radius
+  const uint8_t* slice;     // slice index per offset
+  int end_by_r[21];         // number of offsets with d^2 <= r^2
+  float smooth[3][9];       // half kernels: [0] bg (sigma 2.5, 17 taps), [1] cg (sigma 1.25, 9 taps), [2] none
+  float w_mpb[12], w_gpb[12];
+  float* mpb_all;           // [8][N]
+  float* gcue;              // [8][N] sum of gPb-weighted cues
+  float* cues;              // optional [12][8][N]
+  int W, H;
+};
+
+// One snapshot: chi-squared of all orientations from the lane's histogram.
+template <int NB, int KH>
+__device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
+                                              const float* __restrict__ g, float out[kNumOri]) {
+  // column sums of the truncated smoothing matrix
+  float csum[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = -KH; k <= KH; k++) {
+      const int n = b - k;
+      if (n >= 0 && n < nb) s += g[k < 0 ? -k : k];
+    }
+    csum[b] = (b < nb) ? s : 0.f;
+  }
+  auto load_slice = [&](int s, float h[NB]) {
+    const int grp = s >> 2, sh = 8 * (s & 3);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const uint32_t w = (b < nb) ? hist[(grp * kNbMax + b) * kCueLanes + lane] : 0u;
+      h[b] = float((w >> sh) & 0xffu);
+    }
+  };
+  float alpha[kNumSlices];
+  float UT[NB], V[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) UT[b] = V[b] = 0.f;
+#pragma unroll
+  for (int s = 0; s < kNumSlices; s++) {
+    float h[NB];
+    load_slice(s, h);
+    float m = 0.f;
+#pragma unroll
+    for (int b = 0; b < NB; b++) m = __builtin_fmaf(csum[b], h[b], m);
+    const float a = (m > 0.f) ? 1.0f / m : 0.f;
+    alpha[s] = a;
+    const float sg = (s < kNumOri) ? -a : a;  // orientation 0: left = slices [0,8)
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      UT[b] = __builtin_fmaf(a, h[b], UT[b]);
+      V[b] = __builtin_fmaf(sg, h[b], V[b]);
+    }
+  }
+  // smoothed sum (same for all orientations)
+  float T[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = -KH; k <= KH; k++) {
+      const int n = b + k;
+      if (n >= 0 && n < NB) s = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], UT[n], s) : s;
+    }
+    T[b] = s;
+  }
+#pragma unroll
+  for (int o = 0; o < kNumOri; o++) {
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      float d = 0.f;
+#pragma unroll
+      for (int k = -KH; k <= KH; k++) {
+        const int n = b + k;
+        if (n >= 0 && n < NB) d = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], V[n], d) : d;
+      }
+      if (b < nb && d != 0.f) acc = __builtin_fmaf(d * d, __builtin_amdgcn_rcpf(T[b]), acc);
+    }
+    out[o] = 0.5f * acc;
+    if (o + 1 < kNumOri) {
+      float h0[NB], h8[NB];
+      load_slice(o, h0);
+      load_slice(o + kNumOri, h8);
+      const float a0 = 2.f * alpha[o], a8 = -2.f * alpha[o + kNumOri];
+#pragma unroll
+      for (int b = 0; b < NB; b++) V[b] = __builtin_fmaf(a8, h8[b], __builtin_fmaf(a0, h0[b], V[b]));
+    }
+  }
+}
+
+template <int NB, int KH>
+__device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radii, uint8_t* tile, uint32_t* hist,
+                            int lane, int px, int py, bool valid, float mpb[kNumOri], float gc[kNumOri]) {
+  const int W = P.W, H = P.H;
+  const int gx0 = int(blockIdx.x) * kCueTile - kCueRmax, gy0 = int(blockIdx.y) * kCueTile - kCueRmax;
+  const uint8_t* plane = P.plane[ch];
+  for (int i = lane; i < kCueTileW * kCueTileW; i += kCueLanes) {
+    const int ty = i / kCueTileW, tx = i - ty * kCueTileW;
+    int gx = gx0 + tx, gy = gy0 + ty;
+    // half-sample symmetric border (border_mirror_2D), periodic beyond the image
+    if (W == 1) gx = 0; else { const int p2 = 2 * W; gx %= p2; if (gx < 0) gx += p2; if (gx >= W) gx = p2 - 1 - gx; }
+    if (H == 1) gy = 0; else { const int p2 = 2 * H; gy %= p2; if (gy < 0) gy += p2; if (gy >= H) gy = p2 - 1 - gy; }
+    tile[i] = plane[size_t(gy) * W + gx];
+  }
+#pragma unroll
+  for (int i = 0; i < 4 * kNbMax; i++) hist[i * kCueLanes + lane] = 0u;
+  __syncthreads();
+  const int nb = P.nbins[ch];
+  const float* g = P.smooth[KH == 8 ? 0 : (KH == 4 ? 1 : 2)];
+  const int ty = lane / kCueTile, tx = lane - ty * kCueTile;
+  const uint8_t* centre = tile + (ty + kCueRmax) * kCueTileW + tx + kCueRmax;
+  int i = 0;
+  for (int ri = 0; ri < 3; ri++) {
+    const int end = P.end_by_r[radii[ri]];
+    for (; i < end; i++) {
+      const int s = P.slice[i];
+      const uint32_t lab = centre[P.off[i]];
+      atomicAdd(&hist[((s >> 2) * kNbMax + lab) * kCueLanes + lane], 1u << (8 * (s & 3)));
+    }
+    float chi[kNumOri];
+    chi2_snapshot<NB, KH>(hist, lane, nb, g, chi);
+    const int cue = ri0 + ri;
+    if (valid) {
+      const size_t N = size_t(W) * H, p = size_t(py) * W + px;
+#pragma unroll
+      for (int o = 0; o < kNumOri; o++) {
+        mpb[o] = __builtin_fmaf(P.w_mpb[cue], chi[o], mpb[o]);
+        gc[o] = __builtin_fmaf(P.w_gpb[cue], chi[o], gc[o]);
+        if (P.cues) P.cues[(size_t(cue) * kNumOri + o) * N + p] = chi[o];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
+  __shared__ uint8_t tile[kCueTileW * kCueTileW];
+  __shared__ uint32_t hist[4 * kNbMax * kCueLanes];
+  const int lane = threadIdx.x;
+  const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = blockIdx.y * kCueTile + (lane / kCueTile);
+  const bool valid = px < P.W && py < P.H;
+  float mpb[kNumOri], gc[kNumOri];
+#pragma unroll
+  for (int o = 0; o < kNumOri; o++) mpb[o] = gc[o] = 0.f;
+  static constexpr int rbg[3] = {3, 5, 10}, rcg[3] = {5, 10, 20};
+  cue_channel<kLabBins, 8>(P, 0, 0, rbg, tile, hist, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4>(P, 1, 3, rcg, tile, hist, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4>(P, 2, 6, rcg, tile, hist, lane, px, py, valid, mpb, gc);
+  cue_channel<kTextonK, 0>(P, 3, 9, rcg, tile, hist, lane, px, py, valid, mpb, gc);
+  if (valid) {
+    const size_t N = size_t(P.W) * P.H, p = size_t(py) * P.W + px;
+#pragma unroll
+    for (int o = 0; o < kNumOri; o++) {
+      P.mpb_all[o * N + p] = mpb[o];
+      P.gcue[o * N + p] = gc[o];
+    }
+  }
+}
+
+// ---- oriented nonmax ---------------------------------------------------------
+__global__ __launch_bounds__(256) void max_ori_kernel(const float* __restrict__ mpb_all, size_t n,
+                                                      float* __restrict__ a, uint8_t* __restrict__ k) {
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float mv = -__builtin_huge_valf();
+  int mi = 0;
+#pragma unroll
+  for (int o = 0; o < kNumOri; o++) {
+    const float v = mpb_all[o * n + p];
+    if (v > mv) {
+      mv = v;
+      mi = o;
+    }
+  }
+  a[p] = mv < 0.f ? 0.f : mv;
+  k[p] = uint8_t(mi);
+}
+
+struct NmsTable {
+  int branch[kNumOri];
+  double d[kNumOri];
+  double cosd[kNumOri][kNumOri];
+};
+
+// nonmax_oriented_2D in FP64 on the FP32 inputs, then the multiscalePb clip.
+__global__ __launch_bounds__(256) void nonmax_kernel(const float* __restrict__ a, const uint8_t* __restrict__ k, int W,
+                                                     int H, NmsTable T, float* __restrict__ nmax) {
+#pragma clang fp contract(off)
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // column (lib_image y)
+  const int y = blockIdx.y;                             // row (lib_image x)
+  if (x >= W) return;
+  const size_t n = size_t(y) * W + x;
+  const int kc = k[n];
+  const int br = T.branch[kc];
+  const double d = T.d[kc];
+  bool ok0 = false, ok1 = false;
+  long i0a = 0, i0b = 0, i1a = 0, i1b = 0;
+  const long sy = W;
+  switch (br) {
+    case 0:
+      ok0 = y > 0; ok1 = y < H - 1;
+      i0a = i0b = long(n) - sy; i1a = i1b = long(n) + sy;
+      break;
+    case 1:
+      ok0 = y > 0 && x > 0; ok1 = y < H - 1 && x < W - 1;
+      i0a = long(n) - sy; i0b = i0a - 1; i1a = long(n) + sy; i1b = i1a + 1;
+      break;
+    case 2:
+      ok0 = y > 0 && x > 0; ok1 = y < H - 1 && x < W - 1;
+      i0a = long(n) - 1; i0b = i0a - sy; i1a = long(n) + 1; i1b = i1a + sy;
+      break;
+    case 3:
+      ok0 = x > 0; ok1 = x < W - 1;
+      i0a = i0b = long(n) - 1; i1a = i1b = long(n) + 1;
+      break;
+    case 4:
+      ok0 = y < H - 1 && x > 0; ok1 = y > 0 && x < W - 1;
+      i0a = long(n) - 1; i0b = i0a + sy; i1a = long(n) + 1; i1b = i1a - sy;
+      break;
+    default:
+      ok0 = y < H - 1 && x > 0; ok1 = y > 0 && x < W - 1;
+      i0a = long(n) + sy; i0b = i0a - 1; i1a = long(n) - sy; i1b = i1a + 1;
+      break;
+  }
+  double res = 0;
+  if (ok0 && ok1) {
+    const double v = a[n];
+    const double v0 = (1.0 - d) * double(a[i0a]) * T.cosd[kc][k[i0a]] + d * double(a[i0b]) * T.cosd[kc][k[i0b]];
+    const double v1 = (1.0 - d) * double(a[i1a]) * T.cosd[kc][k[i1a]] + d * double(a[i1b]) * T.cosd[kc][k[i1b]];
+    if (v > v0 && v > v1) res = v;
+  }
+  if (1.2 * res > 1) res = 1;
+  if (res < 0) res = 0;
+  nmax[n] = float(res);
+}
+
+// ---- host side -------------------------------------------------------------------
+namespace {
+struct CueTables {
+  DevBuf<int> off;
+  DevBuf<uint8_t> slice;
+  DevBuf<double> gamma;
+  int end_by_r[21];
+  float smooth[3][9];
+  NmsTable nms;
+};
+std::map<int, std::unique_ptr<CueTables>> g_cue_tables;  // per device
+std::mutex g_cue_mu;
+}  // namespace
+
+static CueTables& cue_tables(Context& c) {
+  std::lock_guard<std::mutex> lk(g_cue_mu);
+  auto& slot = g_cue_tables[c.device];
+  if (slot) return *slot;
+  auto t = std::make_unique<CueTables>();
+  std::vector<std::array<int, 3>> offs;  // d2, dy, dx
+  for (int dy = -kCueRmax; dy <= kCueRmax; dy++)
+    for (int dx = -kCueRmax; dx <= kCueRmax; dx++)
+      if (dx * dx + dy * dy <= kCueRmax * kCueRmax) offs.push_back({dx * dx + dy * dy, dy, dx});
+  std::stable_sort(offs.begin(), offs.end(), [](const auto& a, const auto& b) { return a[0] < b[0]; });
+  std::vector<int> off(offs.size());
+  std::vector<uint8_t> sl(offs.size());
+  for (size_t i = 0; i < offs.size(); i++) {
+    off[i] = offs[i][1] * kCueTileW + offs[i][2];
+    sl[i] = uint8_t(slice_of_offset(offs[i][1], offs[i][2]));
+  }
+  for (int r = 0; r <= kCueRmax; r++) {
+    int e = 0;
+    while (e < int(offs.size()) && offs[size_t(e)][0] <= r * r) e++;
+    t->end_by_r[r] = e;
+  }
+  t->off.alloc(off.size());
+  t->slice.alloc(sl.size());
+  GPB_HIP(hipMemcpy(t->off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  GPB_HIP(hipMemcpy(t->slice.p, sl.data(), sl.size(), hipMemcpyHostToDevice));
+  std::vector<double> gam(256);
+  for (int i = 0; i < 256; i++) gam[size_t(i)] = std::pow(double(i) / 255.0, 2.5);
+  t->gamma.alloc(256);
+  GPB_HIP(hipMemcpy(t->gamma.p, gam.data(), 256 * sizeof(double), hipMemcpyHostToDevice));
+  const std::vector<double> gbg = gaussian_1d(0.1 * kLabBins), gcg = gaussian_1d(0.05 * kLabBins);
+  if (gbg.size() != 17 || gcg.size() != 9) throw std::logic_error("unexpected smoothing kernel size");
+  for (int k = 0; k < 9; k++) {
+    t->smooth[0][k] = float(gbg[size_t(8 + k)]);
+    t->smooth[1][k] = k < 5 ? float(gcg[size_t(4 + k)]) : 0.f;
+    t->smooth[2][k] = k == 0 ? 1.f : 0.f;
+  }
+  const NonmaxTable& nt = nonmax_table();
+  for (int k = 0; k < kNumOri; k++) {
+    t->nms.branch[k] = nt.branch[k];
+    t->nms.d[k] = nt.d[k];
+    for (int j = 0; j < kNumOri; j++) t->nms.cosd[k][j] = nt.cosdiff[k][j];
+  }
+  slot = std::move(t);
+  return *slot;
+}
+
+void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, float* d_mpb_all,
+                       float* d_gcue, float* d_cues) {
+  CueTables& t = cue_tables(c);
+  const size_t N = size_t(W) * H;
+  uint8_t* q = c.ws<uint8_t>("cue.q", 3 * N);
+  int* nb = c.ws<int>("cue.nb", 4);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, c.stream, nb, 4, 0);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lab_quantize_kernel, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_rgb, N, t.gamma.p, q, q + N,
+                     q + 2 * N, nb);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(label_max_kernel, dim3(std::min<unsigned>(1024, div_up(N, 256))), dim3(256), 0, c.stream, d_text, N,
+                     nb + 3);
+  GPB_LAUNCH_CHECK();
+  CueParams P{};
+  P.plane[0] = q;
+  P.plane[1] = q + N;
+  P.plane[2] = q + 2 * N;
+  P.plane[3] = d_text;
+  P.nbins = nb;
+  P.off = t.off.p;
+  P.slice = t.slice.p;
+  for (int r = 0; r <= kCueRmax; r++) P.end_by_r[r] = t.end_by_r[r];
+  for (int a = 0; a < 3; a++)
+    for (int k = 0; k < 9; k++) P.smooth[a][k] = t.smooth[a][k];
+  for (int i = 0; i < kNumCues; i++) {
+    P.w_mpb[i] = float(kMPbWeights[i]);
+    P.w_gpb[i] = float(kGPbWeights[i]);
+  }
+  P.mpb_all = d_mpb_all;
+  P.gcue = d_gcue;
+  P.cues = d_cues;
+  P.W = W;
+  P.H = H;
+  hipLaunchKernelGGL(cues_kernel, dim3(div_up(W, kCueTile), div_up(H, kCueTile)), dim3(kCueLanes), 0, c.stream, P);
+  GPB_LAUNCH_CHECK();
+}
+
+void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d_q /* 3N */, int* d_nb /* 3 */) {
+  CueTables& t = cue_tables(c);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, c.stream, d_nb, 3, 0);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lab_quantize_kernel, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_rgb, N, t.gamma.p, d_q,
+                     d_q + N, d_q + 2 * N, d_nb);
+  GPB_LAUNCH_CHECK();
+}
+
+void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nmax) {
+  CueTables& t = cue_tables(c);
+  const size_t N = size_t(W) * H;
+  float* a = c.ws<float>("nms.a", N);
+  uint8_t* k = c.ws<uint8_t>("nms.k", N);
+  hipLaunchKernelGGL(max_ori_kernel, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_mpb_all, N, a, k);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(nonmax_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, a, k, W, H, t.nms, d_nmax);
+  GPB_LAUNCH_CHECK();
+}
+
+}  // namespace gpb

@@ -43,6 +43,9 @@ _sig("gpb_pack_rgb", None, [_u8p, C.c_size_t, _u32p])
 _sig("gpb_synth_voronoi", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _u8p, C.c_void_p])
 _sig("gpb_make_codebook", C.c_int, [_u32p, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, _f32p])
 _sig("gpb_texton", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, _u8p])
+_sig("gpb_lab_quantize", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, _i32p])
+_sig("gpb_local_cues", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, C.c_void_p, _f32p])
+_sig("gpb_nonmax", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p])
 
 
 class GpbError(RuntimeError):
@@ -123,4 +126,28 @@ class Context:
         out = np.empty((H, W), np.uint8)
         self._chk(lib.gpb_texton(self.h, W, H, packed.reshape(-1),
                                  np.ascontiguousarray(codebook, np.float32).reshape(-1), out.reshape(-1)))
+        return out
+
+    def lab_quantize(self, packed: np.ndarray):
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        bins = np.empty((3, H, W), np.uint8)
+        nb = np.empty(3, np.int32)
+        self._chk(lib.gpb_lab_quantize(self.h, W, H, packed.reshape(-1), bins.reshape(-1), nb))
+        return bins, nb
+
+    def local_cues(self, packed: np.ndarray, textons: np.ndarray, with_cues: bool = True):
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        cues = np.empty((12, 8, H, W), np.float32) if with_cues else None
+        mpb = np.empty((8, H, W), np.float32)
+        self._chk(lib.gpb_local_cues(self.h, W, H, packed.reshape(-1), np.ascontiguousarray(textons, np.uint8).reshape(-1),
+                                     cues.ctypes.data if cues is not None else None, mpb.reshape(-1)))
+        return cues, mpb
+
+    def nonmax(self, mpb_all: np.ndarray) -> np.ndarray:
+        m = np.ascontiguousarray(mpb_all, np.float32)
+        _, H, W = m.shape
+        out = np.empty((H, W), np.float32)
+        self._chk(lib.gpb_nonmax(self.h, W, H, m.reshape(-1), out.reshape(-1)))
         return out
