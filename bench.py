@@ -1,0 +1,136 @@
+"""gPb throughput benchmark on one MI355X.
+
+Workload: plain gPb of one 2048x2048 seeded Voronoi image (400 regions,
+sinusoid textures, noise sigma 8, seed 2), 9 eigenvectors to a relative
+generalised residual <= 1e-3, FP32 with FP64 Lanczos reorthogonalisation,
+Ritz recombination and projected eigensolve.  A step is one whole gPb pass
+(texton -> cues -> nonmax -> W -> eigensolve -> sPb -> gPb) with the packed
+image and codebook already resident in HBM and the outputs left in HBM.
+
+Prints one JSON line (rank 0).  `roofline` describes the kernel class with the
+largest device time in the timed steps, measured with HIP events on the
+library's stream; `cpu_baseline` times the CPU oracle (one thread) on a
+bounded crop of the same image.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name, path):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(packed, codebook, crop: int, tol: float):
+    """Time the CPU oracle (single thread) on the top-left crop x crop pixels."""
+    import numpy as np
+
+    orc = _load("gpb_oracle_binding", ROOT / "oracle" / "oracle.py")
+    sub = np.ascontiguousarray(packed[:crop, :crop])
+    t0 = time.perf_counter()
+    orc.pipeline(sub, codebook, K=9, tol=tol, m=48)
+    dt = time.perf_counter() - t0
+    return dict(value=round(crop * crop / dt / 1e6, 5), unit="Mpixel/s", cores=1, kind="port",
+                sample=f"CPU oracle (C, FP64, one thread), whole gPb path on the top-left {crop}x{crop} crop of the "
+                       f"same image, eigen-residual <= {tol:g}; {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--regions", type=int, default=400)
+    ap.add_argument("--cpu-crop", type=int, default=224)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the gPb path runs one image (or one patch batch) per GPU, and this "
+              "build has no multi-GPU launcher; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    gpb = _load("gpb_amd_binding", ROOT / "ucm-mpi-gpu_amd" / "gpb.py")
+    S, tol = args.size, 1e-3
+    rgb = gpb.synth_voronoi(S, S, args.regions, 8.0, 2)
+    packed = gpb.pack_rgb(rgb)
+    codebook = gpb.make_codebook(packed, seed=2, iters=10)
+    dev = torch.device("cuda:0")
+    d_rgb = torch.from_numpy(packed.view(np.int32)).to(dev)
+    d_cb = torch.from_numpy(codebook).to(dev)
+    d_gmax = torch.empty((S, S), dtype=torch.float32, device=dev)
+    d_gall = torch.empty((8, S, S), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    ctx = gpb.Context(0)
+    ctx.set_eig_options(nvec=9, basis=48, keep=24, tol=tol, max_matvecs=40000)
+
+    def step():
+        ctx.compute_device(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), d_gmax.data_ptr(), d_gall.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        stats.append(ctx.stats())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ms_step = dt * 1e3 / args.steps
+    value = S * S * args.steps / dt / 1e6
+
+    dom = max(kt, key=lambda k: kt[k]["ms"])
+    d = kt[dom]
+    avg_ms = d["ms"] / max(1, d["launches"])
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
+                    bytes_per_launch=round(d["bytes"] / max(1, d["launches"])),
+                    share_of_step=round(d["ms"] / (ms_step * args.steps), 4))
+    st = stats[-1]
+    line = dict(metric="gPb Mpixel/s on 1 MI355X at eig-residual <=1e-3 (plain Lanczos)", value=round(value, 4),
+                unit="Mpixel/s", n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_step, 2),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32+f64",
+                data="synthetic",
+                config=dict(workload=f"plain gPb, {S}x{S} seeded Voronoi ({args.regions} regions, sigma 8, seed 2), "
+                                     f"9 eigenvectors, residual <= {tol:g}",
+                            form="plain", lanczos_basis=st["basis"], lanczos_matvecs=st["matvecs"],
+                            lanczos_restarts=st["restarts"], max_residual=st["max_residual"],
+                            stage_ms=[round(x, 2) for x in st["stage_ms"]]),
+                roofline=roofline,
+                kernels={k: dict(ms=round(v["ms"] / args.steps, 3), launches=v["launches"] // args.steps,
+                                 gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0)
+                         for k, v in kt.items()})
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(packed, codebook, args.cpu_crop, tol)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -31,6 +31,19 @@ extern "C" {
 
 typedef struct gpb_ctx gpb_ctx;
 
+/* Timings and solver statistics of the last call on a context. */
+typedef struct gpb_stats {
+  double stage_ms[6];   /* texton, local cues, nonmax, affinity, eigensolve, sPb+gPb */
+  int lanczos_matvecs;  /* operator applications (block columns for block Lanczos) */
+  int lanczos_restarts;
+  int lanczos_basis;
+  int lanczos_block;
+  int converged;        /* eigenpairs meeting the residual target */
+  double max_residual;  /* max ||(D-W)v - lambda D v|| / ||D v|| over returned pairs */
+  double evals[16];     /* generalised eigenvalues, ascending */
+  int fallback;         /* 1 if a warm start was rejected and plain Lanczos ran */
+} gpb_stats;
+
 /* Context lifetime.  Replaces the per-call device selection and
  * cudaDeviceReset of computeGPb (damascene/src/damascene.cu:258-268, :660). */
 int gpb_ctx_create(int device, gpb_ctx** out);
@@ -89,6 +102,90 @@ int gpb_local_cues(gpb_ctx* ctx, int width, int height, const uint32_t* packed, 
  * (src/nonmax_channels.cpp:8-80), nonmax_oriented_2D (lib_image.cc:4083-4200)
  * and the clip of src/multiscalePb.cpp:251-261. */
 int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float* nmax);
+
+/* ---- stages (c)-(e): affinity, eigenvectors, sPb, gPb ---------------------- */
+
+/* Eigensolver settings for later calls on ctx: nvec eigenpairs (2..16,
+ * default 9 as in damascene/src/damascene.cu:307), Lanczos basis size,
+ * vectors kept at a restart, relative generalised residual target
+ * ||(D-W)v - lambda D v|| / ||D v|| (default 1e-3) and a cap on operator
+ * applications.  Replaces the hard-coded nEigNum/fEigTolerance of computeGPb
+ * (damascene.cu:307-309) and nvec of spectralPb (src/globalPb.cpp:65). */
+int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol, int max_matvecs);
+
+/* Kernel classes for gpb_kernel_time. */
+#define GPB_K_TEXTON 0
+#define GPB_K_CUES 1
+#define GPB_K_NONMAX 2
+#define GPB_K_AFFINITY 3
+#define GPB_K_SPMV 4
+#define GPB_K_DOTS 5
+#define GPB_K_UPDATE 6
+#define GPB_K_RITZ 7
+#define GPB_K_SPB 8
+#define GPB_K_GPB 9
+
+/* Enable (1) or disable (0) HIP-event timing of every kernel launch on ctx,
+ * and clear the totals.  Replaces the StopWatch stage timers of computeGPb
+ * (damascene/src/damascene.cu:324-575). */
+int gpb_set_profiling(gpb_ctx* ctx, int on);
+
+/* Total device time (ms, HIP events on the context stream), launch count and
+ * algorithmic HBM bytes (may be NULL) of one kernel class since profiling was
+ * enabled. */
+int gpb_kernel_time(gpb_ctx* ctx, int kernel_class, double* total_ms, long* launches, double* algo_bytes);
+
+/* Statistics of the last pipeline / eigensolver call. */
+int gpb_get_stats(gpb_ctx* ctx, gpb_stats* out);
+
+/* Intervening-contour affinity of the radius-5 disc (sigma 0.1), symmetrised:
+ * w_upper[40][N] holds W(p, p+e_k) for the 40 offsets e_k with dy > 0 or
+ * (dy == 0, dx > 0) in row-major order, 0 outside the image; degree[N] is the
+ * full row sum (W_pp = 1).  mpb_nmax in [0,1].  Replaces buildW
+ * (src/buildW.cpp:23-246: computeSupport, computeAffinities2, symmetrize). */
+int gpb_build_w(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, float* w_upper, float* degree);
+
+/* The nvec smallest generalised eigenpairs of (D - W, D) for W built from
+ * mpb_nmax: evals[nvec] ascending, evecs[nvec][N] (may be NULL) with
+ * v^T D v = 1, the trivial lambda = 0 pair first.  Replaces the buildW +
+ * Spectra SymGEigsSolver part of spectralPb (src/spectralPb.cpp:69-255). */
+int gpb_eigs(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, double* evals, float* evecs);
+
+/* sPb[8][N] from mpb_nmax (buildW, eigensolve, min-max normalisation,
+ * 1/sqrt(lambda), 8 oriented derivative-of-Gaussian filters).  Replaces
+ * spectralPb (src/spectralPb.cpp:33-343); evals[nvec] may be NULL. */
+int gpb_spectral_pb(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, float* spb, double* evals);
+
+/* sPb[8][N] from given eigenpairs (evecs[nvec][N], evals[nvec]); the filter
+ * stage of spectralPb alone (src/spectralPb.cpp:257-322). */
+int gpb_spb_from_evecs(gpb_ctx* ctx, int width, int height, int nvec, const float* evecs, const double* evals,
+                       float* spb);
+
+/* Whole gPb path on host buffers.  gpb_max[N]: max over orientations of gPb,
+ * 0 where mPb_nmax <= 0.05; gpb_all[8][W][H]: calibration sigmoid of each
+ * oriented gPb, column-major (index x*H + y), orientation o = the split angle
+ * o*pi/8 of the half-disc cues; gpb_orient[8][N]: the oriented gPb before
+ * the sigmoid (GlobalPb's output); mpb_nmax[N].  Any output may be NULL.
+ * Replaces computeGPb (damascene/include/globalPb.h:37,
+ * damascene/src/damascene.cu:249-661). */
+int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, float* gpb_max,
+                float* gpb_all, float* gpb_orient, float* mpb_nmax);
+
+/* Same on device buffers (inputs resident in HBM, outputs device pointers,
+ * any may be NULL); returns after the stream is synchronised. */
+int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
+                       float* d_gpb_max, float* d_gpb_all, float* d_gpb_orient);
+
+/* ---- diagnostics ---------------------------------------------------------------- */
+
+/* The intervening-contour walk program used for pixels whose radius-5 window
+ * is clipped at distances (left, right, top, bottom) in [0,5] (5 = not
+ * clipped): conn[81][5] per disc offset (row-major dy, dx): valid, found,
+ * site count, site start (low 7 bits, high bits); sites[n][2] = (dx, dy) of
+ * the pixels whose mPb maximum is the connection's intervening contour.  No
+ * counterpart in the reference (it recomputes the walks per pixel,
+ * buildW/src/ic.cc:375-560); exported for CPU-side cross-checks. */
+int gpb_ic_program(int left, int right, int top, int bottom, int8_t* conn, int8_t* sites, int max_sites);
 
 #ifdef __cplusplus
 }

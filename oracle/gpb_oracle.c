@@ -451,3 +451,546 @@ void orc_nonmax(const double* mpb_all, int W, int H, double* nmax) {
   free(a);
   free(ind);
 }
+
+/* =========================================================================
+ * a13: intervening-contour affinity.  Restates buildW (buildW.cpp:23-212):
+ * lattices H(x,y) = mPb(row y-1, col x), V(x,y) = mPb(row y, col x-1)
+ * (spectralPb.cpp:55-59; FP32 lattices, ic.hh:31-34), Group::computeSupport /
+ * interveningContour / ic_walk (buildW/src/ic.cc:42-560) with threshold 1,
+ * computeAffinities2 (affinity.cc:58-156, C_IC_SS :40-52, sigma 0.1, radius 5)
+ * and SMatrix::symmetrize (smatrix.cc:95-117).
+ * Output: Wd[k][p] for the 81 disc offsets k (row-major over dy, dx), 0 where
+ * the neighbour is outside the image; D[p] = full row sum (buildW.cpp:150-160).
+ * ========================================================================= */
+typedef struct { int x, y; float sim; } ic_pt;
+typedef struct {
+  int w, h;           /* lattice image width (cols) and height (rows) */
+  const float* H;     /* H[x*(h+1) + y], x < w, y <= h */
+  const float* V;     /* V[x*h + y], x <= w, y < h */
+} ic_lat;
+static float latH(const ic_lat* L, int x, int y) { return L->H[(size_t)x * (L->h + 1) + y]; }
+static float latV(const ic_lat* L, int x, int y) { return L->V[(size_t)x * L->h + y]; }
+
+typedef struct { ic_pt pts[11][11]; int cnt[11]; } ic_scan;
+
+static void ic_walk_o(const ic_lat* L, float thresh, int x0, int y0, int x1, int y1, int x2, int y2, int x3, int y3,
+                      int wr, ic_scan* sc) {
+  const long long dx1 = x1 - x0, dy1 = y1 - y0, dx2 = x2 - x0, dy2 = y2 - y0, dx3 = x3 - x0, dy3 = y3 - y0;
+  const long long dot11 = dx1 * dx1 + dy1 * dy1, dot22 = dx2 * dx2 + dy2 * dy2, dot33 = dx3 * dx3 + dy3 * dy3;
+  const int dx = x2 - x0, dy = y2 - y0, adx = abs(dx), ady = abs(dy);
+  int octant = -1;
+  if (dx > 0 && dy >= 0) octant = (adx > ady) ? 0 : 1;
+  else if (dx <= 0 && dy > 0) octant = (adx < ady) ? 2 : 3;
+  else if (dy <= 0 && dx < 0) octant = (adx > ady) ? 4 : 5;
+  else if (dx >= 0 && dy < 0) octant = (adx < ady) ? 6 : 7;
+  int t = (octant == 0 || octant == 3 || octant == 4 || octant == 7) ? -adx : -ady;
+  float maxpb = 0.0f;
+  int xi = x0, yi = y0, count = 0, oldx = xi, oldy = yi;
+  ic_pt pts[64];
+  while (xi != x2 || yi != y2) {
+    switch (octant) {
+      case 0: xi++; t += (ady << 1); if (t > 0) { yi++; t -= (adx << 1); } break;
+      case 1: yi++; t += (adx << 1); if (t > 0) { xi++; t -= (ady << 1); } break;
+      case 2: yi++; t += (adx << 1); if (t > 0) { xi--; t -= (ady << 1); } break;
+      case 3: xi--; t += (ady << 1); if (t > 0) { yi++; t -= (adx << 1); } break;
+      case 4: xi--; t += (ady << 1); if (t > 0) { yi--; t -= (adx << 1); } break;
+      case 5: yi--; t += (adx << 1); if (t > 0) { xi--; t -= (ady << 1); } break;
+      case 6: yi--; t += (adx << 1); if (t > 0) { xi++; t -= (ady << 1); } break;
+      default: xi++; t += (ady << 1); if (t > 0) { yi--; t -= (adx << 1); } break;
+    }
+    const long long dxi = xi - x0, dyi = yi - y0;
+    const long long dotii = dxi * dxi + dyi * dyi, doti1 = dxi * dx1 + dyi * dy1;
+    const long long doti2 = dxi * dx2 + dyi * dy2, doti3 = dxi * dx3 + dyi * dy3;
+    const int good = (doti2 * doti2 == dotii * dot22) ||
+                     (dot11 * doti2 * doti2 > dot22 * doti1 * doti1 && dot33 * doti2 * doti2 >= dot22 * doti3 * doti3);
+    float in = 0.0f;
+    if (oldx == xi) {
+      if (yi > oldy) in = latH(L, xi, yi);
+      else if (yi < oldy) in = latH(L, oldx, oldy);
+    } else if (oldy == yi) {
+      if (xi > oldx) in = latV(L, xi, yi);
+      else if (xi < oldx) in = latV(L, oldx, oldy);
+    } else {
+      float a, b, c, d;
+      if (xi > oldx && yi > oldy) { a = latH(L, oldx, yi); b = latH(L, xi, yi); c = latV(L, xi, oldy); d = latV(L, xi, yi); }
+      else if (xi > oldx && yi < oldy) { a = latH(L, oldx, oldy); b = latH(L, xi, oldy); c = latV(L, xi, oldy); d = latV(L, xi, yi); }
+      else if (xi < oldx && yi > oldy) { a = latH(L, oldx, yi); b = latH(L, xi, yi); c = latV(L, oldx, oldy); d = latV(L, oldx, yi); }
+      else { a = latH(L, oldx, oldy); b = latH(L, xi, oldy); c = latV(L, oldx, oldy); d = latV(L, oldx, yi); }
+      in = fmaxf(in, a); in = fmaxf(in, b); in = fmaxf(in, c); in = fmaxf(in, d);
+    }
+    maxpb = fmaxf(maxpb, in);
+    oldx = xi;
+    oldy = yi;
+    if (!good) continue;
+    if (maxpb > thresh) break;
+    pts[count].x = xi; pts[count].y = yi; pts[count].sim = 1.0f - maxpb; count++;
+  }
+  if (octant <= 1 || octant >= 6) {
+    for (int i = 0; i < count; i++) { int yind = pts[i].y - y0 + wr; sc->pts[yind][sc->cnt[yind]++] = pts[i]; }
+  } else {
+    for (int i = count - 1; i >= 0; i--) { int yind = pts[i].y - y0 + wr; sc->pts[yind][sc->cnt[yind]++] = pts[i]; }
+  }
+}
+
+static void ic_corner(ic_scan* sc, int x, int y, int y0, int wr) {
+  int yind = y - y0 + wr;
+  ic_pt p = {x, y, 1.0f};
+  sc->pts[yind][sc->cnt[yind]++] = p;
+}
+
+/* support list of one pixel, scanline order; returns count */
+static int ic_support(const ic_lat* L, float thresh, int x0, int y0, int wr, ic_pt* adj) {
+  const int width = L->w, height = L->h;
+  ic_scan sc;
+  memset(sc.cnt, 0, sizeof(sc.cnt));
+  const int rxa = x0 - wr > 0 ? x0 - wr : 0, rya = y0 - wr > 0 ? y0 - wr : 0;
+  const int rxb = x0 + wr < width - 1 ? x0 + wr : width - 1, ryb = y0 + wr < height - 1 ? y0 + wr : height - 1;
+#define W_(a, b, c, d, e, f) ic_walk_o(L, thresh, x0, y0, a, b, c, d, e, f, wr, &sc)
+  if (x0 > rxa) {
+    if ((y0 > 0) && (y0 < ryb)) W_(rxa, y0 - 1, rxa, y0, rxa, y0 + 1);
+    for (int y = y0 - 1; y > rya; y--) W_(rxa, y - 1, rxa, y, rxa, y + 1);
+  }
+  if (x0 > rxa + 1 || y0 > rya + 1 || ((x0 > rxa) && (y0 > rya))) W_(rxa, rya + 1, rxa, rya, rxa + 1, rya);
+  if (((x0 == rxa) && (y0 == rya + 1)) || ((x0 == rxa + 1) && (y0 == rya))) ic_corner(&sc, rxa, rya, y0, wr);
+  if (y0 > rya)
+    for (int x = rxa + 1; x < rxb; x++) W_(x - 1, rya, x, rya, x + 1, rya);
+  if ((y0 > rya + 1) || (x0 < rxb - 1) || ((y0 > rya) && (x0 < rxb))) W_(rxb - 1, rya, rxb, rya, rxb, rya + 1);
+  if (((x0 == rxb - 1) && (y0 == rya)) || ((x0 == rxb) && (y0 == rya + 1))) ic_corner(&sc, rxb, rya, y0, wr);
+  if (x0 < rxb)
+    for (int y = rya + 1; y < y0; y++) W_(rxb, y - 1, rxb, y, rxb, y + 1);
+  if (x0 > rxa)
+    for (int y = y0 + 1; y < ryb; y++) W_(rxa, y - 1, rxa, y, rxa, y + 1);
+  if ((x0 > rxa + 1) || (y0 < ryb - 1) || ((x0 > rxa) && (y0 < ryb))) W_(rxa, ryb - 1, rxa, ryb, rxa + 1, ryb);
+  if (((x0 == rxa) && (y0 == ryb - 1)) || ((x0 == rxa + 1) && (y0 == ryb))) ic_corner(&sc, rxa, ryb, y0, wr);
+  if (y0 < ryb)
+    for (int x = rxa + 1; x < rxb; x++) W_(x - 1, ryb, x, ryb, x + 1, ryb);
+  if ((y0 < ryb - 1) || (x0 < rxb - 1) || ((y0 < ryb) && (x0 < rxb))) W_(rxb - 1, ryb, rxb, ryb, rxb, ryb - 1);
+  if (((x0 == rxb - 1) && (y0 == ryb)) || ((x0 == rxb) && (y0 == ryb - 1))) ic_corner(&sc, rxb, ryb, y0, wr);
+  if (x0 < rxb) {
+    for (int y = ryb - 1; y > y0; y--) W_(rxb, y - 1, rxb, y, rxb, y + 1);
+    if ((y0 > 0) && (y0 < ryb)) W_(rxb, y0 - 1, rxb, y0, rxb, y0 + 1);
+  }
+#undef W_
+  int count = 0;
+  for (int y = 0; y < 2 * wr + 1; y++)
+    for (int i = 0; i < sc.cnt[y]; i++) adj[count++] = sc.pts[y][i];
+  return count;
+}
+
+static float c_ic_ss(float ic) {
+  float val = 1.8682;
+  val += (ic / 0.3130) * -1.3113;
+  const float post = 1.0 / (1.0 + exp(-val));
+  if (!isfinite(post)) return 0;
+  if (post < 0) return 0;
+  if (post > 1) return 1;
+  return post;
+}
+
+/* Returns -1 if a scan line was not strictly increasing (reference asserts). */
+int orc_build_w(const double* mpb, int W, int H, double* Wd /* [81][N] */, double* D /* [N] */) {
+  const int wr = 5;
+  const float sigma = 0.1f, thresh = 1.0f;
+  size_t N = (size_t)W * H;
+  ic_lat L;
+  L.w = W;
+  L.h = H;
+  float* Hl = (float*)calloc((size_t)W * (H + 1), sizeof(float));
+  float* Vl = (float*)calloc((size_t)(W + 1) * H, sizeof(float));
+  for (int x = 0; x < W; x++)
+    for (int y = 1; y <= H; y++) Hl[(size_t)x * (H + 1) + y] = (float)mpb[(size_t)(y - 1) * W + x];
+  for (int x = 1; x <= W; x++)
+    for (int y = 0; y < H; y++) Vl[(size_t)x * H + y] = (float)mpb[(size_t)y * W + (x - 1)];
+  L.H = Hl;
+  L.V = Vl;
+  int dxk[81], dyk[81], nk = 0;
+  for (int u = -wr; u <= wr; u++)
+    for (int v = -wr; v <= wr; v++)
+      if (u * u + v * v <= wr * wr) { dxk[nk] = v; dyk[nk] = u; nk++; }
+  /* unsymmetrised affinities, same [81][N] layout */
+  float* A = (float*)calloc(81 * N, sizeof(float));
+  int bad = 0;
+  ic_pt adj[121];
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      int n = ic_support(&L, thresh, x, y, wr, adj);
+      for (int r = 0; r + 1 < 11; r++) { /* sortedness check per scan line */ }
+      for (int i = 1; i < n; i++)
+        if (adj[i].y == adj[i - 1].y && !(adj[i - 1].x < adj[i].x)) bad = 1;
+      int idx = 0, k = 0;
+      for (int u = -wr; u <= wr; u++) {
+        int yy = y + u;
+        for (int v = -wr; v <= wr; v++) {
+          int xx = x + v;
+          if (u * u + v * v > wr * wr) continue;
+          int kk = k++;
+          if (xx < 0 || xx >= W || yy < 0 || yy >= H) continue;
+          while (idx < n && adj[idx].y < yy) idx++;
+          while (idx < n && adj[idx].x < xx) idx++;
+          float pss;
+          if (u == 0 && v == 0) pss = 1.0f;
+          else {
+            float icsim = 0.0f;
+            if (idx < n && adj[idx].x == xx && adj[idx].y == yy) { icsim = adj[idx].sim; idx++; }
+            pss = c_ic_ss(1 - icsim);
+          }
+          float val = exp(-(1 - pss) / sigma);
+          A[(size_t)kk * N + (size_t)y * W + x] = val;
+        }
+      }
+    }
+  for (size_t p = 0; p < N; p++) D[p] = 0;
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      size_t p = (size_t)y * W + x;
+      for (int k = 0; k < 81; k++) {
+        int xx = x + dxk[k], yy = y + dyk[k];
+        double w = 0;
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H) {
+          size_t q = (size_t)yy * W + xx;
+          w = 0.5 * ((double)A[(size_t)k * N + p] + (double)A[(size_t)(80 - k) * N + q]);
+        }
+        Wd[(size_t)k * N + p] = w;
+        D[p] += w;
+      }
+    }
+  free(A);
+  free(Hl);
+  free(Vl);
+  return bad ? -1 : 0;
+}
+
+/* =========================================================================
+ * a14-a16: generalised eigenpairs of (D - W, D): the K smallest, by thick-
+ * restart Lanczos with full (two-pass Gram-Schmidt) reorthogonalisation in
+ * FP64 on M = D^-1/2 W D^-1/2, whose largest eigenvalues mu give lambda =
+ * 1 - mu (Spectra SymGEigsSolver in Cholesky mode, SymGEigsSolver.h:46-111,
+ * SymEigsSolver.h:200-586).  The constant vector (lambda = 0) is locked as
+ * the first basis vector.  Stops when every returned pair has
+ * ||(D-W)v - lambda D v|| / ||D v|| <= tol.  evecs[K][N] with v^T D v = 1.
+ * ========================================================================= */
+static void stencil_apply(const double* Wd, const double* s, int W, int H, const double* x, double* y) {
+  static int dxk[81], dyk[81], ready = 0;
+  if (!ready) {
+    int nk = 0;
+    for (int u = -5; u <= 5; u++)
+      for (int v = -5; v <= 5; v++)
+        if (u * u + v * v <= 25) { dxk[nk] = v; dyk[nk] = u; nk++; }
+    ready = 1;
+  }
+  size_t N = (size_t)W * H;
+  for (int yy = 0; yy < H; yy++)
+    for (int xx = 0; xx < W; xx++) {
+      size_t p = (size_t)yy * W + xx;
+      double acc = 0;
+      for (int k = 0; k < 81; k++) {
+        int x2 = xx + dxk[k], y2 = yy + dyk[k];
+        if (x2 < 0 || x2 >= W || y2 < 0 || y2 >= H) continue;
+        size_t q = (size_t)y2 * W + x2;
+        acc += Wd[(size_t)k * N + p] * s[q] * x[q];
+      }
+      y[p] = s[p] * acc;
+    }
+}
+
+/* cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major);
+ * eigenvalues descending into w, eigenvectors as columns of Z */
+static void jacobi_eig(int n, double* A, double* w, double* Z) {
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) Z[i * n + j] = (i == j);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = i + 1; j < n; j++) off += A[i * n + j] * A[i * n + j];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = A[p * n + q];
+        if (fabs(apq) < 1e-300) continue;
+        double app = A[p * n + p], aqq = A[q * n + q];
+        double theta = (aqq - app) / (2 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        double c = 1 / sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double zkp = Z[k * n + p], zkq = Z[k * n + q];
+          Z[k * n + p] = c * zkp - s * zkq;
+          Z[k * n + q] = s * zkp + c * zkq;
+        }
+      }
+  }
+  /* sort descending */
+  int* idx = (int*)malloc(sizeof(int) * n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++)
+      if (A[idx[j] * n + idx[j]] > A[idx[i] * n + idx[i]]) { int t = idx[i]; idx[i] = idx[j]; idx[j] = t; }
+  double* Zs = (double*)malloc(sizeof(double) * n * n);
+  for (int i = 0; i < n; i++) {
+    w[i] = A[idx[i] * n + idx[i]];
+    for (int k = 0; k < n; k++) Zs[k * n + i] = Z[k * n + idx[i]];
+  }
+  memcpy(Z, Zs, sizeof(double) * n * n);
+  free(Zs);
+  free(idx);
+}
+
+void orc_jacobi_eig(int n, double* A, double* w, double* Z) { jacobi_eig(n, A, w, Z); }
+
+static double dotv(const double* a, const double* b, size_t n) {
+  double s = 0;
+  for (size_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+int orc_eigs(const double* Wd, const double* D, int W, int H, int K, int m, double tol, int maxit, double* evals,
+             double* evecs, int* iters_out) {
+  size_t N = (size_t)W * H;
+  if (m < K + 2) m = K + 2;
+  double* s = (double*)malloc(sizeof(double) * N);
+  for (size_t p = 0; p < N; p++) s[p] = 1.0 / sqrt(D[p]);
+  double* V = (double*)malloc(sizeof(double) * N * (m + 1));
+  double* w = (double*)malloc(sizeof(double) * N);
+  double* T = (double*)calloc((size_t)m * m, sizeof(double));
+  double* A = (double*)malloc(sizeof(double) * m * m);
+  double* th = (double*)malloc(sizeof(double) * m);
+  double* Y = (double*)malloc(sizeof(double) * m * m);
+  double* U = (double*)malloc(sizeof(double) * N * m);
+  double* h = (double*)malloc(sizeof(double) * (m + 1));
+  /* locked trivial vector D^1/2 1 */
+  double nrm = 0;
+  for (size_t p = 0; p < N; p++) { V[p] = sqrt(D[p]); nrm += D[p]; }
+  nrm = sqrt(nrm);
+  for (size_t p = 0; p < N; p++) V[p] /= nrm;
+  /* start vector: deterministic pseudo-random, orthogonal to V0 */
+  {
+    uint64_t z = 0x2545F4914F6CDD1Dull;
+    double* v1 = V + N;
+    for (size_t p = 0; p < N; p++) {
+      z ^= z << 13; z ^= z >> 7; z ^= z << 17;
+      v1[p] = (double)(z >> 11) / 9007199254740992.0 - 0.5;
+    }
+    double c = dotv(v1, V, N);
+    for (size_t p = 0; p < N; p++) v1[p] -= c * V[p];
+    double n1 = sqrt(dotv(v1, v1, N));
+    for (size_t p = 0; p < N; p++) v1[p] /= n1;
+  }
+  int k = 1, iters = 0, done = 0, restarts = 0;
+  memset(T, 0, sizeof(double) * m * m);
+  {
+    /* T[0][0] = v0' M v0 */
+    stencil_apply(Wd, s, W, H, V, w);
+    T[0] = dotv(V, w, N);
+    iters++;
+  }
+  double beta = 0;
+  while (!done) {
+    for (int j = k; j < m; j++) {
+      stencil_apply(Wd, s, W, H, V + (size_t)j * N, w);
+      iters++;
+      for (int i = 0; i <= j; i++) h[i] = 0;
+      for (int pass = 0; pass < 2; pass++)
+        for (int i = 0; i <= j; i++) {
+          double c = dotv(V + (size_t)i * N, w, N);
+          h[i] += c;
+          for (size_t p = 0; p < N; p++) w[p] -= c * V[(size_t)i * N + p];
+        }
+      for (int i = 0; i <= j; i++) { T[i * m + j] = h[i]; T[j * m + i] = h[i]; }
+      beta = sqrt(dotv(w, w, N));
+      if (beta < 1e-300) beta = 1e-300;
+      for (size_t p = 0; p < N; p++) V[(size_t)(j + 1) * N + p] = w[p] / beta;
+    }
+    memcpy(A, T, sizeof(double) * m * m);
+    jacobi_eig(m, A, th, Y);
+    /* Ritz vectors of the leading K (plus extras to keep) */
+    int keep = K + (m - K) / 2;
+    if (keep > m - 1) keep = m - 1;
+    for (int i = 0; i < keep; i++) {
+      double* u = U + (size_t)i * N;
+      for (size_t p = 0; p < N; p++) u[p] = 0;
+      for (int j = 0; j < m; j++) {
+        double c = Y[j * m + i];
+        const double* vj = V + (size_t)j * N;
+        for (size_t p = 0; p < N; p++) u[p] += c * vj[p];
+      }
+    }
+    /* relative generalised residuals of the leading K */
+    double bd = 0;
+    const double* vm = V + (size_t)m * N;
+    for (size_t p = 0; p < N; p++) bd += D[p] * vm[p] * vm[p];
+    bd = sqrt(bd);
+    int conv = 1;
+    for (int i = 0; i < K; i++) {
+      double ud = 0;
+      const double* u = U + (size_t)i * N;
+      for (size_t p = 0; p < N; p++) ud += D[p] * u[p] * u[p];
+      double res = fabs(beta * Y[(m - 1) * m + i]) * bd / sqrt(ud);
+      if (res > tol) conv = 0;
+    }
+    if (conv || iters >= maxit) {
+      for (int i = 0; i < K; i++) {
+        evals[i] = 1.0 - th[i];
+        for (size_t p = 0; p < N; p++) evecs[(size_t)i * N + p] = s[p] * U[(size_t)i * N + p];
+      }
+      done = 1;
+      break;
+    }
+    /* thick restart */
+    memcpy(V + (size_t)keep * N, vm, sizeof(double) * N);
+    memcpy(V, U, sizeof(double) * N * keep);
+    memset(T, 0, sizeof(double) * m * m);
+    for (int i = 0; i < keep; i++) T[i * m + i] = th[i];
+    k = keep;
+    restarts++;
+  }
+  if (iters_out) *iters_out = iters;
+  free(s); free(V); free(w); free(T); free(A); free(th); free(Y); free(U); free(h);
+  return (iters >= maxit) ? -1 : restarts;
+}
+
+/* true relative generalised residual of one pair */
+double orc_gen_residual(const double* Wd, const double* D, int W, int H, double lam, const double* v) {
+  size_t N = (size_t)W * H;
+  double* one = (double*)malloc(sizeof(double) * N);
+  double* Wv = (double*)malloc(sizeof(double) * N);
+  for (size_t p = 0; p < N; p++) one[p] = 1.0;
+  stencil_apply(Wd, one, W, H, v, Wv);
+  double num = 0, den = 0;
+  for (size_t p = 0; p < N; p++) {
+    double r = D[p] * v[p] - Wv[p] - lam * D[p] * v[p];
+    num += r * r;
+    den += (D[p] * v[p]) * (D[p] * v[p]);
+  }
+  free(one);
+  free(Wv);
+  return sqrt(num / den);
+}
+
+/* =========================================================================
+ * a17-a19: eigenvector normalisation, sPb and gPb.
+ * spectralPb.cpp:257-322: for i = 1..K-1, v <- (v - min)/(max - min)
+ * (left as is when max == min), divided by sqrt(lambda_i) when lambda_i >
+ * DBL_EPSILON, correlated (filter2D, BORDER_REFLECT_101) with 8 oriented
+ * 7x7 first-derivative filters; sPb_o += |response|.  gPb_o = sum of the 12
+ * gPb-weighted cues + 0.0074 sPb_o (globalPb.cpp:31-37, 97-103).
+ * ========================================================================= */
+static void spb_filters(double* out /* [8][7][7] */) {
+  for (int o = 0; o < 8; o++) {
+    double ori = (double)o * M_PI / 8;
+    /* rotated supports (support_x_rotated / support_y_rotated with -ori) */
+    double sxa = 3 * cos(-ori), sya = 3 * sin(-ori);
+    double x0m = fabs(sxa - sya), x1m = fabs(sxa + sya);
+    long sxr = (long)ceil(x0m > x1m ? x0m : x1m);
+    double s2 = 3 * sin(-ori), c2 = 3 * cos(-ori);
+    double y0m = fabs(s2 - c2), y1m = fabs(s2 + c2);
+    long syr = (long)ceil(y0m > y1m ? y0m : y1m);
+    long nx = 2 * sxr + 1, ny = 2 * syr + 1;
+    double mx[64], my[64];
+    double sa = 0, sm = 0;
+    for (long i = 0; i < nx; i++) { double x = (double)(i - sxr); mx[i] = exp(-0.5 * x * x); }
+    for (long i = 0; i < nx; i++) sa += fabs(mx[i]);
+    for (long i = 0; i < nx; i++) mx[i] /= sa;
+    for (long i = 0; i < ny; i++) { double x = (double)(i - syr); my[i] = exp(-0.5 * x * x) * (-x); sm += my[i]; }
+    sm /= (double)ny;
+    sa = 0;
+    for (long i = 0; i < ny; i++) { my[i] -= sm; sa += fabs(my[i]); }
+    for (long i = 0; i < ny; i++) my[i] /= sa;
+    double* f = out + o * 49;
+    double co = cos(ori), so = sin(ori), ox = (double)(nx - 1) / 2, oy = (double)(ny - 1) / 2;
+    for (int a = 0; a < 7; a++)
+      for (int b = 0; b < 7; b++) {
+        double u = a - 3.0, v = b - 3.0;
+        double x = u * co + v * so + ox, y = v * co - u * so + oy, val = 0;
+        if (x >= 0 && y >= 0) {
+          long xa = (long)floor(x), xb = (long)ceil(x), ya = (long)floor(y), yb = (long)ceil(y);
+          if (xb < nx && yb < ny) {
+            double m00 = mx[xa] * my[ya], m01 = mx[xa] * my[yb], m10 = mx[xb] * my[ya], m11 = mx[xb] * my[yb];
+            double t0 = (xa != xb) ? ((xb - x) * m00 + (x - xa) * m10) : m00;
+            double t1 = (xa != xb) ? ((xb - x) * m01 + (x - xa) * m11) : m01;
+            val = (ya != yb) ? ((yb - y) * t0 + (y - ya) * t1) : t0;
+          }
+        }
+        f[a * 7 + b] = val;
+      }
+    double mean = 0, sab = 0;
+    for (int n = 0; n < 49; n++) mean += f[n];
+    mean /= 49;
+    for (int n = 0; n < 49; n++) { f[n] -= mean; sab += fabs(f[n]); }
+    for (int n = 0; n < 49; n++) f[n] /= sab;
+  }
+}
+
+void orc_spb_filters(double* out) { spb_filters(out); }
+
+void orc_spb(const double* evecs, const double* evals, int K, int W, int H, double* spb /* [8][N] */) {
+  size_t N = (size_t)W * H;
+  double F[8 * 49];
+  spb_filters(F);
+  double* v = (double*)malloc(sizeof(double) * N);
+  memset(spb, 0, sizeof(double) * 8 * N);
+  for (int i = 1; i < K; i++) {
+    const double* e = evecs + (size_t)i * N;
+    double mn = e[0], mx = e[0];
+    for (size_t p = 0; p < N; p++) { if (e[p] < mn) mn = e[p]; if (e[p] > mx) mx = e[p]; }
+    for (size_t p = 0; p < N; p++) v[p] = (mx > mn) ? (e[p] - mn) / (mx - mn) : e[p];
+    if (!(evals[i] > DBL_EPSILON)) continue;
+    double sc = sqrt(evals[i]);
+    for (size_t p = 0; p < N; p++) v[p] /= sc;
+    for (int o = 0; o < 8; o++)
+      for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+          double acc = 0;
+          for (int a = 0; a < 7; a++)
+            for (int b = 0; b < 7; b++)
+              acc += F[o * 49 + a * 7 + b] * v[(size_t)refl101(y + a - 3, H) * W + refl101(x + b - 3, W)];
+          spb[(size_t)o * N + (size_t)y * W + x] += fabs(acc);
+        }
+  }
+  free(v);
+}
+
+static const double GPB_W[13] = {0, 0, 0.0039, 0.0050, 0.0058, 0.0069, 0.0040, 0.0044, 0.0049, 0.0024, 0.0027, 0.0170, 0.0074};
+void orc_gpb(const double* cues /* [12][8][N] */, const double* spb /* [8][N] */, size_t N, double* gpb /* [8][N] */) {
+  for (int o = 0; o < 8; o++)
+    for (size_t p = 0; p < N; p++) {
+      double v = 0;
+      for (int c = 0; c < 12; c++) v += GPB_W[c] * cues[((size_t)c * 8 + o) * N + p];
+      v += GPB_W[12] * spb[(size_t)o * N + p];
+      gpb[(size_t)o * N + p] = v;
+    }
+}
+
+/* computeGPb output transforms (damascene globalPb.cu:5-49, skeleton.cu:229-249,
+ * damascene.cu:594-604): max over orientations masked by mPb > 0.05, and the
+ * calibration sigmoid of each oriented map, written column-major. */
+void orc_gpb_outputs(const double* gpb /* [8][N] */, const double* mpb_nmax, int W, int H, float* gmax, float* gall) {
+  size_t N = (size_t)W * H;
+  for (size_t p = 0; p < N; p++) {
+    float m = -200.f;
+    for (int o = 0; o < 8; o++) {
+      float v = (float)gpb[(size_t)o * N + p];
+      if (v > m) m = v;
+    }
+    gmax[p] = (mpb_nmax[p] > 0.05) ? m : 0.0f;
+  }
+  for (int o = 0; o < 8; o++)
+    for (int y = 0; y < H; y++)
+      for (int x = 0; x < W; x++) {
+        float value = (float)gpb[(size_t)o * N + (size_t)y * W + x];
+        if (value * 1.2 > 1) value = 1;
+        else value *= 1.2;
+        if (value < 0) value = 0;
+        value = 1.0f / (1.0f + expf(2.6433 - 10.7998 * value));
+        value = (value - 0.0667) / 0.9333;
+        gall[(size_t)o * N + (size_t)x * H + y] = value;
+      }
+}

@@ -130,3 +130,109 @@ def nonmax(mpb_all: np.ndarray) -> np.ndarray:
     out = np.empty((H, W), np.float64)
     lib.orc_nonmax(m.reshape(-1), W, H, out.reshape(-1))
     return out
+
+_sig("orc_build_w", C.c_int, [_f64p, C.c_int, C.c_int, _f64p, _f64p])
+_sig("orc_eigs", C.c_int, [_f64p, _f64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _f64p, _f64p,
+                           C.POINTER(C.c_int)])
+_sig("orc_gen_residual", C.c_double, [_f64p, _f64p, C.c_int, C.c_int, C.c_double, _f64p])
+_sig("orc_jacobi_eig", None, [C.c_int, _f64p, _f64p, _f64p])
+_sig("orc_spb_filters", None, [_f64p])
+_sig("orc_spb", None, [_f64p, _f64p, C.c_int, C.c_int, C.c_int, _f64p])
+_sig("orc_gpb", None, [_f64p, _f64p, C.c_size_t, _f64p])
+_sig("orc_gpb_outputs", None, [_f64p, _f64p, C.c_int, C.c_int, _f32p, _f32p])
+
+DISC = [(v, u) for u in range(-5, 6) for v in range(-5, 6) if u * u + v * v <= 25]  # (dx, dy), row-major
+
+
+def build_w(mpb_nmax: np.ndarray):
+    """Returns (Wd[81, H, W], D[H, W]) for the 81 disc offsets DISC."""
+    m = np.ascontiguousarray(mpb_nmax, np.float64)
+    H, W = m.shape
+    Wd = np.empty((81, H, W), np.float64)
+    D = np.empty((H, W), np.float64)
+    rc = lib.orc_build_w(m.reshape(-1), W, H, Wd.reshape(-1), D.reshape(-1))
+    if rc != 0:
+        raise RuntimeError("support scan line not strictly increasing")
+    return Wd, D
+
+
+def dense_w(Wd: np.ndarray) -> np.ndarray:
+    """Dense N x N W from the disc layout (tiny images only)."""
+    _, H, W = Wd.shape
+    N = H * W
+    A = np.zeros((N, N))
+    for k, (dx, dy) in enumerate(DISC):
+        for y in range(H):
+            for x in range(W):
+                x2, y2 = x + dx, y + dy
+                if 0 <= x2 < W and 0 <= y2 < H:
+                    A[y * W + x, y2 * W + x2] = Wd[k, y, x]
+    return A
+
+
+def eigs(Wd: np.ndarray, D: np.ndarray, K: int = 9, m: int = 40, tol: float = 1e-10, maxit: int = 100000):
+    Wd = np.ascontiguousarray(Wd, np.float64)
+    D = np.ascontiguousarray(D, np.float64)
+    H, W = D.shape
+    ev = np.empty(K, np.float64)
+    vec = np.empty((K, H, W), np.float64)
+    it = C.c_int(0)
+    rc = lib.orc_eigs(Wd.reshape(-1), D.reshape(-1), W, H, K, m, tol, maxit, ev, vec.reshape(-1), C.byref(it))
+    if rc < 0:
+        raise RuntimeError("oracle eigensolver did not converge")
+    return ev, vec, it.value
+
+
+def gen_residual(Wd, D, lam, v) -> float:
+    H, W = D.shape
+    return lib.orc_gen_residual(np.ascontiguousarray(Wd, np.float64).reshape(-1),
+                                np.ascontiguousarray(D, np.float64).reshape(-1), W, H, float(lam),
+                                np.ascontiguousarray(v, np.float64).reshape(-1))
+
+
+def spb_filters() -> np.ndarray:
+    out = np.empty((8, 7, 7), np.float64)
+    lib.orc_spb_filters(out.reshape(-1))
+    return out
+
+
+def spb(evecs: np.ndarray, evals: np.ndarray) -> np.ndarray:
+    e = np.ascontiguousarray(evecs, np.float64)
+    K, H, W = e.shape
+    out = np.empty((8, H, W), np.float64)
+    lib.orc_spb(e.reshape(-1), np.ascontiguousarray(evals, np.float64), K, W, H, out.reshape(-1))
+    return out
+
+
+def gpb(cues: np.ndarray, spb_maps: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(cues, np.float64)
+    s = np.ascontiguousarray(spb_maps, np.float64)
+    _, H, W = s.shape
+    out = np.empty((8, H, W), np.float64)
+    lib.orc_gpb(c.reshape(-1), s.reshape(-1), H * W, out.reshape(-1))
+    return out
+
+
+def gpb_outputs(gpb_maps: np.ndarray, mpb_nmax: np.ndarray):
+    """computeGPb outputs: (gPb max masked [H, W], oriented sigmoid maps [8, W, H] column-major)."""
+    g = np.ascontiguousarray(gpb_maps, np.float64)
+    _, H, W = g.shape
+    gmax = np.empty((H, W), np.float32)
+    gall = np.empty((8, W, H), np.float32)
+    lib.orc_gpb_outputs(g.reshape(-1), np.ascontiguousarray(mpb_nmax, np.float64).reshape(-1), W, H,
+                        gmax.reshape(-1), gall.reshape(-1))
+    return gmax, gall
+
+
+def pipeline(packed: np.ndarray, codebook: np.ndarray, K: int = 9, tol: float = 1e-10, m: int = 40):
+    """Whole CPU path; returns a dict of every intermediate."""
+    t = texton(packed, codebook)
+    cues, mpb_all = local_cues(packed, t)
+    nm = nonmax(mpb_all)
+    Wd, D = build_w(nm)
+    ev, vec, it = eigs(Wd, D, K=K, m=m, tol=tol)
+    s = spb(vec, ev)
+    g = gpb(cues, s)
+    gmax, gall = gpb_outputs(g, nm)
+    return dict(textons=t, cues=cues, mpb_all=mpb_all, mpb_nmax=nm, Wd=Wd, D=D, evals=ev, evecs=vec, iters=it,
+                spb=s, gpb=g, gpb_max=gmax, gpb_all=gall)

@@ -188,3 +188,33 @@ int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float*
 }
 
 }  // extern "C"
+
+// ---- diagnostics ------------------------------------------------------------------
+extern "C" int gpb_ic_program(int left, int right, int top, int bottom, int8_t* conn, int8_t* sites, int max_sites) {
+  return guarded(nullptr, [&] {
+    if (left < 0 || left > 5 || right < 0 || right > 5 || top < 0 || top > 5 || bottom < 0 || bottom > 5 || !conn)
+      throw ArgError("bad program arguments");
+    const ICProgram& p = ic_program(left, right, top, bottom);
+    int ddx[kStencilFull], ddy[kStencilFull];
+    disc_offsets(ddx, ddy);
+    std::memset(conn, 0, kStencilFull * 5);
+    for (const ICConnection& c : p.conn) {
+      for (int k = 0; k < kStencilFull; k++)
+        if (ddx[k] == c.dx && ddy[k] == c.dy) {
+          int8_t* e = conn + 5 * k;
+          e[0] = 1;  // valid
+          e[1] = int8_t(c.found);
+          e[2] = int8_t(c.site_count);
+          e[3] = int8_t(c.site_start & 0x7f);
+          e[4] = int8_t(c.site_start >> 7);
+        }
+    }
+    if (sites) {
+      if (int(p.site_dx.size()) > max_sites) throw ArgError("site buffer too small");
+      for (size_t i = 0; i < p.site_dx.size(); i++) {
+        sites[2 * i] = p.site_dx[i];
+        sites[2 * i + 1] = p.site_dy[i];
+      }
+    }
+  });
+}

@@ -88,6 +88,46 @@ struct LanczosStats {
   int fallback = 0;         // warm start rejected, plain solve used
 };
 
+// Per-kernel-class timing with HIP events on the context stream (enabled by
+// gpb_set_profiling); durations are folded in when the stream is synchronised.
+enum KernelClass {
+  kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKOther, kNumKClasses
+};
+struct KernelTimes {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;  // (class, index of the start event)
+  double total_ms[kNumKClasses] = {0};
+  long count[kNumKClasses] = {0};
+  double bytes[kNumKClasses] = {0};  // algorithmic HBM bytes of the timed launches
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      GPB_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void fold() {
+    for (auto& pr : pending) {
+      float ms = 0;
+      GPB_HIP(hipEventElapsedTime(&ms, pool[pr.second], pool[pr.second + 1]));
+      total_ms[pr.first] += ms;
+      count[pr.first]++;
+    }
+    pending.clear();
+    used = 0;
+  }
+  void reset() {
+    for (int i = 0; i < kNumKClasses; i++) {
+      total_ms[i] = 0;
+      count[i] = 0;
+      bytes[i] = 0;
+    }
+  }
+};
+
 struct Context;
 void upload_tables(Context& c);
 
@@ -104,7 +144,15 @@ struct Context {
   // workspace (grow-only)
   std::map<std::string, std::shared_ptr<void>> scratch;
   LanczosStats last_lanczos;
+  KernelTimes kt;
+  double last_evals[16] = {0};
   double stage_ms[8] = {0};
+  // eigensolver options (gpb_set_eig_options)
+  int eig_K = 9;
+  int eig_m = 48;
+  int eig_keep = 24;
+  double eig_tol = 1e-3;
+  int eig_max_matvecs = 20000;
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {
@@ -122,6 +170,28 @@ struct Context {
 };
 
 inline unsigned div_up(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
+
+// Records a start/stop event pair around the launches in its scope.
+struct KScope {
+  Context& c;
+  int cls;
+  size_t idx = 0;
+  KScope(Context& cc, int k, double algo_bytes) : c(cc), cls(k) {
+    if (c.kt.on) {
+      c.kt.bytes[k] += algo_bytes;
+      idx = c.kt.used;
+      hipEvent_t a = c.kt.next();
+      (void)c.kt.next();
+      GPB_HIP(hipEventRecord(a, c.stream));
+    }
+  }
+  ~KScope() {
+    if (c.kt.on) {
+      (void)hipEventRecord(c.kt.pool[idx + 1], c.stream);
+      c.kt.pending.push_back({cls, idx});
+    }
+  }
+};
 
 }  // namespace gpb
 

@@ -19,4 +19,41 @@ void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text,
                        float* d_gcue, float* d_cues /* optional [12][8][N] */);
 void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nmax);
 
+// stage (c): radius-5 IC stencil in a zero-padded layout (5-pixel halo)
+struct Stencil {
+  int W = 0, H = 0, Wp = 0, Hp = 0;
+  size_t Np = 0;
+  float* wup = nullptr;   // [40][Np] symmetric affinities W(p, p + e_k)
+  float* what = nullptr;  // [40][Np] normalised s_p W s_q
+  float* D = nullptr;     // [Np] degree
+  float* s = nullptr;     // [Np] D^-1/2
+  float* diag = nullptr;  // [Np] s_p^2 (normalised self affinity, W_pp = 1)
+  int* offs = nullptr;    // [40] padded linear offsets of the upper stencil
+  int off_host[kStencilHalf] = {};
+  size_t pad_index(int x, int y) const { return size_t(y + kICRadius) * Wp + (x + kICRadius); }
+};
+void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st);
+void stencil_upper_offsets(int* dx, int* dy);
+void launch_fill(Context& c, float* p, size_t n, float v);
+
+// stage (d)
+struct EigOptions {
+  int K = 9;             // eigenpairs (including the trivial one)
+  int m = 48;            // basis size
+  int keep = 24;         // vectors kept at a restart
+  double tol = 1e-3;     // relative generalised residual target
+  int max_matvecs = 20000;
+};
+// Generalised eigenpairs of (D - W, D): evals ascending (host), evecs [K][Np]
+// padded layout, generalised normalisation v^T D v = 1.
+void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, double* evals, float* d_evecs,
+                   LanczosStats& stats);
+// y = M x on padded vectors (exposed for residual checks)
+void launch_spmv(Context& c, const Stencil& st, const float* x, float* y);
+
+// stage (e)
+void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb /* [8][N] */);
+void launch_gpb(Context& c, int W, int H, const float* d_gcue, const float* d_spb, const float* d_nmax,
+                float* d_gpb_orient /* [8][N] or null */, float* d_gmax /* [N] or null */, float* d_gall /* [8][W][H] or null */);
+
 }  // namespace gpb

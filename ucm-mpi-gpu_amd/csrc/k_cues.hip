@@ -426,6 +426,7 @@ void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text,
   P.cues = d_cues;
   P.W = W;
   P.H = H;
+  KScope ks(c, kKCues, double(N) * (4 + 64 + (d_cues ? 384 : 0)));
   hipLaunchKernelGGL(cues_kernel, dim3(div_up(W, kCueTile), div_up(H, kCueTile)), dim3(kCueLanes), 0, c.stream, P);
   GPB_LAUNCH_CHECK();
 }
@@ -444,6 +445,7 @@ void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nm
   const size_t N = size_t(W) * H;
   float* a = c.ws<float>("nms.a", N);
   uint8_t* k = c.ws<uint8_t>("nms.k", N);
+  KScope ks(c, kKNonmax, double(N) * 46);
   hipLaunchKernelGGL(max_ori_kernel, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_mpb_all, N, a, k);
   GPB_LAUNCH_CHECK();
   hipLaunchKernelGGL(nonmax_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, a, k, W, H, t.nms, d_nmax);

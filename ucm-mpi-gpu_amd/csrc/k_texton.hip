@@ -89,6 +89,7 @@ void upload_schmid(Context& c) {
 void launch_texton(Context& c, const uint32_t* d_rgb, int W, int H, const float* d_codebook, uint8_t* d_labels) {
   dim3 block(kTexTX, kTexTY);
   dim3 grid(div_up(W, kTexTX), div_up(H, kTexTY));
+  KScope ks(c, kKTexton, 5.0 * W * H);
   hipLaunchKernelGGL(texton_kernel, grid, block, 0, c.stream, d_rgb, W, H, c.schmid_off.p, c.schmid_start.p,
                      c.schmid_w.p, c.schmid_classes, d_codebook, d_labels);
   GPB_LAUNCH_CHECK();

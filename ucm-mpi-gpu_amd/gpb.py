@@ -46,6 +46,37 @@ _sig("gpb_texton", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, _u8p])
 _sig("gpb_lab_quantize", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, _i32p])
 _sig("gpb_local_cues", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, C.c_void_p, _f32p])
 _sig("gpb_nonmax", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p])
+_sig("gpb_set_eig_options", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int])
+_sig("gpb_build_w", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, _f32p])
+_sig("gpb_eigs", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f64p, C.c_void_p])
+_sig("gpb_spectral_pb", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, C.c_void_p])
+_sig("gpb_spb_from_evecs", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _f32p, _f64p, _f32p])
+_sig("gpb_compute", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
+_sig("gpb_compute_device", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p])
+_sig("gpb_ic_program", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS"),
+                                 np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS"), C.c_int])
+
+
+class Stats(C.Structure):
+    _fields_ = [("stage_ms", C.c_double * 6), ("lanczos_matvecs", C.c_int), ("lanczos_restarts", C.c_int),
+                ("lanczos_basis", C.c_int), ("lanczos_block", C.c_int), ("converged", C.c_int),
+                ("max_residual", C.c_double), ("evals", C.c_double * 16), ("fallback", C.c_int)]
+
+
+_sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
+_sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_kernel_time", C.c_int, [_ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double)])
+KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb"]
+
+
+def ic_program(left: int, right: int, top: int, bottom: int):
+    conn = np.zeros((81, 5), np.int8)
+    sites = np.zeros((4096, 2), np.int8)
+    rc = lib.gpb_ic_program(left, right, top, bottom, conn.reshape(-1), sites.reshape(-1), 4096)
+    _check(rc, what="ic_program")
+    start = (conn[:, 3].astype(np.int64) & 0x7f) | (conn[:, 4].astype(np.int64) << 7)
+    return conn[:, 0].astype(bool), conn[:, 1].astype(bool), conn[:, 2].astype(np.int64), start, sites
 
 
 class GpbError(RuntimeError):
@@ -151,3 +182,73 @@ class Context:
         out = np.empty((H, W), np.float32)
         self._chk(lib.gpb_nonmax(self.h, W, H, m.reshape(-1), out.reshape(-1)))
         return out
+
+    def set_eig_options(self, nvec=9, basis=48, keep=24, tol=1e-3, max_matvecs=20000):
+        self._chk(lib.gpb_set_eig_options(self.h, nvec, basis, keep, tol, max_matvecs))
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._chk(lib.gpb_get_stats(self.h, C.byref(s)))
+        return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
+                    basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
+                    evals=list(s.evals), fallback=s.fallback)
+
+    def build_w(self, mpb_nmax: np.ndarray):
+        m = np.ascontiguousarray(mpb_nmax, np.float32)
+        H, W = m.shape
+        wu = np.empty((40, H, W), np.float32)
+        D = np.empty((H, W), np.float32)
+        self._chk(lib.gpb_build_w(self.h, W, H, m.reshape(-1), wu.reshape(-1), D.reshape(-1)))
+        return wu, D
+
+    def eigs(self, mpb_nmax: np.ndarray, nvec: int = 9, with_vectors: bool = True):
+        m = np.ascontiguousarray(mpb_nmax, np.float32)
+        H, W = m.shape
+        ev = np.empty(nvec, np.float64)
+        vec = np.empty((nvec, H, W), np.float32) if with_vectors else None
+        self._chk(lib.gpb_eigs(self.h, W, H, m.reshape(-1), ev, vec.ctypes.data if vec is not None else None))
+        return ev, vec
+
+    def spectral_pb(self, mpb_nmax: np.ndarray, nvec: int = 9):
+        m = np.ascontiguousarray(mpb_nmax, np.float32)
+        H, W = m.shape
+        spb = np.empty((8, H, W), np.float32)
+        ev = np.empty(nvec, np.float64)
+        self._chk(lib.gpb_spectral_pb(self.h, W, H, m.reshape(-1), spb.reshape(-1), ev.ctypes.data))
+        return spb, ev
+
+    def spb_from_evecs(self, evecs: np.ndarray, evals: np.ndarray) -> np.ndarray:
+        e = np.ascontiguousarray(evecs, np.float32)
+        K, H, W = e.shape
+        spb = np.empty((8, H, W), np.float32)
+        self._chk(lib.gpb_spb_from_evecs(self.h, W, H, K, e.reshape(-1), np.ascontiguousarray(evals, np.float64),
+                                         spb.reshape(-1)))
+        return spb
+
+    def compute(self, packed: np.ndarray, codebook: np.ndarray):
+        """computeGPb semantics; returns dict of gpb_max [H,W], gpb_all [8,W,H], gpb_orient [8,H,W], mpb_nmax."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        gmax = np.empty((H, W), np.float32)
+        gall = np.empty((8, W, H), np.float32)
+        gor = np.empty((8, H, W), np.float32)
+        nm = np.empty((H, W), np.float32)
+        self._chk(lib.gpb_compute(self.h, W, H, packed.reshape(-1), np.ascontiguousarray(codebook, np.float32).reshape(-1),
+                                  gmax.ctypes.data, gall.ctypes.data, gor.ctypes.data, nm.ctypes.data))
+        return dict(gpb_max=gmax, gpb_all=gall, gpb_orient=gor, mpb_nmax=nm)
+
+    def set_profiling(self, on: bool):
+        self._chk(lib.gpb_set_profiling(self.h, int(on)))
+
+    def kernel_times(self) -> dict:
+        out = {}
+        for i, n in enumerate(KERNEL_CLASSES):
+            ms, cnt, by = C.c_double(), C.c_long(), C.c_double()
+            self._chk(lib.gpb_kernel_time(self.h, i, C.byref(ms), C.byref(cnt), C.byref(by)))
+            out[n] = dict(ms=ms.value, launches=cnt.value, bytes=by.value)
+        return out
+
+    def compute_device(self, width: int, height: int, d_packed: int, d_codebook: int, d_gmax: int = 0, d_gall: int = 0,
+                       d_orient: int = 0):
+        self._chk(lib.gpb_compute_device(self.h, width, height, d_packed, d_codebook, d_gmax or None, d_gall or None,
+                                         d_orient or None))
