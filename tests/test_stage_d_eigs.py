@@ -65,7 +65,7 @@ def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol):
     print(f"  matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e}", flush=True)
     assert st["max_residual"] <= tol
     rel = np.abs(ev[1:] - ev_ref[1:9]) / ev_ref[1:9]
-    print(f"  max eigenvalue rel err {rel.max():.2e}", flush=True)
+    print(f"  max eigenvalue rel err {rel.max():.2e}\n  gpu {ev}\n  ref {ev_ref}", flush=True)
     assert rel.max() <= 1e-5
     gap_ok = ev_ref[9] - ev_ref[8] > 1e-3 * ev_ref[8]
     if gap_ok:

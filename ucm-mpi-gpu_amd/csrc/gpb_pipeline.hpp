@@ -24,10 +24,8 @@ struct Stencil {
   int W = 0, H = 0, Wp = 0, Hp = 0;
   size_t Np = 0;
   float* wup = nullptr;   // [40][Np] symmetric affinities W(p, p + e_k)
-  float* what = nullptr;  // [40][Np] normalised s_p W s_q
   float* D = nullptr;     // [Np] degree
   float* s = nullptr;     // [Np] D^-1/2
-  float* diag = nullptr;  // [Np] s_p^2 (normalised self affinity, W_pp = 1)
   int* offs = nullptr;    // [40] padded linear offsets of the upper stencil
   int off_host[kStencilHalf] = {};
   size_t pad_index(int x, int y) const { return size_t(y + kICRadius) * Wp + (x + kICRadius); }
@@ -48,7 +46,7 @@ struct EigOptions {
 // padded layout, generalised normalisation v^T D v = 1.
 void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, double* evals, float* d_evecs,
                    LanczosStats& stats);
-// y = M x on padded vectors (exposed for residual checks)
+// y = D^-1/2 (D - W) D^-1/2 x on padded vectors
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y);
 
 // stage (e)

@@ -9,7 +9,7 @@
 // neighbour becomes a short list of pixel offsets whose mPb maximum is the
 // intervening contour.  The kernel stages an mPb tile with a 5-pixel halo in
 // LDS and evaluates, for every upper disc offset, both directed affinities and
-// their average.  W is stored once per unordered pair: Wup[k][p] for the 40
+// their average; the degree kernel then forms D and D^-1/2.  W is stored once per unordered pair: Wup[k][p] for the 40
 // offsets with dy > 0 or (dy == 0, dx > 0), in a zero-padded layout (5-pixel
 // halo) so that every stencil access downstream is in bounds.
 #include <mutex>
@@ -52,6 +52,9 @@ static AffTables& aff_tables(Context& c) {
   std::vector<int16_t> sites;
   for (int cfg = 0; cfg < kNumConfigs; cfg++) {
     const int l = cfg / 216, r = (cfg / 36) % 6, tp = (cfg / 6) % 6, b = cfg % 6;
+    // Windows clipped to a single row or column (images 1 pixel wide or high)
+    // have no valid walk set (ic.cc asserts); such images are rejected below.
+    if ((l == 0 && r == 0) || (tp == 0 && b == 0)) continue;
     const ICProgram& prog = ic_program(l, r, tp, b);
     for (const ICConnection& cn : prog.conn) {
       int k = -1;
@@ -159,19 +162,8 @@ __global__ __launch_bounds__(256) void degree_kernel(const float* __restrict__ w
   s[pp] = float(1.0 / sqrt(acc));
 }
 
-// Normalised stencil: what[k][p] = s_p W[k][p] s_{p+e_k}; diag_p = s_p^2.
-__global__ __launch_bounds__(256) void normalize_kernel(const float* __restrict__ wup, const float* __restrict__ s,
-                                                        int W, int H, int Wp, size_t Np, const int* __restrict__ offs,
-                                                        float* __restrict__ what, float* __restrict__ diag) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= W) return;
-  const size_t pp = size_t(y + kICRadius) * Wp + (x + kICRadius);
-  const float sp = s[pp];
-  for (int j = 0; j < kStencilHalf; j++) what[size_t(j) * Np + pp] = sp * wup[size_t(j) * Np + pp] * s[pp + offs[j]];
-  diag[pp] = sp * sp;
-}
-
 void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
+  if (W < 2 || H < 2) throw ArgError("the affinity needs an image at least 2 pixels wide and high");
   AffTables& t = aff_tables(c);
   st.W = W;
   st.H = H;
@@ -179,10 +171,8 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
   st.Hp = H + 2 * kICRadius;
   st.Np = size_t(st.Wp) * st.Hp;
   st.wup = c.ws<float>("st.wup", kStencilHalf * st.Np);
-  st.what = c.ws<float>("st.what", kStencilHalf * st.Np);
   st.D = c.ws<float>("st.D", st.Np);
   st.s = c.ws<float>("st.s", st.Np);
-  st.diag = c.ws<float>("st.diag", st.Np);
   st.offs = c.ws<int>("st.offs", kStencilHalf);
   int offs[kStencilHalf];
   for (int j = 0; j < kStencilHalf; j++) offs[j] = t.updy[j] * st.Wp + t.updx[j];
@@ -190,10 +180,8 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
   for (int j = 0; j < kStencilHalf; j++) st.off_host[j] = offs[j];
   // zero halos: the kernels below write interior pixels only
   launch_fill(c, st.wup, kStencilHalf * st.Np, 0.f);
-  launch_fill(c, st.what, kStencilHalf * st.Np, 0.f);
   launch_fill(c, st.D, st.Np, 0.f);
   launch_fill(c, st.s, st.Np, 0.f);
-  launch_fill(c, st.diag, st.Np, 0.f);
   AffParams P{};
   P.mpb = d_mpb;
   P.conn = t.conn.p;
@@ -213,9 +201,6 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
   GPB_LAUNCH_CHECK();
   hipLaunchKernelGGL(degree_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, st.wup, W, H, st.Wp, st.Np,
                      st.offs, st.D, st.s);
-  GPB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(normalize_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, st.wup, st.s, W, H, st.Wp,
-                     st.Np, st.offs, st.what, st.diag);
   GPB_LAUNCH_CHECK();
 }
 

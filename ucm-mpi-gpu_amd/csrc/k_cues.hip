@@ -106,89 +106,112 @@ struct CueParams {
 };
 
 // One snapshot: chi-squared of all orientations from the lane's histogram.
+// Histogram word [grp][bin][lane] holds the 8-bit counts of slices 4*grp..+3.
+// csum[b] (LDS, shared by the wave) = column sums of the truncated smoothing.
 template <int NB, int KH>
 __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
-                                              const float* __restrict__ g, float out[kNumOri]) {
-  // column sums of the truncated smoothing matrix
-  float csum[NB];
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    float s = 0.f;
-#pragma unroll
-    for (int k = -KH; k <= KH; k++) {
-      const int n = b - k;
-      if (n >= 0 && n < nb) s += g[k < 0 ? -k : k];
-    }
-    csum[b] = (b < nb) ? s : 0.f;
-  }
-  auto load_slice = [&](int s, float h[NB]) {
-    const int grp = s >> 2, sh = 8 * (s & 3);
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      const uint32_t w = (b < nb) ? hist[(grp * kNbMax + b) * kCueLanes + lane] : 0u;
-      h[b] = float((w >> sh) & 0xffu);
-    }
-  };
+                                           const float* __restrict__ g, const float* __restrict__ csum,
+                                           float out[kNumOri]) {
+  // pass A: per-slice mass after smoothing, a_s = 1 / mass
   float alpha[kNumSlices];
-  float UT[NB], V[NB];
 #pragma unroll
-  for (int b = 0; b < NB; b++) UT[b] = V[b] = 0.f;
-#pragma unroll
-  for (int s = 0; s < kNumSlices; s++) {
-    float h[NB];
-    load_slice(s, h);
-    float m = 0.f;
-#pragma unroll
-    for (int b = 0; b < NB; b++) m = __builtin_fmaf(csum[b], h[b], m);
-    const float a = (m > 0.f) ? 1.0f / m : 0.f;
-    alpha[s] = a;
-    const float sg = (s < kNumOri) ? -a : a;  // orientation 0: left = slices [0,8)
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      UT[b] = __builtin_fmaf(a, h[b], UT[b]);
-      V[b] = __builtin_fmaf(sg, h[b], V[b]);
-    }
-  }
-  // smoothed sum (same for all orientations)
-  float T[NB];
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = 0.f;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    float s = 0.f;
+    if (b < nb) {
+      const float cb = csum[b];
 #pragma unroll
-    for (int k = -KH; k <= KH; k++) {
-      const int n = b + k;
-      if (n >= 0 && n < NB) s = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], UT[n], s) : s;
+      for (int grp = 0; grp < 4; grp++) {
+        const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+#pragma unroll
+        for (int q = 0; q < 4; q++) alpha[4 * grp + q] = __builtin_fmaf(cb, float((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
+      }
     }
-    T[b] = s;
   }
 #pragma unroll
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = alpha[s] > 0.f ? 1.0f / alpha[s] : 0.f;
+  // pass B: U_T = sum_s a_s h_s, V_0 = right - left for orientation 0
+  float T[NB], V[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    float u = 0.f, v = 0.f;
+    if (b < nb) {
+#pragma unroll
+      for (int grp = 0; grp < 4; grp++) {
+        const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int s = 4 * grp + q;
+          const float c = float((w >> (8 * q)) & 0xffu);
+          u = __builtin_fmaf(alpha[s], c, u);
+          v = __builtin_fmaf(s < kNumOri ? -alpha[s] : alpha[s], c, v);
+        }
+      }
+    }
+    T[b] = u;  // U_T for now, smoothed below
+    V[b] = v;
+  }
+  if (KH > 0) {
+    float U[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) U[b] = T[b];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = -KH; k <= KH; k++) {
+        const int n = b + k;
+        if (n >= 0 && n < NB) acc = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], U[n], acc) : acc;
+      }
+      T[b] = acc;
+    }
+  }
+  float rT[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) rT[b] = (b < nb && T[b] > 0.f) ? 1.0f / T[b] : 0.f;
   for (int o = 0; o < kNumOri; o++) {
     float acc = 0.f;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      float d = 0.f;
+      float d = V[b];
+      if (KH > 0) {
+        d = 0.f;
 #pragma unroll
-      for (int k = -KH; k <= KH; k++) {
-        const int n = b + k;
-        if (n >= 0 && n < NB) d = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], V[n], d) : d;
+        for (int k = -KH; k <= KH; k++) {
+          const int n = b + k;
+          if (n >= 0 && n < NB) d = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], V[n], d) : d;
+        }
       }
-      if (b < nb && d != 0.f) acc = __builtin_fmaf(d * d, __builtin_amdgcn_rcpf(T[b]), acc);
+      if (b < nb && d != 0.f) acc = __builtin_fmaf(d * d, rT[b], acc);
     }
     out[o] = 0.5f * acc;
     if (o + 1 < kNumOri) {
-      float h0[NB], h8[NB];
-      load_slice(o, h0);
-      load_slice(o + kNumOri, h8);
-      const float a0 = 2.f * alpha[o], a8 = -2.f * alpha[o + kNumOri];
+      float a0 = 0.f, a8 = 0.f;
 #pragma unroll
-      for (int b = 0; b < NB; b++) V[b] = __builtin_fmaf(a8, h8[b], __builtin_fmaf(a0, h0[b], V[b]));
+      for (int s = 0; s < kNumOri; s++) {
+        a0 = (s == o) ? alpha[s] : a0;
+        a8 = (s == o) ? alpha[s + kNumOri] : a8;
+      }
+      a0 *= 2.f;
+      a8 *= -2.f;
+      const int sh = 8 * (o & 3);
+      const uint32_t* h0 = hist + ((o >> 2) * kNbMax) * kCueLanes + lane;
+      const uint32_t* h8 = hist + (((o >> 2) + 2) * kNbMax) * kCueLanes + lane;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (b < nb) {
+          const float c0 = float((h0[b * kCueLanes] >> sh) & 0xffu);
+          const float c8 = float((h8[b * kCueLanes] >> sh) & 0xffu);
+          V[b] = __builtin_fmaf(a8, c8, __builtin_fmaf(a0, c0, V[b]));
+        }
+      }
     }
   }
 }
 
 template <int NB, int KH>
 __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radii, uint8_t* tile, uint32_t* hist,
-                            int lane, int px, int py, bool valid, float mpb[kNumOri], float gc[kNumOri]) {
+                            float* csum, int lane, int px, int py, bool valid, float mpb[kNumOri], float gc[kNumOri]) {
   const int W = P.W, H = P.H;
   const int gx0 = int(blockIdx.x) * kCueTile - kCueRmax, gy0 = int(blockIdx.y) * kCueTile - kCueRmax;
   const uint8_t* plane = P.plane[ch];
@@ -200,23 +223,31 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
     if (H == 1) gy = 0; else { const int p2 = 2 * H; gy %= p2; if (gy < 0) gy += p2; if (gy >= H) gy = p2 - 1 - gy; }
     tile[i] = plane[size_t(gy) * W + gx];
   }
-#pragma unroll
   for (int i = 0; i < 4 * kNbMax; i++) hist[i * kCueLanes + lane] = 0u;
-  __syncthreads();
   const int nb = P.nbins[ch];
   const float* g = P.smooth[KH == 8 ? 0 : (KH == 4 ? 1 : 2)];
+  if (lane < kNbMax) {
+    float s = 0.f;
+    for (int k = -KH; k <= KH; k++) {
+      const int n = lane - k;
+      if (n >= 0 && n < nb) s += g[k < 0 ? -k : k];
+    }
+    csum[lane] = (lane < nb) ? s : 0.f;
+  }
+  __syncthreads();
   const int ty = lane / kCueTile, tx = lane - ty * kCueTile;
   const uint8_t* centre = tile + (ty + kCueRmax) * kCueTileW + tx + kCueRmax;
   int i = 0;
   for (int ri = 0; ri < 3; ri++) {
     const int end = P.end_by_r[radii[ri]];
+#pragma unroll 8
     for (; i < end; i++) {
       const int s = P.slice[i];
       const uint32_t lab = centre[P.off[i]];
       atomicAdd(&hist[((s >> 2) * kNbMax + lab) * kCueLanes + lane], 1u << (8 * (s & 3)));
     }
     float chi[kNumOri];
-    chi2_snapshot<NB, KH>(hist, lane, nb, g, chi);
+    chi2_snapshot<NB, KH>(hist, lane, nb, g, csum, chi);
     const int cue = ri0 + ri;
     if (valid) {
       const size_t N = size_t(W) * H, p = size_t(py) * W + px;
@@ -234,6 +265,7 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
 __global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
   __shared__ uint8_t tile[kCueTileW * kCueTileW];
   __shared__ uint32_t hist[4 * kNbMax * kCueLanes];
+  __shared__ float csum[kNbMax];
   const int lane = threadIdx.x;
   const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = blockIdx.y * kCueTile + (lane / kCueTile);
   const bool valid = px < P.W && py < P.H;
@@ -241,10 +273,10 @@ __global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
 #pragma unroll
   for (int o = 0; o < kNumOri; o++) mpb[o] = gc[o] = 0.f;
   static constexpr int rbg[3] = {3, 5, 10}, rcg[3] = {5, 10, 20};
-  cue_channel<kLabBins, 8>(P, 0, 0, rbg, tile, hist, lane, px, py, valid, mpb, gc);
-  cue_channel<kLabBins, 4>(P, 1, 3, rcg, tile, hist, lane, px, py, valid, mpb, gc);
-  cue_channel<kLabBins, 4>(P, 2, 6, rcg, tile, hist, lane, px, py, valid, mpb, gc);
-  cue_channel<kTextonK, 0>(P, 3, 9, rcg, tile, hist, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 8>(P, 0, 0, rbg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4>(P, 1, 3, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4>(P, 2, 6, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kTextonK, 0>(P, 3, 9, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
   if (valid) {
     const size_t N = size_t(P.W) * P.H, p = size_t(py) * P.W + px;
 #pragma unroll

@@ -7,7 +7,9 @@
 // 1100-), which runs up to 6000 steps without reorthogonalisation and filters
 // spurious Ritz values on the host every 500 steps.
 //
-// Here: the exact eigenvector D^1/2 1 (lambda = 0) is locked as basis vector 0;
+// Here: Lanczos runs on the normalised Laplacian L = D^-1/2 (D - W) D^-1/2,
+// whose smallest eigenvalues are the wanted lambda.  The exact eigenvector
+// D^1/2 1 (lambda = 0) is locked as basis vector 0;
 // each step applies the FP32 stencil (SpMV), orthogonalises against the whole
 // basis by classical Gram-Schmidt run twice with FP64 accumulation (dots,
 // updates, norms), and keeps every scalar on the device, so a restart cycle of
@@ -43,21 +45,28 @@ struct SpmvOffsets {
   int o[kStencilHalf];
 };
 
-// y_p = diag_p x_p + sum_k what[k][p] x_{p+e_k} + what[k][p-e_k] x_{p-e_k}
-__global__ __launch_bounds__(256) void spmv_kernel(const float* __restrict__ what, const float* __restrict__ diag,
+// Normalised Laplacian in difference form: with u = s .* x (s = D^-1/2),
+// y_p = s_p * sum_q W_pq (u_p - u_q).  Differences of neighbouring values are
+// formed before the products, so eigenvalues near 0 keep their relative
+// accuracy in FP32 (the form I - D^-1/2 W D^-1/2 would lose it to cancellation
+// against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
+// W[k][p - e_k] for q = p - e_k.
+__global__ __launch_bounds__(256) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                    const float* __restrict__ x, float* __restrict__ y, int W, int H,
                                                    int Wp, size_t Np, SpmvOffsets A) {
   const int xi = blockIdx.x * blockDim.x + threadIdx.x, yi = blockIdx.y * blockDim.y + threadIdx.y;
   if (xi >= W || yi >= H) return;
   const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
-  float acc = diag[p] * x[p];
+  const float sp = s[p];
+  const float up = sp * x[p];
+  float acc = 0.f;
 #pragma unroll
   for (int j = 0; j < kStencilHalf; j++) {
     const int o = A.o[j];
-    acc = __builtin_fmaf(what[size_t(j) * Np + p], x[p + o], acc);
-    acc = __builtin_fmaf(what[size_t(j) * Np + p - o], x[p - o], acc);
+    acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - s[p + o] * x[p + o], acc);
+    acc = __builtin_fmaf(wup[size_t(j) * Np + p - o], up - s[p - o] * x[p - o], acc);
   }
-  y[p] = acc;
+  y[p] = sp * acc;
 }
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) {
@@ -65,7 +74,7 @@ void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) {
   for (int j = 0; j < kStencilHalf; j++) A.o[j] = st.off_host[j];
   dim3 block(64, 4), grid(div_up(st.W, 64), div_up(st.H, 4));
   KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
-  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.what, st.diag, x, y, st.W, st.H, st.Wp, st.Np, A);
+  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Np, A);
   GPB_LAUNCH_CHECK();
 }
 
@@ -288,7 +297,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   red.dots(V, Np, 0, V, nullptr, true, h);
   hipLaunchKernelGGL(scale_by_kernel, dim3(vg), dim3(256), 0, c.stream, V, h, Np);
   GPB_LAUNCH_CHECK();
-  // T00 = V0' M V0
+  // T00 = V0' L V0
   launch_spmv(c, st, V, w);
   red.dots(V, Np, 1, w, nullptr, false, h);
   hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, h, 0, h, beta);
@@ -342,7 +351,17 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     std::vector<double> A = Th;
-    symeig_desc(m, A, th, Y);
+    {
+      // smallest Ritz values first
+      std::vector<double> thd, Yd;
+      symeig_desc(m, A, thd, Yd);
+      th.assign(size_t(m), 0.0);
+      Y.assign(size_t(m) * m, 0.0);
+      for (int i = 0; i < m; i++) {
+        th[size_t(i)] = thd[size_t(m - 1 - i)];
+        for (int j = 0; j < m; j++) Y[size_t(j) * m + i] = Yd[size_t(j) * m + (m - 1 - i)];
+      }
+    }
     const int nk = keep;
     std::vector<double> Yk(size_t(m) * nk);
     for (int j = 0; j < m; j++)
@@ -383,7 +402,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   stats.converged = 0;
   for (int i = 0; i < K; i++) stats.converged += resid[i] <= opt.tol;
   for (int i = 0; i < K; i++) {
-    evals[i] = 1.0 - th[size_t(i)];
+    evals[i] = th[size_t(i)];
     hipLaunchKernelGGL(to_general_kernel, dim3(vg), dim3(256), 0, c.stream, V + size_t(i) * Np, st.s,
                        d_evecs + size_t(i) * Np, Np);
     GPB_LAUNCH_CHECK();
