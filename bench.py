@@ -120,7 +120,8 @@ def main():
                 config=dict(workload=f"plain gPb, {S}x{S} seeded Voronoi ({args.regions} regions, sigma 8, seed 2), "
                                      f"9 eigenvectors, residual <= {tol:g}",
                             form="plain", lanczos_basis=st["basis"], lanczos_matvecs=st["matvecs"],
-                            lanczos_restarts=st["restarts"], max_residual=st["max_residual"],
+                            lanczos_restarts=st["restarts"], reorth_second_passes=st["second_passes"],
+                            max_residual=st["max_residual"],
                             stage_ms=[round(x, 2) for x in st["stage_ms"]]),
                 roofline=roofline,
                 kernels={k: dict(ms=round(v["ms"] / args.steps, 3), launches=v["launches"] // args.steps,

@@ -42,6 +42,7 @@ typedef struct gpb_stats {
   double max_residual;  /* max ||(D-W)v - lambda D v|| / ||D v|| over returned pairs */
   double evals[16];     /* generalised eigenvalues, ascending */
   int fallback;         /* 1 if a warm start was rejected and plain Lanczos ran */
+  int reorth_second_passes; /* Gram-Schmidt steps that needed a second pass */
 } gpb_stats;
 
 /* Context lifetime.  Replaces the per-call device selection and

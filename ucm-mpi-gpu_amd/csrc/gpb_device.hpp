@@ -86,6 +86,8 @@ struct LanczosStats {
   int converged = 0;        // number of converged pairs
   double max_residual = 0;  // max relative generalized residual of returned pairs
   int fallback = 0;         // warm start rejected, plain solve used
+  int second_passes = 0;    // Gram-Schmidt steps that needed a second pass
+  double pass2_bytes[2] = {0, 0};
 };
 
 // Per-kernel-class timing with HIP events on the context stream (enabled by

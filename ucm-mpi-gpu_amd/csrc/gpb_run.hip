@@ -186,6 +186,7 @@ int gpb_get_stats(gpb_ctx* ctx, gpb_stats* out) {
     out->converged = ctx->c.last_lanczos.converged;
     out->max_residual = ctx->c.last_lanczos.max_residual;
     out->fallback = ctx->c.last_lanczos.fallback;
+    out->reorth_second_passes = ctx->c.last_lanczos.second_passes;
     for (int i = 0; i < 16; i++) out->evals[i] = ctx->c.last_evals[i];
   });
 }

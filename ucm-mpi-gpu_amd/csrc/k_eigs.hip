@@ -31,6 +31,8 @@ void symeig_desc(int n, std::vector<double>& a, std::vector<double>& w, std::vec
 constexpr int kRedThreads = 256;
 constexpr int kMaxBasis = 128;
 
+__global__ void fill_int1_kernel(int* p, int v) { *p = v; }
+
 __global__ void fill_f32_kernel(float* __restrict__ p, size_t n, float v) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
 }
@@ -50,124 +52,184 @@ struct SpmvOffsets {
 // formed before the products, so eigenvalues near 0 keep their relative
 // accuracy in FP32 (the form I - D^-1/2 W D^-1/2 would lose it to cancellation
 // against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
-// W[k][p - e_k] for q = p - e_k.
-__global__ __launch_bounds__(256) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
-                                                   const float* __restrict__ x, float* __restrict__ y, int W, int H,
-                                                   int Wp, size_t Np, SpmvOffsets A) {
-  const int xi = blockIdx.x * blockDim.x + threadIdx.x, yi = blockIdx.y * blockDim.y + threadIdx.y;
-  if (xi >= W || yi >= H) return;
-  const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
-  const float sp = s[p];
-  const float up = sp * x[p];
-  float acc = 0.f;
-#pragma unroll
-  for (int j = 0; j < kStencilHalf; j++) {
-    const int o = A.o[j];
-    acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - s[p + o] * x[p + o], acc);
-    acc = __builtin_fmaf(wup[size_t(j) * Np + p - o], up - s[p - o] * x[p - o], acc);
+// W[k][p - e_k] for q = p - e_k.  A 64x16 tile of u (plus a 5-pixel halo) is
+// staged in LDS; W streams from HBM with coalesced row reads.
+constexpr int kSpmvTX = 64, kSpmvTY = 16, kSpmvRows = 4;  // 64x4 threads, 4 rows each
+constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+
+struct SpmvArgs {
+  int o[kStencilHalf];   // padded linear offsets
+  int lo[kStencilHalf];  // LDS tile offsets
+};
+
+__global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
+                                                           const float* __restrict__ x, float* __restrict__ y, int W,
+                                                           int H, int Wp, int Hp, size_t Np, SpmvArgs A) {
+  __shared__ float u[kSpmvTileH * kSpmvTileW];
+  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
+  const int x0 = blockIdx.x * kSpmvTX, y0 = blockIdx.y * kSpmvTY;  // interior coordinates of the tile origin
+  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
+    const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
+    const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
+    float v = 0.f;
+    if (py < Hp && px < Wp) {
+      const size_t q = size_t(py) * Wp + px;
+      v = s[q] * x[q];
+    }
+    u[i] = v;
   }
-  y[p] = sp * acc;
+  __syncthreads();
+  const int xi = x0 + threadIdx.x;
+  if (xi >= W) return;
+#pragma unroll
+  for (int r = 0; r < kSpmvRows; r++) {
+    const int ty = threadIdx.y + 4 * r;
+    const int yi = y0 + ty;
+    if (yi >= H) break;
+    const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
+    const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
+    const float up = u[c];
+    float acc = 0.f;
+#pragma unroll
+    for (int j = 0; j < kStencilHalf; j++) {
+      acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - u[c + A.lo[j]], acc);
+      acc = __builtin_fmaf(wup[size_t(j) * Np + p - A.o[j]], up - u[c - A.lo[j]], acc);
+    }
+    y[p] = s[p] * acc;
+  }
 }
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) {
-  SpmvOffsets A;
-  for (int j = 0; j < kStencilHalf; j++) A.o[j] = st.off_host[j];
-  dim3 block(64, 4), grid(div_up(st.W, 64), div_up(st.H, 4));
+  SpmvArgs A;
+  int dx[kStencilHalf], dy[kStencilHalf];
+  stencil_upper_offsets(dx, dy);
+  for (int j = 0; j < kStencilHalf; j++) {
+    A.o[j] = st.off_host[j];
+    A.lo[j] = dy[j] * kSpmvTileW + dx[j];
+  }
+  dim3 block(kSpmvTX, 4), grid(div_up(st.W, kSpmvTX), div_up(st.H, kSpmvTY));
   KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
-  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Np, A);
+  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, A);
   GPB_LAUNCH_CHECK();
 }
 
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+// Sum of 4 values over the 256-thread block; result valid in thread 0.
+__device__ __forceinline__ void block_sum4(double v[4], double* sh /* [4][4] */) {
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_down(v[k], o, 64);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) sh[wv * 4 + k] = v[k];
   __syncthreads();
-  if (lane == 0) sh[wv] = v;
-  __syncthreads();
-  double r = 0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < int(blockDim.x >> 6); i++) r += sh[i];
-  return r;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = sh[k] + sh[4 + k] + sh[8 + k] + sh[12 + k];
+  __syncthreads();
 }
 
-// partial[i][blk] = sum over the block's chunk of V_i . w (FP64), i < n;
-// V rows are ld apart.  If wsq, partial[n][blk] = w . w (with weights if D).
-__global__ __launch_bounds__(kRedThreads) void multidot_kernel(const float* __restrict__ V, size_t ld, int n,
-                                                               const float* __restrict__ w, const float* __restrict__ D,
-                                                               size_t Np, int wsq, double* __restrict__ partial) {
-  __shared__ double sh[kRedThreads / 64];
-  const size_t per = (Np + gridDim.x - 1) / gridDim.x;
-  const size_t lo = size_t(blockIdx.x) * per, hi = lo + per < Np ? lo + per : Np;
-  for (int i0 = 0; i0 < n; i0 += 4) {
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const float* v0 = V + size_t(i0) * ld;
-    for (size_t p = lo + threadIdx.x; p < hi; p += kRedThreads) {
-      const double wv = w[p];
-      a0 += double(v0[p]) * wv;
-      if (i0 + 1 < n) a1 += double(v0[ld + p]) * wv;
-      if (i0 + 2 < n) a2 += double(v0[2 * ld + p]) * wv;
-      if (i0 + 3 < n) a3 += double(v0[3 * ld + p]) * wv;
-    }
-    double r;
-    r = block_sum(a0, sh);
-    if (threadIdx.x == 0) partial[size_t(i0) * gridDim.x + blockIdx.x] = r;
-    if (i0 + 1 < n) {
-      r = block_sum(a1, sh);
-      if (threadIdx.x == 0) partial[size_t(i0 + 1) * gridDim.x + blockIdx.x] = r;
-    }
-    if (i0 + 2 < n) {
-      r = block_sum(a2, sh);
-      if (threadIdx.x == 0) partial[size_t(i0 + 2) * gridDim.x + blockIdx.x] = r;
-    }
-    if (i0 + 3 < n) {
-      r = block_sum(a3, sh);
-      if (threadIdx.x == 0) partial[size_t(i0 + 3) * gridDim.x + blockIdx.x] = r;
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  double a[4] = {v, 0, 0, 0};
+  block_sum4(a, sh);
+  return a[0];
+}
+
+
+// Dot products as a 2-D grid: block (c, i) reduces V_i . w over element chunk
+// c (kDotChunk elements, 16 per thread) and writes partial[i][c]; column n is
+// w . (D) w when wsq.  Every block streams one contiguous chunk of one vector
+// and reduces once, so the grid saturates HBM; w is re-read from L2.
+constexpr int kDotChunk = 4096;
+
+__global__ __launch_bounds__(256) void multidot2_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                        const float* __restrict__ w, const float* __restrict__ D,
+                                                        size_t Np, double* __restrict__ partial) {
+  __shared__ double sh[16];
+  const int i = blockIdx.y;
+  const size_t lo = size_t(blockIdx.x) * kDotChunk;
+  const float* v = (i < n) ? V + size_t(i) * ld : w;
+  const bool weighted = (i == n) && D;
+  double acc = 0;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < kDotChunk; e += 256) {
+    const size_t p = lo + e;
+    if (p < Np) {
+      const double x = v[p], y = w[p];
+      acc += (weighted ? double(D[p]) : 1.0) * x * y;
     }
   }
-  if (wsq) {
-    double a = 0;
-    for (size_t p = lo + threadIdx.x; p < hi; p += kRedThreads) {
-      const double wv = w[p];
-      a += (D ? double(D[p]) : 1.0) * wv * wv;
-    }
-    const double r = block_sum(a, sh);
-    if (threadIdx.x == 0) partial[size_t(n) * gridDim.x + blockIdx.x] = r;
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[size_t(i) * gridDim.x + blockIdx.x] = r;
+}
+
+// w <- w - sum_i h_i V_i (FP64 accumulation), grid-stride; per-block partial
+// of ||w||^2 into nrm_partial[blk] when non-null.
+__global__ __launch_bounds__(256) void update_gs_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                        const double* __restrict__ h, float* __restrict__ w, size_t Np,
+                                                        double* __restrict__ nrm_partial) {
+  __shared__ double hs[kMaxBasis];
+  __shared__ double sh[16];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) hs[i] = h[i];
+  __syncthreads();
+  double nrm = 0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    double acc = w[p];
+    for (int i = 0; i < n; i++) acc -= hs[i] * double(V[size_t(i) * ld + p]);
+    const float f = float(acc);
+    w[p] = f;
+    nrm += double(f) * double(f);
+  }
+  if (nrm_partial) {
+    const double r = block_sum(nrm, sh);
+    if (threadIdx.x == 0) nrm_partial[blockIdx.x] = r;
   }
 }
 
-// out[i] = sum_blk partial[i][blk]  (one block per i, fixed order)
+// out[i] = sum_blk partial[i][blk]  (one block per i, fixed order); gated.
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const double* __restrict__ partial, int G,
-                                                             double* __restrict__ out) {
-  __shared__ double sh[kRedThreads / 64];
+                                                             double* __restrict__ out, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[16];
   double a = 0;
   for (int b = threadIdx.x; b < G; b += kRedThreads) a += partial[size_t(blockIdx.x) * G + b];
   const double r = block_sum(a, sh);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
-// w <- w - sum_i h_i V_i  (FP64 accumulation), over the whole padded range
-__global__ __launch_bounds__(256) void update_kernel(const float* __restrict__ V, size_t ld, int n,
-                                                     const double* __restrict__ h, float* __restrict__ w, size_t Np) {
-  __shared__ double hs[kMaxBasis];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) hs[i] = h[i];
-  __syncthreads();
-  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
-    double acc = w[p];
-    for (int i = 0; i < n; i++) acc -= hs[i] * double(V[size_t(i) * ld + p]);
-    w[p] = float(acc);
+
+
+// After the first Gram-Schmidt pass the correction of the second pass is
+// applied only when the norm dropped below 1/sqrt(2) of its value before the
+// pass (Daniel, Gragg, Kaufman and Stewart 1976).  The second pass's
+// projections are always computed, fused into the first update.
+// acc[0], acc[1]: algorithmic bytes of the gated dots / update launches that
+// run; acc[2]: number of second passes.
+__global__ void reorth_decide_kernel(const double* __restrict__ w0sq, const double* __restrict__ w1sq,
+                                     int* __restrict__ flag, int n, double np, double* __restrict__ acc) {
+  const int f = (*w1sq < 0.5 * *w0sq) ? 1 : 0;
+  *flag = f;
+  if (f) {
+    acc[1] += np * (4.0 * n + 8);
+    acc[2] += 1.0;
   }
 }
 
-// T[i][j] += h[i] (and T[j][i]) for i <= j; on the final pass also beta.
-__global__ void tcol_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ h, int final_pass,
-                            const double* __restrict__ nrm, double* __restrict__ beta) {
+// T[:, j] = h1 + (second pass ? h2 : 0) (mirrored into row j); beta_j from
+// the norm after the last pass that ran.
+__global__ void tcol_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ h1,
+                            const double* __restrict__ h2, const int* __restrict__ flag,
+                            const double* __restrict__ nrm1, const double* __restrict__ nrm2,
+                            double* __restrict__ beta) {
   const int i = threadIdx.x;
+  const bool two = flag && *flag;
   if (i <= j) {
-    T[size_t(i) * m + j] += h[i];
-    if (i != j) T[size_t(j) * m + i] = T[size_t(i) * m + j];
+    const double v = T[size_t(i) * m + j] + h1[i] + (two ? h2[i] : 0.0);
+    T[size_t(i) * m + j] = v;
+    if (i != j) T[size_t(j) * m + i] = v;
   }
-  if (final_pass && i == 0) {
-    double b = sqrt(nrm[0]);
+  if (i == 0 && beta) {
+    const double b = sqrt(two ? *nrm2 : *nrm1);
     beta[j] = b;
     beta[kMaxBasis + j] = b > 1e-300 ? 1.0 / b : 0.0;
   }
@@ -247,21 +309,39 @@ namespace {
 struct Reducer {
   Context& c;
   size_t Np;
-  int G;
+  int NW;   // element chunks of the dot kernel
+  int GU;   // blocks of the update kernel
   double* partial;
-  double* out;
+  double* npart;
   Reducer(Context& cc, size_t np) : c(cc), Np(np) {
-    G = int(std::min<size_t>(1024, std::max<size_t>(1, (np + 4095) / 4096)));
-    partial = c.ws<double>("eig.partial", size_t(kMaxBasis + 2) * G);
-    out = c.ws<double>("eig.redout", kMaxBasis + 2);
+    NW = int((np + kDotChunk - 1) / kDotChunk);
+    GU = int(std::min<size_t>(2048, std::max<size_t>(1, (np + 255) / 256)));
+    partial = c.ws<double>("eig.partial", size_t(kMaxBasis + 2) * NW);
+    npart = c.ws<double>("eig.npart", size_t(GU));
   }
-  // out[0..n) = V_i . w, out[n] = w . (D) w if wsq
+  // dst[0..n) = V_i . w, dst[n] = w . (D) w if wsq
   void dots(const float* V, size_t ld, int n, const float* w, const float* D, bool wsq, double* dst) {
-    KScope ks(c, kKDots, double(Np) * (4.0 * n + 4 + (wsq && D ? 4 : 0)));
-    hipLaunchKernelGGL(multidot_kernel, dim3(G), dim3(kRedThreads), 0, c.stream, V, ld, n, w, D, Np, int(wsq), partial);
+    {
+      KScope ks(c, kKDots, double(Np) * (4.0 * n + 4 + (wsq && D ? 4 : 0)));
+      hipLaunchKernelGGL(multidot2_kernel, dim3(NW, n + (wsq ? 1 : 0)), dim3(256), 0, c.stream, V, ld, n, w, D, Np,
+                         partial);
+      GPB_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(reduce_kernel, dim3(n + (wsq ? 1 : 0)), dim3(kRedThreads), 0, c.stream, partial, NW, dst, nullptr);
     GPB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_kernel, dim3(n + (wsq ? 1 : 0)), dim3(kRedThreads), 0, c.stream, partial, G, dst);
-    GPB_LAUNCH_CHECK();
+  }
+  // w -= V h; nrm_dst = ||w||^2 afterwards (if non-null)
+  void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst) {
+    {
+      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8));
+      hipLaunchKernelGGL(update_gs_kernel, dim3(GU), dim3(256), 0, c.stream, V, ld, n, h, w, Np,
+                         nrm_dst ? npart : nullptr);
+      GPB_LAUNCH_CHECK();
+    }
+    if (nrm_dst) {
+      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, npart, GU, nrm_dst, nullptr);
+      GPB_LAUNCH_CHECK();
+    }
   }
 };
 
@@ -285,6 +365,16 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   float* w = c.ws<float>("eig.w", Np);
   double* T = c.ws<double>("eig.T", size_t(m) * m);
   double* h = c.ws<double>("eig.h", kMaxBasis + 2);
+  double* h1 = c.ws<double>("eig.h1", kMaxBasis + 2);
+  double* h2 = c.ws<double>("eig.h2", kMaxBasis + 2);
+  double* nrm = c.ws<double>("eig.nrm", 2);
+  int* flag = c.ws<int>("eig.flag", 1);
+  double* gacc = c.ws<double>("eig.gacc", 3);
+  GPB_HIP(hipMemsetAsync(gacc, 0, 3 * sizeof(double), c.stream));
+  double gacc_h[3] = {0, 0, 0};
+  int* one_flag = c.ws<int>("eig.one", 1);
+  hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, one_flag, 1);
+  GPB_LAUNCH_CHECK();
   double* beta = c.ws<double>("eig.beta", 2 * kMaxBasis);
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   Reducer red(c, Np);
@@ -300,7 +390,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   // T00 = V0' L V0
   launch_spmv(c, st, V, w);
   red.dots(V, Np, 1, w, nullptr, false, h);
-  hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, h, 0, h, beta);
+  hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, h, h, nullptr, h, h, nullptr);
   GPB_LAUNCH_CHECK();
   // V1: random, orthogonal to V0, normalised
   float* V1 = V + Np;
@@ -309,8 +399,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   GPB_LAUNCH_CHECK();
   for (int pass = 0; pass < 2; pass++) {
     red.dots(V, Np, 1, V1, nullptr, false, h);
-    hipLaunchKernelGGL(update_kernel, dim3(vg), dim3(256), 0, c.stream, V, Np, 1, h, V1, Np);
-    GPB_LAUNCH_CHECK();
+    red.update(V, Np, 1, h, V1, nullptr);
   }
   red.dots(V, Np, 0, V1, nullptr, true, h);
   hipLaunchKernelGGL(scale_by_kernel, dim3(vg), dim3(256), 0, c.stream, V1, h, Np);
@@ -328,21 +417,17 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   while (true) {
     for (int j = k; j < m; j++) {
       float* vj = V + size_t(j) * Np;
+      const int n = j + 1;
       launch_spmv(c, st, vj, w);
       matvecs++;
-      for (int pass = 0; pass < 2; pass++) {
-        const bool last = pass == 1;
-        red.dots(V, Np, j + 1, w, nullptr, false, h);
-        {
-          KScope ks(c, kKUpdate, double(Np) * (4.0 * (j + 1) + 8));
-          hipLaunchKernelGGL(update_kernel, dim3(vg), dim3(256), 0, c.stream, V, Np, j + 1, h, w, Np);
-          GPB_LAUNCH_CHECK();
-        }
-        if (last) red.dots(V, Np, 0, w, nullptr, true, h + (kMaxBasis + 1));
-        hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, h, int(last),
-                           h + (kMaxBasis + 1), beta);
-        GPB_LAUNCH_CHECK();
-      }
+      // classical Gram-Schmidt twice ("twice is enough"), FP64 accumulation
+      red.dots(V, Np, n, w, nullptr, false, h1);
+      red.update(V, Np, n, h1, w, nullptr);
+      red.dots(V, Np, n, w, nullptr, false, h2);
+      red.update(V, Np, n, h2, w, nrm + 1);
+      hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, h1, h2, one_flag, nrm + 1, nrm + 1,
+                         beta);
+      GPB_LAUNCH_CHECK();
       hipLaunchKernelGGL(scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np);
       GPB_LAUNCH_CHECK();
     }
@@ -379,8 +464,16 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     for (int i = 0; i < K; i++) red.dots(V2, Np, 0, V2 + size_t(i) * Np, st.D, true, h + i);
     red.dots(V2, Np, 0, V2 + size_t(nk) * Np, st.D, true, h + K);
     GPB_HIP(hipMemcpyAsync(nrm.data(), h, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(gacc_h, gacc, sizeof(gacc_h), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     c.kt.fold();
+    if (c.kt.on) {
+      c.kt.bytes[kKDots] += gacc_h[0] - stats.pass2_bytes[0];
+      c.kt.bytes[kKUpdate] += gacc_h[1] - stats.pass2_bytes[1];
+    }
+    stats.pass2_bytes[0] = gacc_h[0];
+    stats.pass2_bytes[1] = gacc_h[1];
+    stats.second_passes = int(gacc_h[2]);
     std::swap(V, V2);
     const double bm = betah[m - 1];
     double worst = 0;

@@ -61,7 +61,8 @@ _sig("gpb_ic_program", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, np.ctypesli
 class Stats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 6), ("lanczos_matvecs", C.c_int), ("lanczos_restarts", C.c_int),
                 ("lanczos_basis", C.c_int), ("lanczos_block", C.c_int), ("converged", C.c_int),
-                ("max_residual", C.c_double), ("evals", C.c_double * 16), ("fallback", C.c_int)]
+                ("max_residual", C.c_double), ("evals", C.c_double * 16), ("fallback", C.c_int),
+                ("reorth_second_passes", C.c_int)]
 
 
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
@@ -191,7 +192,8 @@ class Context:
         self._chk(lib.gpb_get_stats(self.h, C.byref(s)))
         return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
                     basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
-                    evals=list(s.evals), fallback=s.fallback)
+                    evals=list(s.evals), fallback=s.fallback,
+                    second_passes=s.reorth_second_passes)
 
     def build_w(self, mpb_nmax: np.ndarray):
         m = np.ascontiguousarray(mpb_nmax, np.float32)
