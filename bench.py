@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--regions", type=int, default=400)
     ap.add_argument("--cpu-crop", type=int, default=224)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=1, help="Lanczos block size (1 = single-vector)")
+    ap.add_argument("--basis", type=int, default=0)
+    ap.add_argument("--keep", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the gPb path runs one image (or one patch batch) per GPU, and this "
@@ -81,7 +84,10 @@ def main():
     torch.cuda.synchronize()
 
     ctx = gpb.Context(0)
-    ctx.set_eig_options(nvec=9, basis=48, keep=24, tol=tol, max_matvecs=40000)
+    basis = args.basis or (48 if args.block == 1 else 96)
+    keep = args.keep or (24 if args.block == 1 else 48)
+    ctx.set_eig_options(nvec=9, basis=basis, keep=keep, tol=tol, max_matvecs=40000)
+    ctx.set_block_size(args.block)
 
     def step():
         ctx.compute_device(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), d_gmax.data_ptr(), d_gall.data_ptr())
@@ -119,7 +125,8 @@ def main():
                 data="synthetic",
                 config=dict(workload=f"plain gPb, {S}x{S} seeded Voronoi ({args.regions} regions, sigma 8, seed 2), "
                                      f"9 eigenvectors, residual <= {tol:g}",
-                            form="plain", lanczos_basis=st["basis"], lanczos_matvecs=st["matvecs"],
+                            form="plain", lanczos_block=st["block"], lanczos_basis=st["basis"],
+                            lanczos_matvecs=st["matvecs"],
                             lanczos_restarts=st["restarts"], reorth_second_passes=st["second_passes"],
                             max_residual=st["max_residual"],
                             stage_ms=[round(x, 2) for x in st["stage_ms"]]),

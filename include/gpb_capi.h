@@ -114,6 +114,13 @@ int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float*
  * (damascene.cu:307-309) and nvec of spectralPb (src/globalPb.cpp:65). */
 int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol, int max_matvecs);
 
+/* Lanczos block size for later calls on ctx: 1 = single-vector thick-restart
+ * Lanczos (default); 4, 8, 12 or 16 = block thick-restart Lanczos, which
+ * applies W to the whole block with one read of the stencil.  No counterpart
+ * in the reference (its solvers are single-vector, lanczos.cu:192-282,
+ * SymEigsSolver.h:200-280). */
+int gpb_set_block_size(gpb_ctx* ctx, int block);
+
 /* Kernel classes for gpb_kernel_time. */
 #define GPB_K_TEXTON 0
 #define GPB_K_CUES 1
@@ -187,6 +194,14 @@ int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_pa
  * counterpart in the reference (it recomputes the walks per pixel,
  * buildW/src/ic.cc:375-560); exported for CPU-side cross-checks. */
 int gpb_ic_program(int left, int right, int top, int bottom, int8_t* conn, int8_t* sites, int max_sites);
+
+/* y[c] = D^-1/2 (D - W) D^-1/2 x[c] for ncols columns (x, y: [ncols][N]),
+ * with W built from mpb_nmax; block = 1 applies the single-vector kernel per
+ * column, block = ncols (4, 8 or 16) the block kernel.  The operator whose
+ * smallest eigenpairs spectralPb needs (src/spectralPb.cpp:186-234);
+ * exported for checks of the eigensolver's operator. */
+int gpb_apply_laplacian(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, int ncols, int block,
+                        const float* x, float* y);
 
 #ifdef __cplusplus
 }

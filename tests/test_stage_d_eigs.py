@@ -53,16 +53,24 @@ def test_oracle_eigs_vs_lapack(gpb, orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("W,H,seed,tol", [(24, 20, 3, 1e-6), (64, 48, 4, 1e-5)])
-def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol):
-    print(f"eigs parity {W}x{H} seed {seed} tol {tol}", flush=True)
+@pytest.mark.parametrize("W,H,seed,tol,block", [(24, 20, 3, 1e-6, 1), (64, 48, 4, 1e-5, 1), (24, 20, 3, 1e-6, 8),
+                                                (256, 224, 4, 1e-5, 8), (160, 120, 5, 1e-5, 4)])
+def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol, block):
+    print(f"eigs parity {W}x{H} seed {seed} tol {tol} block {block}", flush=True)
     nm = _tiny_w(orc, gpb, W, H, seed)
     Wd, D = orc.build_w(nm)
     ev_ref, vec_ref, _ = orc.eigs(Wd, D, K=10, tol=1e-12)
-    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
-    ev, vec = ctx.eigs(nm.astype(np.float32))
+    ctx.set_eig_options(nvec=9, basis=40 if block == 1 else 192, keep=20 if block == 1 else 64, tol=tol)
+    ctx.set_block_size(block)
+    try:
+        ev, vec = ctx.eigs(nm.astype(np.float32))
+    finally:
+        ctx.set_block_size(1)
     st = ctx.stats()
-    print(f"  matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e}", flush=True)
+    print(f"  matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e} "
+          f"block {st['block']} fallback {st['fallback']}", flush=True)
+    if block > 1 and W * H >= 64 * 192:  # size gate of the library
+        assert st["block"] == block and st["fallback"] == 0
     assert st["max_residual"] <= tol
     rel = np.abs(ev[1:] - ev_ref[1:9]) / ev_ref[1:9]
     print(f"  max eigenvalue rel err {rel.max():.2e}\n  gpu {ev}\n  ref {ev_ref}", flush=True)
@@ -74,3 +82,25 @@ def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol):
         assert ang <= 1e-3
     for i in range(9):  # the returned pairs meet the target when recomputed in FP64
         assert orc.gen_residual(Wd, D, ev[i], vec[i].astype(np.float64)) <= 2 * tol + 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [1, 4, 8, 16])
+def test_laplacian_operator_gpu(gpb, orc, ctx, block):
+    """The stencil operator (single and block kernels) against the oracle's W."""
+    print(f"laplacian operator block {block}", flush=True)
+    H, W = 45, 70
+    rng = np.random.default_rng(block)
+    m = (rng.random((H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float32)
+    Wd, D = orc.build_w(m.astype(np.float64))
+    A = orc.dense_w(Wd)
+    d = D.reshape(-1)
+    s = 1 / np.sqrt(d)
+    L = np.eye(H * W) - (s[:, None] * A * s[None, :])
+    ncols = max(block, 2)
+    x = rng.normal(size=(ncols, H, W)).astype(np.float32)
+    y = ctx.apply_laplacian(m, x, block=block if block > 1 else 1)
+    ref = (L @ x.reshape(ncols, -1).T.astype(np.float64)).T.reshape(ncols, H, W)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err < 1e-5

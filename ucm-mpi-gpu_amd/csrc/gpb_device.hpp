@@ -155,6 +155,7 @@ struct Context {
   int eig_keep = 24;
   double eig_tol = 1e-3;
   int eig_max_matvecs = 20000;
+  int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {

@@ -8,6 +8,9 @@ namespace gpb {
 struct NoConvergence : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+struct BlockBreakdown : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 // stage (a)
 void upload_schmid(Context& c);
@@ -46,6 +49,22 @@ struct EigOptions {
 // padded layout, generalised normalisation v^T D v = 1.
 void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, double* evals, float* d_evecs,
                    LanczosStats& stats);
+// block form: nvec > 1 columns per Lanczos step (block thick-restart Lanczos);
+// init_block (optional, [block][Np] padded, generalised or normalised-Laplacian
+// vectors in u-space) replaces the random starting block.
+struct BlockOptions {
+  int block = 8;
+  const float* init_block = nullptr;
+};
+void block_lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, const BlockOptions& bo, double* evals,
+                         float* d_evecs, LanczosStats& stats);
+void debug_block_spmv(Context& c, const Stencil& st, int B, const float* X, float* Y);
+// helpers shared by the solvers
+void launch_reduce(Context& c, const double* partial, int G, int n, double* dst);
+void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np);
+void launch_locked_start(Context& c, const Stencil& st, float* v0, float* v1_block, int B, size_t ld, uint64_t seed);
+void launch_to_general(Context& c, const float* u, const float* s, float* v, size_t Np);
+void launch_scale_by(Context& c, float* v, const double* nrm2, size_t Np);
 // y = D^-1/2 (D - W) D^-1/2 x on padded vectors
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y);
 

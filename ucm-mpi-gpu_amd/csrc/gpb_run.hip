@@ -53,6 +53,8 @@ static PipelineBuffers pipeline_buffers(Context& c, int W, int H, int K, bool wa
   return b;
 }
 
+void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* evals, float* d_evecs);
+
 // Runs the whole path; outputs are device pointers (any may be null).
 static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo, bool want_cues, float* d_gmax,
                          float* d_gall, float* d_orient, double* evals, PipelineBuffers* keep_buffers) {
@@ -70,7 +72,7 @@ static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo
   Stencil st;
   build_stencil(c, b.nmax, W, H, st);
   GPB_HIP(hipEventRecord(se.ev[4], c.stream));
-  lanczos_solve(c, st, eo, evals, b.evecs, c.last_lanczos);
+  solve_eigs(c, st, eo, evals, b.evecs);
   GPB_HIP(hipEventRecord(se.ev[5], c.stream));
   launch_spb(c, st, b.evecs, evals, eo.K, b.spb);
   launch_gpb(c, W, H, b.gcue, b.spb, b.nmax, d_orient, d_gmax, d_gall);
@@ -83,6 +85,28 @@ static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo
     c.stage_ms[s] = ms;
   }
   if (keep_buffers) *keep_buffers = b;
+}
+
+// Dispatch to the single-vector or the block solver (gpb_set_block_size).
+// Images with fewer than 64 pixels per basis vector use the single-vector
+// solver, and so does any solve whose block panel turns rank-deficient
+// (stats.fallback = 2).
+void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* evals, float* d_evecs) {
+  const size_t N = size_t(st.W) * st.H;
+  if (c.eig_block <= 1 || N < size_t(64) * eo.m) {
+    lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
+    return;
+  }
+  BlockOptions bo;
+  bo.block = c.eig_block;
+  try {
+    block_lanczos_solve(c, st, eo, bo, evals, d_evecs, c.last_lanczos);
+  } catch (const BlockBreakdown&) {
+    const int spent = c.last_lanczos.iterations;
+    lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
+    c.last_lanczos.fallback = 2;
+    c.last_lanczos.iterations += spent;
+  }
 }
 
 static EigOptions ctx_eig_options(const Context& c) {
@@ -143,7 +167,7 @@ extern "C" {
 
 int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol, int max_matvecs) {
   return guarded2(ctx, [&] {
-    if (nvec < 2 || nvec > 16 || basis < nvec + 2 || basis > 127 || keep < nvec || keep >= basis || !(tol > 0) ||
+    if (nvec < 2 || nvec > 16 || basis < nvec + 2 || basis > 320 || keep < nvec || keep >= basis || !(tol > 0) ||
         max_matvecs < basis)
       throw ArgError("invalid eigensolver options");
     ctx->c.eig_K = nvec;
@@ -151,6 +175,14 @@ int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol,
     ctx->c.eig_keep = keep;
     ctx->c.eig_tol = tol;
     ctx->c.eig_max_matvecs = max_matvecs;
+  });
+}
+
+int gpb_set_block_size(gpb_ctx* ctx, int block) {
+  return guarded2(ctx, [&] {
+    if (!(block == 1 || block == 4 || block == 8 || block == 12 || block == 16))
+      throw ArgError("block size must be 1, 4, 8, 12 or 16");
+    ctx->c.eig_block = block;
   });
 }
 
@@ -219,7 +251,7 @@ int gpb_eigs(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, double*
     Stencil st;
     build_stencil(c, d_m, width, height, st);
     float* d_ev = c.ws<float>("io.evecs", size_t(eo.K) * st.Np);
-    lanczos_solve(c, st, eo, evals, d_ev, c.last_lanczos);
+    solve_eigs(c, st, eo, evals, d_ev);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = evals[i];
     if (evecs) unpad_to_host(c, d_ev, eo.K, width, height, evecs);
     GPB_HIP(hipStreamSynchronize(c.stream));
@@ -239,7 +271,7 @@ int gpb_spectral_pb(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, 
     build_stencil(c, d_m, width, height, st);
     float* d_ev = c.ws<float>("io.evecs", size_t(eo.K) * st.Np);
     double ev[16];
-    lanczos_solve(c, st, eo, ev, d_ev, c.last_lanczos);
+    solve_eigs(c, st, eo, ev, d_ev);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];
     float* d_spb = c.ws<float>("io.spb", 8 * N);
     launch_spb(c, st, d_ev, ev, eo.K, d_spb);
@@ -315,3 +347,32 @@ int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, con
 }
 
 }  // extern "C"
+
+extern "C" int gpb_apply_laplacian(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, int ncols, int block,
+                                   const float* x, float* y) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!mpb_nmax || !x || !y || ncols < 1 || ncols > 16) throw ArgError("bad arguments");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    float* d_m = c.ws<float>("io.nmax", N);
+    GPB_HIP(hipMemcpyAsync(d_m, mpb_nmax, N * 4, hipMemcpyHostToDevice, c.stream));
+    Stencil st;
+    build_stencil(c, d_m, width, height, st);
+    float* dx = c.ws<float>("io.lx", size_t(ncols) * st.Np);
+    float* dy = c.ws<float>("io.ly", size_t(ncols) * st.Np);
+    launch_fill(c, dx, size_t(ncols) * st.Np, 0.f);
+    launch_fill(c, dy, size_t(ncols) * st.Np, 0.f);
+    for (int k = 0; k < ncols; k++)
+      GPB_HIP(hipMemcpy2DAsync(dx + size_t(k) * st.Np + st.pad_index(0, 0), size_t(st.Wp) * 4, x + size_t(k) * N,
+                               size_t(width) * 4, size_t(width) * 4, height, hipMemcpyHostToDevice, c.stream));
+    if (block == 1) {
+      for (int k = 0; k < ncols; k++) debug_block_spmv(c, st, 1, dx + size_t(k) * st.Np, dy + size_t(k) * st.Np);
+    } else {
+      if (block != ncols) throw ArgError("block must equal ncols");
+      debug_block_spmv(c, st, block, dx, dy);
+    }
+    unpad_to_host(c, dy, ncols, width, height, y);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}

@@ -242,28 +242,32 @@ __global__ __launch_bounds__(256) void scale_kernel(const float* __restrict__ w,
     v[p] = float(double(w[p]) * ib);
 }
 
-// out[i] = sum_j Y[j][i] V[j], i < nk, j < m (FP64 accumulation)
+// out[i] = sum_j Y[j][i] V[j], i < nk, j < m <= kRitzMaxM (FP64
+// accumulation); Y (m x nk, row-major) is staged in LDS 16 columns at a time.
+constexpr int kRitzMaxM = 344;
+constexpr int kRitzCols = 16;
+
 __global__ __launch_bounds__(256) void ritz_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
                                                    const double* __restrict__ Y /* m x nk */, float* __restrict__ out,
                                                    size_t Np) {
-  __shared__ double ys[kMaxBasis * 32];
-  for (int i0 = 0; i0 < nk; i0 += 32) {
-    const int cnt = nk - i0 < 32 ? nk - i0 : 32;
+  __shared__ double ys[kRitzMaxM * kRitzCols];
+  for (int i0 = 0; i0 < nk; i0 += kRitzCols) {
+    const int cnt = nk - i0 < kRitzCols ? nk - i0 : kRitzCols;
     __syncthreads();
     for (int t = threadIdx.x; t < m * cnt; t += blockDim.x) ys[t] = Y[size_t(t / cnt) * nk + i0 + (t % cnt)];
     __syncthreads();
     for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
-      double acc[32];
+      double acc[kRitzCols];
 #pragma unroll
-      for (int i = 0; i < 32; i++) acc[i] = 0;
+      for (int i = 0; i < kRitzCols; i++) acc[i] = 0;
       for (int j = 0; j < m; j++) {
         const double v = V[size_t(j) * ld + p];
 #pragma unroll
-        for (int i = 0; i < 32; i++)
+        for (int i = 0; i < kRitzCols; i++)
           if (i < cnt) acc[i] += ys[j * cnt + i] * v;
       }
 #pragma unroll
-      for (int i = 0; i < 32; i++)
+      for (int i = 0; i < kRitzCols; i++)
         if (i < cnt) out[size_t(i0 + i) * ld + p] = float(acc[i]);
     }
   }
@@ -302,6 +306,43 @@ __global__ __launch_bounds__(256) void to_general_kernel(const float* __restrict
                                                          float* __restrict__ v, size_t Np) {
   for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
     v[p] = u[p] * s[p];
+}
+
+// ---- launch helpers shared with the block solver -----------------------------
+void launch_reduce(Context& c, const double* partial, int G, int n, double* dst) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(n), dim3(kRedThreads), 0, c.stream, partial, G, dst, nullptr);
+  GPB_LAUNCH_CHECK();
+}
+
+void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np) {
+  if (m > kRitzMaxM) throw ArgError("basis too large for the Ritz recombination");
+  KScope ks(c, kKRitz, double(Np) * 4.0 * (m + nk));
+  hipLaunchKernelGGL(ritz_kernel, dim3(std::min<unsigned>(2048, div_up(Np, 256))), dim3(256), 0, c.stream, V, ld, m, nk,
+                     Ydev, out, Np);
+  GPB_LAUNCH_CHECK();
+}
+
+void launch_locked_start(Context& c, const Stencil& st, float* v0, float* v1_block, int B, size_t ld, uint64_t seed) {
+  const unsigned vg = std::min<unsigned>(2048, div_up(st.Np, 256));
+  hipLaunchKernelGGL(sqrt_degree_kernel, dim3(vg), dim3(256), 0, c.stream, st.D, v0, st.Np);
+  GPB_LAUNCH_CHECK();
+  for (int r = 0; r < B; r++) {
+    hipLaunchKernelGGL(random_start_kernel, dim3(vg), dim3(256), 0, c.stream, v1_block + size_t(r) * ld, st.W, st.H,
+                       st.Wp, st.Np, seed + 0x9E3779B97F4A7C15ull * uint64_t(r + 1));
+    GPB_LAUNCH_CHECK();
+  }
+}
+
+void launch_to_general(Context& c, const float* u, const float* s, float* v, size_t Np) {
+  hipLaunchKernelGGL(to_general_kernel, dim3(std::min<unsigned>(2048, div_up(Np, 256))), dim3(256), 0, c.stream, u, s, v,
+                     Np);
+  GPB_LAUNCH_CHECK();
+}
+
+void launch_scale_by(Context& c, float* v, const double* nrm2, size_t Np) {
+  hipLaunchKernelGGL(scale_by_kernel, dim3(std::min<unsigned>(2048, div_up(Np, 256))), dim3(256), 0, c.stream, v, nrm2,
+                     Np);
+  GPB_LAUNCH_CHECK();
 }
 
 namespace {

@@ -67,6 +67,8 @@ class Stats(C.Structure):
 
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
 _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_apply_laplacian", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, C.c_int, C.c_int, _f32p, _f32p])
 _sig("gpb_kernel_time", C.c_int, [_ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double)])
 KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb"]
 
@@ -254,3 +256,14 @@ class Context:
                        d_orient: int = 0):
         self._chk(lib.gpb_compute_device(self.h, width, height, d_packed, d_codebook, d_gmax or None, d_gall or None,
                                          d_orient or None))
+
+    def set_block_size(self, block: int):
+        self._chk(lib.gpb_set_block_size(self.h, block))
+
+    def apply_laplacian(self, mpb_nmax: np.ndarray, x: np.ndarray, block: int = 1) -> np.ndarray:
+        m = np.ascontiguousarray(mpb_nmax, np.float32)
+        xx = np.ascontiguousarray(x, np.float32)
+        k, H, W = xx.shape
+        y = np.empty_like(xx)
+        self._chk(lib.gpb_apply_laplacian(self.h, W, H, m.reshape(-1), k, block, xx.reshape(-1), y.reshape(-1)))
+        return y
