@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--block", type=int, default=1, help="Lanczos block size (1 = single-vector)")
     ap.add_argument("--basis", type=int, default=0)
     ap.add_argument("--keep", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the gPb path runs one image (or one patch batch) per GPU, and this "
@@ -70,6 +72,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes_read=n * 4, copy_probe_bytes_written=n * 4, launches=3)), flush=True)
+        return
 
     gpb = _load("gpb_amd_binding", ROOT / "ucm-mpi-gpu_amd" / "gpb.py")
     S, tol = args.size, 1e-3

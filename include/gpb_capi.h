@@ -40,7 +40,7 @@ typedef struct gpb_stats {
   int lanczos_block;
   int converged;        /* eigenpairs meeting the residual target */
   double max_residual;  /* max ||(D-W)v - lambda D v|| / ||D v|| over returned pairs */
-  double evals[16];     /* generalised eigenvalues, ascending */
+  double evals[32];     /* generalised eigenvalues, ascending (first nvec valid) */
   int fallback;         /* 1 if a warm start was rejected and plain Lanczos ran */
   int reorth_second_passes; /* Gram-Schmidt steps that needed a second pass */
 } gpb_stats;
@@ -106,7 +106,7 @@ int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float*
 
 /* ---- stages (c)-(e): affinity, eigenvectors, sPb, gPb ---------------------- */
 
-/* Eigensolver settings for later calls on ctx: nvec eigenpairs (2..16,
+/* Eigensolver settings for later calls on ctx: nvec eigenpairs (2..32,
  * default 9 as in damascene/src/damascene.cu:307), Lanczos basis size,
  * vectors kept at a restart, relative generalised residual target
  * ||(D-W)v - lambda D v|| / ||D v|| (default 1e-3) and a cap on operator

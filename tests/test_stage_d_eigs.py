@@ -53,14 +53,20 @@ def test_oracle_eigs_vs_lapack(gpb, orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("W,H,seed,tol,block", [(24, 20, 3, 1e-6, 1), (64, 48, 4, 1e-5, 1), (24, 20, 3, 1e-6, 8),
-                                                (256, 224, 4, 1e-5, 8), (160, 120, 5, 1e-5, 4)])
-def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol, block):
-    print(f"eigs parity {W}x{H} seed {seed} tol {tol} block {block}", flush=True)
+# (W, H, seed, tol, block, basis, keep): single-vector with the three-term
+# recurrence and one projection pass (basis <= 64), single-vector with
+# Gram-Schmidt twice on every step (basis > 64), the block solver, and the
+# block request on an image below the block solver's size gate (falls back).
+@pytest.mark.parametrize("W,H,seed,tol,block,basis,keep", [
+    (24, 20, 3, 1e-6, 1, 40, 20), (64, 48, 4, 1e-5, 1, 40, 20), (64, 48, 4, 1e-5, 1, 96, 32),
+    (96, 80, 6, 1e-5, 1, 64, 32), (24, 20, 3, 1e-6, 8, 192, 64), (256, 224, 4, 1e-5, 8, 192, 64),
+    (160, 120, 5, 1e-5, 4, 192, 64)])
+def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol, block, basis, keep):
+    print(f"eigs parity {W}x{H} seed {seed} tol {tol} block {block} basis {basis}", flush=True)
     nm = _tiny_w(orc, gpb, W, H, seed)
     Wd, D = orc.build_w(nm)
     ev_ref, vec_ref, _ = orc.eigs(Wd, D, K=10, tol=1e-12)
-    ctx.set_eig_options(nvec=9, basis=40 if block == 1 else 192, keep=20 if block == 1 else 64, tol=tol)
+    ctx.set_eig_options(nvec=9, basis=basis, keep=keep, tol=tol)
     ctx.set_block_size(block)
     try:
         ev, vec = ctx.eigs(nm.astype(np.float32))
@@ -69,7 +75,7 @@ def test_eigs_gpu_vs_oracle(gpb, orc, ctx, W, H, seed, tol, block):
     st = ctx.stats()
     print(f"  matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e} "
           f"block {st['block']} fallback {st['fallback']}", flush=True)
-    if block > 1 and W * H >= 64 * 192:  # size gate of the library
+    if block > 1 and W * H >= 64 * basis:  # size gate of the library
         assert st["block"] == block and st["fallback"] == 0
     assert st["max_residual"] <= tol
     rel = np.abs(ev[1:] - ev_ref[1:9]) / ev_ref[1:9]

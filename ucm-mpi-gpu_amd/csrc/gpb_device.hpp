@@ -147,7 +147,7 @@ struct Context {
   std::map<std::string, std::shared_ptr<void>> scratch;
   LanczosStats last_lanczos;
   KernelTimes kt;
-  double last_evals[16] = {0};
+  double last_evals[kMaxNvec] = {0};
   double stage_ms[8] = {0};
   // eigensolver options (gpb_set_eig_options)
   int eig_K = 9;

@@ -167,7 +167,7 @@ extern "C" {
 
 int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol, int max_matvecs) {
   return guarded2(ctx, [&] {
-    if (nvec < 2 || nvec > 16 || basis < nvec + 2 || basis > 320 || keep < nvec || keep >= basis || !(tol > 0) ||
+    if (nvec < 2 || nvec > kMaxNvec || basis < nvec + 2 || basis > 320 || keep < nvec || keep >= basis || !(tol > 0) ||
         max_matvecs < basis)
       throw ArgError("invalid eigensolver options");
     ctx->c.eig_K = nvec;
@@ -219,7 +219,7 @@ int gpb_get_stats(gpb_ctx* ctx, gpb_stats* out) {
     out->max_residual = ctx->c.last_lanczos.max_residual;
     out->fallback = ctx->c.last_lanczos.fallback;
     out->reorth_second_passes = ctx->c.last_lanczos.second_passes;
-    for (int i = 0; i < 16; i++) out->evals[i] = ctx->c.last_evals[i];
+    for (int i = 0; i < kMaxNvec; i++) out->evals[i] = ctx->c.last_evals[i];
   });
 }
 
@@ -270,7 +270,7 @@ int gpb_spectral_pb(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, 
     Stencil st;
     build_stencil(c, d_m, width, height, st);
     float* d_ev = c.ws<float>("io.evecs", size_t(eo.K) * st.Np);
-    double ev[16];
+    double ev[kMaxNvec];
     solve_eigs(c, st, eo, ev, d_ev);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];
     float* d_spb = c.ws<float>("io.spb", 8 * N);
@@ -285,7 +285,7 @@ int gpb_spb_from_evecs(gpb_ctx* ctx, int width, int height, int nvec, const floa
                        float* spb) {
   return guarded2(ctx, [&] {
     check_dims(width, height);
-    if (!evecs || !evals || !spb || nvec < 2 || nvec > 16) throw ArgError("bad arguments");
+    if (!evecs || !evals || !spb || nvec < 2 || nvec > kMaxNvec) throw ArgError("bad arguments");
     Context& c = ctx->c;
     const size_t N = size_t(width) * height;
     Stencil st;
@@ -312,7 +312,7 @@ int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_pa
     if (!d_packed || !d_codebook) throw ArgError("null pointer");
     Context& c = ctx->c;
     const EigOptions eo = ctx_eig_options(c);
-    double ev[16];
+    double ev[kMaxNvec];
     run_pipeline(c, DeviceImage{width, height, d_packed, d_codebook}, eo, false, d_gpb_max, d_gpb_all, d_gpb_orient, ev,
                  nullptr);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];
@@ -334,7 +334,7 @@ int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, con
     float* d_or = gpb_orient ? c.ws<float>("io.gorient", 8 * N) : nullptr;
     GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
     GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
-    double ev[16];
+    double ev[kMaxNvec];
     PipelineBuffers b;
     run_pipeline(c, DeviceImage{width, height, d_rgb, d_cb}, eo, false, d_gmax, d_gall, d_or, ev, &b);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];

@@ -64,8 +64,10 @@ struct SpmvArgs {
 
 __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                            const float* __restrict__ x, float* __restrict__ y, int W,
-                                                           int H, int Wp, int Hp, size_t Np, SpmvArgs A) {
+                                                           int H, int Wp, int Hp, size_t Np, SpmvArgs A,
+                                                           double* __restrict__ xy_partial) {
   __shared__ float u[kSpmvTileH * kSpmvTileW];
+  __shared__ double red[4];
   const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
   const int x0 = blockIdx.x * kSpmvTX, y0 = blockIdx.y * kSpmvTY;  // interior coordinates of the tile origin
   for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
@@ -80,26 +82,37 @@ __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restri
   }
   __syncthreads();
   const int xi = x0 + threadIdx.x;
-  if (xi >= W) return;
+  double xy = 0;
+  if (xi < W) {
 #pragma unroll
-  for (int r = 0; r < kSpmvRows; r++) {
-    const int ty = threadIdx.y + 4 * r;
-    const int yi = y0 + ty;
-    if (yi >= H) break;
-    const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
-    const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
-    const float up = u[c];
-    float acc = 0.f;
+    for (int r = 0; r < kSpmvRows; r++) {
+      const int ty = threadIdx.y + 4 * r;
+      const int yi = y0 + ty;
+      if (yi >= H) break;
+      const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
+      const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
+      const float up = u[c];
+      float acc = 0.f;
 #pragma unroll
-    for (int j = 0; j < kStencilHalf; j++) {
-      acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - u[c + A.lo[j]], acc);
-      acc = __builtin_fmaf(wup[size_t(j) * Np + p - A.o[j]], up - u[c - A.lo[j]], acc);
+      for (int j = 0; j < kStencilHalf; j++) {
+        acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - u[c + A.lo[j]], acc);
+        acc = __builtin_fmaf(wup[size_t(j) * Np + p - A.o[j]], up - u[c - A.lo[j]], acc);
+      }
+      const float yp = s[p] * acc;
+      y[p] = yp;
+      xy += double(x[p]) * double(yp);
     }
-    y[p] = s[p] * acc;
   }
+  if (!xy_partial) return;  // uniform
+  for (int o = 32; o > 0; o >>= 1) xy += __shfl_xor(xy, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = xy;
+  __syncthreads();
+  if (tid == 0) xy_partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) {
+// y = L x; with xy_partial, also the per-block partial sums of x . y
+// (spmv_blocks(st) of them).
+void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double* xy_partial) {
   SpmvArgs A;
   int dx[kStencilHalf], dy[kStencilHalf];
   stencil_upper_offsets(dx, dy);
@@ -109,9 +122,14 @@ void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) {
   }
   dim3 block(kSpmvTX, 4), grid(div_up(st.W, kSpmvTX), div_up(st.H, kSpmvTY));
   KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
-  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, A);
+  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, A,
+                     xy_partial);
   GPB_LAUNCH_CHECK();
 }
+
+void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) { launch_spmv(c, st, x, y, nullptr); }
+
+int spmv_blocks(const Stencil& st) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, kSpmvTY)); }
 
 // Sum of 4 values over the 256-thread block; result valid in thread 0.
 __device__ __forceinline__ void block_sum4(double v[4], double* sh /* [4][4] */) {
@@ -164,10 +182,11 @@ __global__ __launch_bounds__(256) void multidot2_kernel(const float* __restrict_
 }
 
 // w <- w - sum_i h_i V_i (FP64 accumulation), grid-stride; per-block partial
-// of ||w||^2 into nrm_partial[blk] when non-null.
+// of ||w||^2 into nrm_partial[blk] when non-null.  Skipped when *gate == 0.
 __global__ __launch_bounds__(256) void update_gs_kernel(const float* __restrict__ V, size_t ld, int n,
                                                         const double* __restrict__ h, float* __restrict__ w, size_t Np,
-                                                        double* __restrict__ nrm_partial) {
+                                                        double* __restrict__ nrm_partial, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ double hs[kMaxBasis];
   __shared__ double sh[16];
   for (int i = threadIdx.x; i < n; i += blockDim.x) hs[i] = h[i];
@@ -186,6 +205,93 @@ __global__ __launch_bounds__(256) void update_gs_kernel(const float* __restrict_
   }
 }
 
+// Lanczos projection in one pass over the basis.  With alpha: first the
+// three-term step w <- w - alpha v_j - beta_{j-1} v_{j-1} (FP64, rounded to
+// FP32 and stored), then, for the result, the dots V_i . w (i < n) and
+// ||w||^2, FP64 accumulation in NV per-thread registers, so w is read once
+// however many basis vectors there are.  partial[i][blk] for i < n, the
+// squared norm in row n.  Skipped when *gate == 0.
+template <int NV>
+__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                      float* __restrict__ w, const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta, int j,
+                                                      const int* __restrict__ gate, size_t Np,
+                                                      double* __restrict__ partial) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[4][NV + 1];
+  const double a = alpha ? *alpha : 0.0;
+  const double b = (alpha && j > 0) ? beta[j - 1] : 0.0;
+  const float* vj = V + size_t(j) * ld;
+  const float* vjm = V + size_t(j > 0 ? j - 1 : 0) * ld;
+  double acc[NV + 1];
+#pragma unroll
+  for (int i = 0; i <= NV; i++) acc[i] = 0.0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    double wn = w[p];
+    if (alpha) {
+      wn -= a * double(vj[p]) + b * double(vjm[p]);
+      const float f = float(wn);
+      w[p] = f;
+      wn = f;
+    }
+    acc[NV] += wn * wn;
+    // all NV loads issued before any use (rows past n re-read row 0 and are
+    // discarded), so the wave keeps NV loads in flight
+    const float* q = V + p;
+    float vv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) vv[i] = q[size_t(i < n ? i : 0) * ld];
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc[i] += double(vv[i]) * wn;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i <= NV; i++) {
+    if (i < n || i == NV) {
+      double v = acc[i];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) sh[wv][i] = v;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t <= n; t += blockDim.x) {
+    const int i = t < n ? t : NV;
+    partial[size_t(t) * gridDim.x + blockIdx.x] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+  }
+}
+
+constexpr int kProjectMaxN = 64;
+
+void launch_project(Context& c, const float* V, size_t ld, int n, float* w, const double* alpha, const double* beta,
+                    int j, const int* gate, size_t Np, double* partial, int G) {
+  if (n < 1 || n > kProjectMaxN) throw ArgError("projection width out of range");
+  const dim3 g(G), b(256);
+#define GPB_PROJ(NV)                                                                                                \
+  case NV / 4:                                                                                                      \
+    hipLaunchKernelGGL(project_kernel<NV>, g, b, 0, c.stream, V, ld, n, w, alpha, beta, j, gate, Np, partial);      \
+    break;
+  switch ((n + 3) / 4) {
+    GPB_PROJ(4)
+    GPB_PROJ(8)
+    GPB_PROJ(12)
+    GPB_PROJ(16)
+    GPB_PROJ(20)
+    GPB_PROJ(24)
+    GPB_PROJ(28)
+    GPB_PROJ(32)
+    GPB_PROJ(36)
+    GPB_PROJ(40)
+    GPB_PROJ(44)
+    GPB_PROJ(48)
+    GPB_PROJ(52)
+    GPB_PROJ(56)
+    GPB_PROJ(60)
+    GPB_PROJ(64)
+  }
+#undef GPB_PROJ
+  GPB_LAUNCH_CHECK();
+}
+
 // out[i] = sum_blk partial[i][blk]  (one block per i, fixed order); gated.
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const double* __restrict__ partial, int G,
                                                              double* __restrict__ out, const int* __restrict__ gate) {
@@ -199,32 +305,35 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const double* __res
 
 
 
-// After the first Gram-Schmidt pass the correction of the second pass is
-// applied only when the norm dropped below 1/sqrt(2) of its value before the
-// pass (Daniel, Gragg, Kaufman and Stewart 1976).  The second pass's
-// projections are always computed, fused into the first update.
-// acc[0], acc[1]: algorithmic bytes of the gated dots / update launches that
-// run; acc[2]: number of second passes.
+// Daniel-Gragg-Kaufman-Stewart test (1976): after the projection pass a
+// second pass runs only when the norm dropped below 1/sqrt(2) of its value
+// before the pass (w1sq < w0sq / 2).  acc[0], acc[1]: algorithmic bytes of the
+// gated projection / update launches that run; acc[2]: second passes.
 __global__ void reorth_decide_kernel(const double* __restrict__ w0sq, const double* __restrict__ w1sq,
                                      int* __restrict__ flag, int n, double np, double* __restrict__ acc) {
   const int f = (*w1sq < 0.5 * *w0sq) ? 1 : 0;
   *flag = f;
   if (f) {
+    acc[0] += np * (4.0 * n + 4);
     acc[1] += np * (4.0 * n + 8);
     acc[2] += 1.0;
   }
 }
 
-// T[:, j] = h1 + (second pass ? h2 : 0) (mirrored into row j); beta_j from
-// the norm after the last pass that ran.
-__global__ void tcol_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ h1,
-                            const double* __restrict__ h2, const int* __restrict__ flag,
+// Column j of the projected matrix: T[i][j] += h1[i] (+ h2[i] after a second
+// pass) (+ alpha on the diagonal), mirrored into row j; beta_j from the norm
+// after the last pass that ran, stored in beta[j] (1/beta_j in
+// beta[kMaxBasis + j]) and, when couple_next, as T[j][j+1] = T[j+1][j], the
+// coupling the next step's three-term recurrence uses (a next step that
+// projects against the whole basis finds it itself).
+__global__ void tcol_kernel(double* __restrict__ T, int m, int j, int couple_next, const double* __restrict__ alpha,
+                            const double* __restrict__ h1, const double* __restrict__ h2, const int* __restrict__ flag,
                             const double* __restrict__ nrm1, const double* __restrict__ nrm2,
                             double* __restrict__ beta) {
   const int i = threadIdx.x;
   const bool two = flag && *flag;
   if (i <= j) {
-    const double v = T[size_t(i) * m + j] + h1[i] + (two ? h2[i] : 0.0);
+    const double v = T[size_t(i) * m + j] + h1[i] + (two ? h2[i] : 0.0) + ((i == j && alpha) ? *alpha : 0.0);
     T[size_t(i) * m + j] = v;
     if (i != j) T[size_t(j) * m + i] = v;
   }
@@ -232,6 +341,10 @@ __global__ void tcol_kernel(double* __restrict__ T, int m, int j, const double* 
     const double b = sqrt(two ? *nrm2 : *nrm1);
     beta[j] = b;
     beta[kMaxBasis + j] = b > 1e-300 ? 1.0 / b : 0.0;
+    if (couple_next && j + 1 < m) {
+      T[size_t(j) * m + j + 1] = b;
+      T[size_t(j + 1) * m + j] = b;
+    }
   }
 }
 
@@ -352,12 +465,14 @@ struct Reducer {
   size_t Np;
   int NW;   // element chunks of the dot kernel
   int GU;   // blocks of the update kernel
+  int GP;   // blocks of the projection kernel
   double* partial;
   double* npart;
   Reducer(Context& cc, size_t np) : c(cc), Np(np) {
     NW = int((np + kDotChunk - 1) / kDotChunk);
     GU = int(std::min<size_t>(2048, std::max<size_t>(1, (np + 255) / 256)));
-    partial = c.ws<double>("eig.partial", size_t(kMaxBasis + 2) * NW);
+    GP = int(std::min<size_t>(512, std::max<size_t>(1, (np + 255) / 256)));
+    partial = c.ws<double>("eig.partial", std::max(size_t(kMaxBasis + 2) * NW, size_t(kProjectMaxN + 2) * GP));
     npart = c.ws<double>("eig.npart", size_t(GU));
   }
   // dst[0..n) = V_i . w, dst[n] = w . (D) w if wsq
@@ -371,16 +486,28 @@ struct Reducer {
     hipLaunchKernelGGL(reduce_kernel, dim3(n + (wsq ? 1 : 0)), dim3(kRedThreads), 0, c.stream, partial, NW, dst, nullptr);
     GPB_LAUNCH_CHECK();
   }
-  // w -= V h; nrm_dst = ||w||^2 afterwards (if non-null)
-  void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst) {
+  // [alpha: w <- w - alpha v_j - beta_{j-1} v_{j-1}]; dst[0..n) = V_i . w,
+  // dst[n] = ||w||^2 (n <= kProjectMaxN); gated when gate is non-null.
+  void project(const float* V, size_t ld, int n, float* w, const double* alpha, const double* beta, int j,
+               const int* gate, double* dst) {
     {
-      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8));
+      KScope ks(c, kKDots, gate ? 0.0 : double(Np) * (4.0 * n + 4 + (alpha ? 12 : 0)));
+      launch_project(c, V, ld, n, w, alpha, beta, j, gate, Np, partial, GP);
+    }
+    hipLaunchKernelGGL(reduce_kernel, dim3(n + 1), dim3(kRedThreads), 0, c.stream, partial, GP, dst, gate);
+    GPB_LAUNCH_CHECK();
+  }
+  // w -= V h; nrm_dst = ||w||^2 afterwards (if non-null); gated when gate is non-null
+  void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst,
+              const int* gate = nullptr) {
+    {
+      KScope ks(c, kKUpdate, gate ? 0.0 : double(Np) * (4.0 * n + 8));
       hipLaunchKernelGGL(update_gs_kernel, dim3(GU), dim3(256), 0, c.stream, V, ld, n, h, w, Np,
-                         nrm_dst ? npart : nullptr);
+                         nrm_dst ? npart : nullptr, gate);
       GPB_LAUNCH_CHECK();
     }
     if (nrm_dst) {
-      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, npart, GU, nrm_dst, nullptr);
+      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, npart, GU, nrm_dst, gate);
       GPB_LAUNCH_CHECK();
     }
   }
@@ -417,6 +544,9 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, one_flag, 1);
   GPB_LAUNCH_CHECK();
   double* beta = c.ws<double>("eig.beta", 2 * kMaxBasis);
+  double* alpha = c.ws<double>("eig.alpha", 1);
+  const int spmv_g = spmv_blocks(st);
+  double* xy_part = c.ws<double>("eig.xy", size_t(spmv_g));
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   Reducer red(c, Np);
   const unsigned vg = vec_grid(Np);
@@ -431,7 +561,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   // T00 = V0' L V0
   launch_spmv(c, st, V, w);
   red.dots(V, Np, 1, w, nullptr, false, h);
-  hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, h, h, nullptr, h, h, nullptr);
+  hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, 0, nullptr, h, h, nullptr, h, h, nullptr);
   GPB_LAUNCH_CHECK();
   // V1: random, orthogonal to V0, normalised
   float* V1 = V + Np;
@@ -459,16 +589,43 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     for (int j = k; j < m; j++) {
       float* vj = V + size_t(j) * Np;
       const int n = j + 1;
-      launch_spmv(c, st, vj, w);
       matvecs++;
-      // classical Gram-Schmidt twice ("twice is enough"), FP64 accumulation
-      red.dots(V, Np, n, w, nullptr, false, h1);
-      red.update(V, Np, n, h1, w, nullptr);
-      red.dots(V, Np, n, w, nullptr, false, h2);
-      red.update(V, Np, n, h2, w, nrm + 1);
-      hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, h1, h2, one_flag, nrm + 1, nrm + 1,
-                         beta);
-      GPB_LAUNCH_CHECK();
+      if (j == k || m > kProjectMaxN) {
+        // first step of a cycle (its column couples to every kept Ritz
+        // vector): classical Gram-Schmidt twice, FP64 accumulation
+        launch_spmv(c, st, vj, w);
+        if (n <= kProjectMaxN) {
+          red.project(V, Np, n, w, nullptr, nullptr, j, nullptr, h1);
+        } else {
+          red.dots(V, Np, n, w, nullptr, false, h1);
+        }
+        red.update(V, Np, n, h1, w, nullptr);
+        if (n <= kProjectMaxN) {
+          red.project(V, Np, n, w, nullptr, nullptr, j, nullptr, h2);
+        } else {
+          red.dots(V, Np, n, w, nullptr, false, h2);
+        }
+        red.update(V, Np, n, h2, w, nrm + 1);
+        hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, int(m <= kProjectMaxN), nullptr,
+                           h1, h2, one_flag,
+                           nrm + 1, nrm + 1, beta);
+        GPB_LAUNCH_CHECK();
+      } else {
+        // three-term step, one full projection pass, and a second pass only
+        // when the DGKS test asks for it
+        launch_spmv(c, st, vj, w, xy_part);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, xy_part, spmv_g, alpha, nullptr);
+        GPB_LAUNCH_CHECK();
+        red.project(V, Np, n, w, alpha, beta, j, nullptr, h1);  // h1[n] = ||w'||^2
+        red.update(V, Np, n, h1, w, nrm);
+        hipLaunchKernelGGL(reorth_decide_kernel, dim3(1), dim3(1), 0, c.stream, h1 + n, nrm, flag, n, double(Np), gacc);
+        GPB_LAUNCH_CHECK();
+        red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
+        red.update(V, Np, n, h2, w, nrm + 1, flag);
+        hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, 1, alpha, h1, h2, flag, nrm,
+                           nrm + 1, beta);
+        GPB_LAUNCH_CHECK();
+      }
       hipLaunchKernelGGL(scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np);
       GPB_LAUNCH_CHECK();
     }

@@ -18,7 +18,7 @@ struct SpbParams {
   float filt[kNumOri][kSPbSize * kSPbSize];
   const float* evecs;      // [K][Np] padded
   const float* mnmx;       // [K][2] min, max over the image
-  float inv_sqrt_lam[16];  // 0 => vector skipped
+  float inv_sqrt_lam[kMaxNvec];  // 0 => vector skipped
   int K, W, H, Wp;
   size_t Np;
   float* spb;              // [8][N]
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kSpbT * kSpbT) void spb_kernel(SpbParams P) {
 }
 
 void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb) {
-  if (K > 16) throw ArgError("at most 16 eigenvectors");
+  if (K > kMaxNvec) throw ArgError("too many eigenvectors");
   const int G = 256;
   float* part = c.ws<float>("spb.part", 2 * G);
   float* mnmx = c.ws<float>("spb.mnmx", 2 * K);
@@ -143,7 +143,7 @@ void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const doubl
     for (int t = 0; t < kSPbSize * kSPbSize; t++) P.filt[o][t] = float(f[size_t(o) * kSPbSize * kSPbSize + t]);
   P.evecs = d_evecs;
   P.mnmx = mnmx;
-  for (int i = 0; i < 16; i++)
+  for (int i = 0; i < kMaxNvec; i++)
     P.inv_sqrt_lam[i] = (i > 0 && i < K && evals[i] > 2.220446049250313e-16) ? float(1.0 / std::sqrt(evals[i])) : 0.f;
   P.K = K;
   P.W = st.W;

@@ -61,7 +61,7 @@ _sig("gpb_ic_program", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, np.ctypesli
 class Stats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 6), ("lanczos_matvecs", C.c_int), ("lanczos_restarts", C.c_int),
                 ("lanczos_basis", C.c_int), ("lanczos_block", C.c_int), ("converged", C.c_int),
-                ("max_residual", C.c_double), ("evals", C.c_double * 16), ("fallback", C.c_int),
+                ("max_residual", C.c_double), ("evals", C.c_double * 32), ("fallback", C.c_int),
                 ("reorth_second_passes", C.c_int)]
 
 
