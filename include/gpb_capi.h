@@ -121,6 +121,15 @@ int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol,
  * SymEigsSolver.h:200-280). */
 int gpb_set_block_size(gpb_ctx* ctx, int block);
 
+/* SpMV tile for later solves on ctx: rows = 2, 4 or 8 rows per thread
+ * (tiles of 64 x 4*rows pixels), or 0 (default) to time the three variants on
+ * the first solve of each image shape and keep the fastest.
+ * gpb_get_spmv_tile reports the tile in use for a shape (0 = not tuned yet).
+ * No counterpart in the reference (its SpMV has one fixed launch shape,
+ * damascene/src/stencilMVM/stencilMVM.cu:129-182). */
+int gpb_set_spmv_tile(gpb_ctx* ctx, int rows);
+int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows);
+
 /* Kernel classes for gpb_kernel_time. */
 #define GPB_K_TEXTON 0
 #define GPB_K_CUES 1

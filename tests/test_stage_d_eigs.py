@@ -110,3 +110,29 @@ def test_laplacian_operator_gpu(gpb, orc, ctx, block):
     err = np.abs(y - ref).max() / np.abs(ref).max()
     print(f"  rel err {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", [2, 4, 8, 0])
+def test_spmv_tiles_gpu(gpb, orc, ctx, rows):
+    """Every SpMV tile variant (and the autotuned choice) applies the same operator."""
+    print(f"spmv tile rows {rows}", flush=True)
+    H, W = 77, 150
+    rng = np.random.default_rng(7)
+    m = (rng.random((H, W)) * (rng.random((H, W)) < 0.3)).astype(np.float32)
+    Wd, D = orc.build_w(m.astype(np.float64))
+    A = orc.dense_w(Wd)
+    s = 1 / np.sqrt(D.reshape(-1))
+    L = np.eye(H * W) - (s[:, None] * A * s[None, :])
+    x = rng.normal(size=(2, H, W)).astype(np.float32)
+    ctx.set_spmv_tile(rows)
+    try:
+        y = ctx.apply_laplacian(m, x, block=1)
+        chosen = ctx.spmv_tile(W, H)
+    finally:
+        ctx.set_spmv_tile(0)
+    ref = (L @ x.reshape(2, -1).T.astype(np.float64)).T.reshape(2, H, W)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"  tile {chosen} rel err {err:.2e}", flush=True)
+    assert chosen == (rows or chosen) and chosen in (2, 4, 8)
+    assert err < 1e-5

@@ -1,7 +1,8 @@
 """Build recipe for libgpb.so (HIP kernels for gfx950 + C++ host code).
 
 Compiles every ``csrc/*.hip`` with hipcc for gfx950 and every ``csrc/*.cpp``
-with g++, then links ``libgpb.so`` next to this file.  Objects go to
+with g++, then links ``libgpb.so`` next to this file, and ``gpb_cli`` (the
+command-line front end, ``apps/gpb_cli.cpp``) against it.  Objects go to
 ``_build/`` (git-ignored).  Run ``python build.py`` or call :func:`build`.
 """
 from __future__ import annotations
@@ -18,6 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libgpb.so"
+CLI = PKG / "gpb_cli"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
@@ -58,6 +60,11 @@ def build(verbose: bool = True) -> Path:
     _run(["g++", "-shared", "-fopenmp", "-o", str(tmp), *map(str, objs),
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
+    # command-line front end over the reference-named C++ entry points
+    cli_tmp = CLI.with_suffix(".tmp")
+    _run(["g++", *CXX_FLAGS, "-I", str(ROOT / "include"), str(PKG / "apps" / "gpb_cli.cpp"), "-o", str(cli_tmp),
+          f"-L{PKG}", "-lgpb", "-Wl,-rpath,$ORIGIN", f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(cli_tmp, CLI)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources", flush=True)
     return LIB

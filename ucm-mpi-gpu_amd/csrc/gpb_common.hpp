@@ -23,6 +23,7 @@ constexpr int kICRadius = 5;        // intervening-contour radius (buildW.cpp:24
 constexpr int kStencilHalf = 40;    // disc offsets with (dy>0) or (dy==0, dx>0) at radius 5
 constexpr int kStencilFull = 81;    // all disc offsets including the centre
 constexpr int kNumCues = 12;        // bg1..3, cga1..3, cgb1..3, tg1..3
+constexpr int kNumXcd = 8;         // accelerator complex dies of an MI355X (one L2 each)
 constexpr int kMaxNvec = 32;        // eigenpairs per solve (spectralPb's CPU default is 17)
 
 // Truncated pi used by the reference for filter phases and nonmax angles

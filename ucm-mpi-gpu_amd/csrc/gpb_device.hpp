@@ -156,6 +156,9 @@ struct Context {
   double eig_tol = 1e-3;
   int eig_max_matvecs = 20000;
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
+  // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
+  std::map<uint64_t, int> spmv_rows;
+  int spmv_rows_forced = 0;
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {

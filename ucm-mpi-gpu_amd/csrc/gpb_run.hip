@@ -178,6 +178,25 @@ int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol,
   });
 }
 
+int gpb_set_spmv_tile(gpb_ctx* ctx, int rows) {
+  return guarded2(ctx, [&] {
+    if (!(rows == 0 || rows == 2 || rows == 4 || rows == 8)) throw ArgError("SpMV tile rows must be 0, 2, 4 or 8");
+    ctx->c.spmv_rows_forced = rows;
+  });
+}
+
+int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows) {
+  return guarded2(ctx, [&] {
+    if (!rows) throw ArgError("null pointer");
+    if (ctx->c.spmv_rows_forced) {
+      *rows = ctx->c.spmv_rows_forced;
+      return;
+    }
+    auto it = ctx->c.spmv_rows.find((uint64_t(width) << 32) | uint32_t(height));
+    *rows = it == ctx->c.spmv_rows.end() ? 0 : it->second;
+  });
+}
+
 int gpb_set_block_size(gpb_ctx* ctx, int block) {
   return guarded2(ctx, [&] {
     if (!(block == 1 || block == 4 || block == 8 || block == 12 || block == 16))

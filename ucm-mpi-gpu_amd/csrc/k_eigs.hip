@@ -54,22 +54,34 @@ struct SpmvOffsets {
 // against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
 // W[k][p - e_k] for q = p - e_k.  A 64x16 tile of u (plus a 5-pixel halo) is
 // staged in LDS; W streams from HBM with coalesced row reads.
-constexpr int kSpmvTX = 64, kSpmvTY = 16, kSpmvRows = 4;  // 64x4 threads, 4 rows each
-constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+// Tiles are 64 columns by 4 * ROWS rows (64x4 threads, ROWS rows each); ROWS
+// is picked per image shape by timing (tune_spmv).
+constexpr int kSpmvTX = 64;
+constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius;
 
 struct SpmvArgs {
   int o[kStencilHalf];   // padded linear offsets
   int lo[kStencilHalf];  // LDS tile offsets
 };
 
+template <int ROWS>
 __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                            const float* __restrict__ x, float* __restrict__ y, int W,
                                                            int H, int Wp, int Hp, size_t Np, SpmvArgs A,
                                                            double* __restrict__ xy_partial) {
+  constexpr int kSpmvTY = 4 * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
   __shared__ float u[kSpmvTileH * kSpmvTileW];
   __shared__ double red[4];
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
+  // workgroup b runs on XCD b % 8; each XCD takes a contiguous run of tiles
+  // (row-major), so horizontally and vertically adjacent tiles, which read
+  // each other's W rows and halo, share that XCD's L2.
+  const int tiles_x = (W + kSpmvTX - 1) / kSpmvTX, ntiles = tiles_x * ((H + kSpmvTY - 1) / kSpmvTY);
+  const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
+  const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
+  if (tile >= ntiles) return;  // whole workgroup
   const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
-  const int x0 = blockIdx.x * kSpmvTX, y0 = blockIdx.y * kSpmvTY;  // interior coordinates of the tile origin
+  const int x0 = (tile % tiles_x) * kSpmvTX, y0 = (tile / tiles_x) * kSpmvTY;  // interior coordinates of the tile origin
   for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
     const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
     const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
@@ -85,7 +97,7 @@ __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restri
   double xy = 0;
   if (xi < W) {
 #pragma unroll
-    for (int r = 0; r < kSpmvRows; r++) {
+    for (int r = 0; r < ROWS; r++) {
       const int ty = threadIdx.y + 4 * r;
       const int yi = y0 + ty;
       if (yi >= H) break;
@@ -107,12 +119,14 @@ __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restri
   for (int o = 32; o > 0; o >>= 1) xy += __shfl_xor(xy, o, 64);
   if ((tid & 63) == 0) red[tid >> 6] = xy;
   __syncthreads();
-  if (tid == 0) xy_partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (tid == 0) xy_partial[tile] = red[0] + red[1] + red[2] + red[3];
 }
 
-// y = L x; with xy_partial, also the per-block partial sums of x . y
-// (spmv_blocks(st) of them).
-void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double* xy_partial) {
+int spmv_blocks(const Stencil& st, int rows) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, 4 * rows)); }
+int spmv_blocks(const Stencil& st) { return spmv_blocks(st, 2); }  // upper bound over the variants
+
+namespace {
+void spmv_dispatch(Context& c, const Stencil& st, int rows, const float* x, float* y, double* xy_partial) {
   SpmvArgs A;
   int dx[kStencilHalf], dy[kStencilHalf];
   stencil_upper_offsets(dx, dy);
@@ -120,16 +134,69 @@ void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double
     A.o[j] = st.off_host[j];
     A.lo[j] = dy[j] * kSpmvTileW + dx[j];
   }
-  dim3 block(kSpmvTX, 4), grid(div_up(st.W, kSpmvTX), div_up(st.H, kSpmvTY));
-  KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
-  hipLaunchKernelGGL(spmv_kernel, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, A,
-                     xy_partial);
+  const int ntiles = spmv_blocks(st, rows);
+  dim3 block(kSpmvTX, 4), grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
+#define GPB_SPMV(R)                                                                                                   \
+  case R:                                                                                                             \
+    hipLaunchKernelGGL(spmv_kernel<R>, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, \
+                       A, xy_partial);                                                                                \
+    break;
+  switch (rows) {
+    GPB_SPMV(2)
+    GPB_SPMV(4)
+    GPB_SPMV(8)
+    default:
+      throw ArgError("unsupported SpMV tile");
+  }
+#undef GPB_SPMV
   GPB_LAUNCH_CHECK();
+}
+}  // namespace
+
+// Rows per thread of the SpMV tile for this image shape: each variant is
+// timed (HIP events, 3 applications after one warm-up) on the stencil itself
+// the first time a shape is seen on the context; the fastest is cached.
+int tune_spmv(Context& c, const Stencil& st) {
+  if (c.spmv_rows_forced) return c.spmv_rows_forced;
+  const uint64_t key = (uint64_t(st.W) << 32) | uint32_t(st.H);
+  auto it = c.spmv_rows.find(key);
+  if (it != c.spmv_rows.end()) return it->second;
+  float* x = c.ws<float>("tune.x", st.Np);
+  float* y = c.ws<float>("tune.y", st.Np);
+  launch_fill(c, x, st.Np, 1.0f);
+  hipEvent_t e0, e1;
+  GPB_HIP(hipEventCreate(&e0));
+  GPB_HIP(hipEventCreate(&e1));
+  int best = 4;
+  float best_ms = 1e30f;
+  for (int rows : {2, 4, 8}) {
+    spmv_dispatch(c, st, rows, x, y, nullptr);
+    GPB_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < 3; r++) spmv_dispatch(c, st, rows, x, y, nullptr);
+    GPB_HIP(hipEventRecord(e1, c.stream));
+    GPB_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = rows;
+    }
+  }
+  GPB_HIP(hipEventDestroy(e0));
+  GPB_HIP(hipEventDestroy(e1));
+  c.spmv_rows[key] = best;
+  return best;
+}
+
+// y = L x; with xy_partial, also the per-tile partial sums of x . y
+// (spmv_blocks(st, rows) of them, rows = the tile the shape was tuned to).
+void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double* xy_partial) {
+  const int rows = tune_spmv(c, st);
+  KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
+  spmv_dispatch(c, st, rows, x, y, xy_partial);
 }
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) { launch_spmv(c, st, x, y, nullptr); }
-
-int spmv_blocks(const Stencil& st) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, kSpmvTY)); }
 
 // Sum of 4 values over the 256-thread block; result valid in thread 0.
 __device__ __forceinline__ void block_sum4(double v[4], double* sh /* [4][4] */) {
@@ -356,28 +423,35 @@ __global__ __launch_bounds__(256) void scale_kernel(const float* __restrict__ w,
 }
 
 // out[i] = sum_j Y[j][i] V[j], i < nk, j < m <= kRitzMaxM (FP64
-// accumulation); Y (m x nk, row-major) is staged in LDS 16 columns at a time.
+// accumulation), kRitzCols outputs per pass over V.  Y is read with
+// wave-uniform indices, so its values come through the scalar cache and feed
+// the FMAs as scalar operands; the basis is loaded 8 rows at a time, all 8
+// loads issued before use.
 constexpr int kRitzMaxM = 344;
-constexpr int kRitzCols = 16;
+constexpr int kRitzCols = 24;
 
 __global__ __launch_bounds__(256) void ritz_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
                                                    const double* __restrict__ Y /* m x nk */, float* __restrict__ out,
                                                    size_t Np) {
-  __shared__ double ys[kRitzMaxM * kRitzCols];
   for (int i0 = 0; i0 < nk; i0 += kRitzCols) {
     const int cnt = nk - i0 < kRitzCols ? nk - i0 : kRitzCols;
-    __syncthreads();
-    for (int t = threadIdx.x; t < m * cnt; t += blockDim.x) ys[t] = Y[size_t(t / cnt) * nk + i0 + (t % cnt)];
-    __syncthreads();
     for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
       double acc[kRitzCols];
 #pragma unroll
       for (int i = 0; i < kRitzCols; i++) acc[i] = 0;
-      for (int j = 0; j < m; j++) {
-        const double v = V[size_t(j) * ld + p];
+      for (int j0 = 0; j0 < m; j0 += 8) {
+        float v[8];
 #pragma unroll
-        for (int i = 0; i < kRitzCols; i++)
-          if (i < cnt) acc[i] += ys[j * cnt + i] * v;
+        for (int t = 0; t < 8; t++) v[t] = V[size_t(j0 + t < m ? j0 + t : 0) * ld + p];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          if (j0 + t >= m) break;  // uniform
+          const double* y = Y + size_t(j0 + t) * nk + i0;
+          const double vt = double(v[t]);
+#pragma unroll
+          for (int i = 0; i < kRitzCols; i++)
+            if (i < cnt) acc[i] += y[i] * vt;
+        }
       }
 #pragma unroll
       for (int i = 0; i < kRitzCols; i++)
@@ -545,7 +619,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   GPB_LAUNCH_CHECK();
   double* beta = c.ws<double>("eig.beta", 2 * kMaxBasis);
   double* alpha = c.ws<double>("eig.alpha", 1);
-  const int spmv_g = spmv_blocks(st);
+  const int spmv_g = spmv_blocks(st, tune_spmv(c, st));
   double* xy_part = c.ws<double>("eig.xy", size_t(spmv_g));
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   Reducer red(c, Np);
@@ -650,11 +724,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     for (int j = 0; j < m; j++)
       for (int i = 0; i < nk; i++) Yk[size_t(j) * nk + i] = Y[size_t(j) * m + i];
     GPB_HIP(hipMemcpyAsync(Ydev, Yk.data(), sizeof(double) * m * nk, hipMemcpyHostToDevice, c.stream));
-    {
-      KScope ks(c, kKRitz, double(Np) * 4.0 * (m + nk));
-      hipLaunchKernelGGL(ritz_kernel, dim3(vg), dim3(256), 0, c.stream, V, Np, m, nk, Ydev, V2, Np);
-      GPB_LAUNCH_CHECK();
-    }
+    launch_ritz(c, V, Np, m, nk, Ydev, V2, Np);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(nk) * Np, V + size_t(m) * Np, Np * sizeof(float), hipMemcpyDeviceToDevice,
                            c.stream));
     // weighted norms ||D^1/2 u_i||, ||D^1/2 v_m||

@@ -1,0 +1,372 @@
+// The reference-named C++ entry points (gpb_ucm.hpp) on top of the C ABI.
+//
+// References: src/det_mPb.cpp:9-22, src/multiscalePb.cpp:16-283,
+// src/spectralPb.cpp:33-343, src/buildW.cpp:23-246, src/globalPb.cpp:15-126,
+// damascene/src/damascene.cu:242-661 (computeGPb, getCudaDeviceCount),
+// include/detmpb.h, include/otherheaders.h:42-55.
+#include "gpb_ucm.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+
+#include "gpb_capi.h"
+
+namespace gpb_ucm {
+namespace {
+
+constexpr int kOri = 8;
+constexpr int kCues = 12;
+
+std::mutex g_mu;                        // guards the tables below
+std::map<int, gpb_ctx*> g_ctx;          // one context per device, kept for the process
+std::map<int, std::unique_ptr<std::mutex>> g_dev_mu;  // serialises calls per device
+std::vector<float> g_codebook;          // empty: per-image codebook
+int g_device = 0;
+
+struct DeviceLease {
+  gpb_ctx* ctx;
+  std::unique_lock<std::mutex> lock;
+};
+
+DeviceLease lease(int device) {
+  std::mutex* m;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    auto& slot = g_dev_mu[device];
+    if (!slot) slot.reset(new std::mutex);
+    m = slot.get();
+  }
+  std::unique_lock<std::mutex> l(*m);
+  gpb_ctx* c = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_ctx.find(device);
+    if (it != g_ctx.end()) c = it->second;
+  }
+  if (!c) {
+    if (gpb_ctx_create(device, &c) != GPB_OK)
+      throw std::runtime_error("gpb: cannot create a context on device " + std::to_string(device));
+    std::lock_guard<std::mutex> g(g_mu);
+    g_ctx[device] = c;
+  }
+  return DeviceLease{c, std::move(l)};
+}
+
+void check(int rc, gpb_ctx* ctx) {
+  if (rc == GPB_OK) return;
+  const std::string msg = std::string("gpb: ") + (ctx ? gpb_ctx_last_error(ctx) : "error");
+  if (rc == GPB_ERR_ARG) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+std::vector<float> codebook_for(const std::vector<uint32_t>& packed, int W, int H) {
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    if (!g_codebook.empty()) return g_codebook;
+  }
+  std::vector<float> cb(13 * 32);
+  check(gpb_make_codebook(packed.data(), W, H, 1, 10, 65536, cb.data()), nullptr);
+  return cb;
+}
+
+// [0,1] doubles (channel 0 = r) -> packed 0x00BBGGRR words
+std::vector<uint32_t> pack_image(const Mat& im) {
+  if (im.channels != 3 || im.rows < 1 || im.cols < 1 || im.data.size() != size_t(im.rows) * im.cols * 3)
+    throw std::invalid_argument("gpb: expected a non-empty 3-channel image");
+  const size_t N = size_t(im.rows) * im.cols;
+  std::vector<uint8_t> rgb(N * 3);
+  for (size_t i = 0; i < N * 3; i++) {
+    double v = std::nearbyint(im.data[i] * 255.0);
+    rgb[i] = uint8_t(v < 0 ? 0 : v > 255 ? 255 : v);
+  }
+  std::vector<uint32_t> packed(N);
+  gpb_pack_rgb(rgb.data(), N, packed.data());
+  return packed;
+}
+
+Mat* plane_to_mat(const float* p, int H, int W) {
+  Mat* m = new Mat(H, W);
+  for (size_t i = 0; i < size_t(H) * W; i++) m->data[i] = p[i];
+  return m;
+}
+
+void set_eigs(gpb_ctx* ctx, int nvec) {
+  if (nvec < 2 || nvec > 32) throw std::invalid_argument("gpb: nvec must be in [2, 32]");
+  const int keep = std::max(20, nvec + 8);
+  check(gpb_set_eig_options(ctx, nvec, std::max(40, 2 * keep), keep, 1e-3, 20000), ctx);
+}
+
+// textons + cues (+ mPb_all) of one image on a leased context
+void local_cues(gpb_ctx* ctx, const std::vector<uint32_t>& packed, int W, int H, std::vector<uint8_t>& labels,
+                std::vector<float>& cues, std::vector<float>& mpb_all) {
+  const size_t N = size_t(W) * H;
+  const std::vector<float> cb = codebook_for(packed, W, H);
+  labels.resize(N);
+  cues.resize(size_t(kCues) * kOri * N);
+  mpb_all.resize(size_t(kOri) * N);
+  check(gpb_texton(ctx, W, H, packed.data(), cb.data(), labels.data()), ctx);
+  check(gpb_local_cues(ctx, W, H, packed.data(), labels.data(), cues.data(), mpb_all.data()), ctx);
+}
+
+void fill_detmpb(detmpb& d, const std::vector<float>& cues, const std::vector<uint8_t>& labels, int H, int W) {
+  const size_t N = size_t(W) * H;
+  std::vector<Mat*> planes[kCues];
+  for (int c = 0; c < kCues; c++)
+    for (int o = 0; o < kOri; o++) planes[c].push_back(plane_to_mat(&cues[(size_t(c) * kOri + o) * N], H, W));
+  Mat* t = new Mat(H, W);
+  for (size_t i = 0; i < N; i++) t->data[i] = labels[i];
+  d.Assign(planes, t);
+}
+
+void write_yaml_matrix(FILE* f, const std::string& name, const Mat& m) {
+  std::fprintf(f, "%s: !!opencv-matrix\n   rows: %d\n   cols: %d\n   dt: d\n   data: [", name.c_str(), m.rows, m.cols);
+  const size_t n = size_t(m.rows) * m.cols;
+  for (size_t i = 0; i < n; i++) {
+    std::fprintf(f, "%s%.17g", i == 0 ? " " : (i % 4 == 0 ? ",\n       " : ", "), m.data[i * m.channels]);
+  }
+  std::fprintf(f, " ]\n");
+}
+
+}  // namespace
+
+detmpb::~detmpb() { clear(); }
+
+void detmpb::clear() {
+  for (auto& v : cue_) {
+    for (Mat* m : v) delete m;
+    v.clear();
+  }
+  delete textons_;
+  textons_ = nullptr;
+}
+
+void detmpb::Assign(std::vector<Mat*> (&cues)[12], Mat* textons) {
+  clear();
+  for (int c = 0; c < 12; c++) cue_[c].swap(cues[c]);
+  textons_ = textons;
+}
+
+void set_device(int device) {
+  std::lock_guard<std::mutex> g(g_mu);
+  g_device = device;
+}
+
+void set_texton_codebook(const float* codebook) {
+  std::lock_guard<std::mutex> g(g_mu);
+  if (codebook)
+    g_codebook.assign(codebook, codebook + 13 * 32);
+  else
+    g_codebook.clear();
+}
+
+static int current_device() {
+  std::lock_guard<std::mutex> g(g_mu);
+  return g_device;
+}
+
+void det_mPb(detmpb& returnVals, Mat& im) {
+  const std::vector<uint32_t> packed = pack_image(im);
+  const int H = im.rows, W = im.cols;
+  const size_t N = size_t(W) * H;
+  std::vector<uint8_t> labels;
+  std::vector<float> cues, mpb_all;
+  {
+    DeviceLease d = lease(current_device());
+    local_cues(d.ctx, packed, W, H, labels, cues, mpb_all);
+  }
+  fill_detmpb(returnVals, cues, labels, H, W);
+}
+
+void multiscalePb(Mat*& mPb_nmax, Mat*& mPb_nmax_rsz, detmpb& detmpb_obj, Mat& imd, Size orig_size, int nChan,
+                  std::string outFile, double rsz) {
+  (void)orig_size;
+  (void)outFile;
+  if (nChan != 3) throw std::invalid_argument("gpb: multiscalePb needs a 3-channel image");
+  if (rsz < 1.0) throw std::invalid_argument("gpb: rsz < 1 is not supported");
+  const std::vector<uint32_t> packed = pack_image(imd);
+  const int H = imd.rows, W = imd.cols;
+  const size_t N = size_t(W) * H;
+  std::vector<uint8_t> labels;
+  std::vector<float> cues, mpb_all, nmax(N);
+  {
+    DeviceLease d = lease(current_device());
+    local_cues(d.ctx, packed, W, H, labels, cues, mpb_all);
+    check(gpb_nonmax(d.ctx, W, H, mpb_all.data(), nmax.data()), d.ctx);
+  }
+  fill_detmpb(detmpb_obj, cues, labels, H, W);
+  mPb_nmax = plane_to_mat(nmax.data(), H, W);
+  mPb_nmax_rsz = mPb_nmax;
+}
+
+void spectralPb(std::vector<Mat*>& sPb, Mat* mPb, Size orig_size, std::string outFile2, int nvec) {
+  (void)orig_size;
+  (void)outFile2;
+  if (!mPb || mPb->channels != 1 || mPb->rows < 2 || mPb->cols < 2)
+    throw std::invalid_argument("gpb: spectralPb needs a single-channel mPb of at least 2 x 2");
+  const int H = mPb->rows, W = mPb->cols;
+  const size_t N = size_t(W) * H;
+  std::vector<float> m(N), spb(size_t(kOri) * N);
+  for (size_t i = 0; i < N; i++) m[i] = float(mPb->data[i]);
+  {
+    DeviceLease d = lease(current_device());
+    set_eigs(d.ctx, nvec);
+    check(gpb_spectral_pb(d.ctx, W, H, m.data(), spb.data(), nullptr), d.ctx);
+  }
+  sPb.resize(kOri);
+  for (int o = 0; o < kOri; o++) sPb[o] = plane_to_mat(&spb[size_t(o) * N], H, W);
+}
+
+void buildW(std::vector<double>& valD, std::vector<double>& val, std::vector<int>& I, std::vector<int>& J, Mat& l1,
+            Mat& l2) {
+  const int H = l2.rows, W = l1.cols;
+  if (l1.channels != 1 || l2.channels != 1 || H < 2 || W < 2 || l1.rows != H + 1 || l2.cols != W + 1)
+    throw std::invalid_argument("gpb: buildW expects l1 of (H+1) x W and l2 of H x (W+1)");
+  const size_t N = size_t(W) * H;
+  std::vector<float> m(N);
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const double a = l1.at(y + 1, x);
+      if (a != l2.at(y, x + 1)) throw std::invalid_argument("gpb: l1 and l2 must hold the same mPb map");
+      m[size_t(y) * W + x] = float(a);
+    }
+  std::vector<float> wup(size_t(40) * N), deg(N);
+  {
+    DeviceLease d = lease(current_device());
+    check(gpb_build_w(d.ctx, W, H, m.data(), wup.data(), deg.data()), d.ctx);
+  }
+  // the 40 offsets with dy > 0 or (dy == 0, dx > 0) inside the radius-5 disc, row-major
+  int odx[40], ody[40], k = 0;
+  for (int dy = 0; dy <= 5; dy++)
+    for (int dx = -5; dx <= 5; dx++)
+      if (dx * dx + dy * dy <= 25 && (dy > 0 || dx > 0)) {
+        odx[k] = dx;
+        ody[k] = dy;
+        k++;
+      }
+  valD.assign(N, 0.0);
+  val.clear();
+  I.clear();
+  J.clear();
+  val.reserve(N * 21);
+  I.reserve(N * 21);
+  J.reserve(N * 21);
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const int p = y * W + x;
+      valD[size_t(p)] = deg[size_t(p)];
+      I.push_back(p);
+      J.push_back(p);
+      val.push_back(1.0);
+      for (int e = 0; e < 40; e++) {
+        const int qx = x + odx[e], qy = y + ody[e];
+        if (qx < 0 || qx >= W || qy >= H) continue;
+        I.push_back(p);
+        J.push_back(qy * W + qx);
+        val.push_back(wup[size_t(e) * N + size_t(p)]);
+      }
+    }
+}
+
+bool read_ppm(const std::string& path, int& width, int& height, std::vector<uint8_t>& rgb) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  std::string magic;
+  f >> magic;
+  if (magic != "P6") return false;
+  int vals[3], got = 0;
+  while (got < 3 && f) {
+    f >> std::ws;
+    if (f.peek() == '#') {
+      std::string line;
+      std::getline(f, line);
+      continue;
+    }
+    f >> vals[got++];
+  }
+  if (got < 3 || vals[0] < 1 || vals[1] < 1 || vals[2] != 255) return false;
+  f.get();  // the single whitespace byte before the raster
+  width = vals[0];
+  height = vals[1];
+  rgb.resize(size_t(width) * height * 3);
+  f.read(reinterpret_cast<char*>(rgb.data()), std::streamsize(rgb.size()));
+  return bool(f);
+}
+
+void write_opencv_yaml(const std::string& path, const std::vector<std::pair<std::string, const Mat*>>& mats) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) throw std::runtime_error("gpb: cannot write " + path);
+  std::fprintf(f, "%%YAML:1.0\n---\n");
+  for (const auto& nm : mats) write_yaml_matrix(f, nm.first, *nm.second);
+  std::fclose(f);
+}
+
+void GlobalPb(std::string imgFile, std::string outFile, std::vector<Mat>& gPb, double rsz) {
+  if (rsz != 1.0) throw std::invalid_argument("gpb: GlobalPb supports rsz = 1 only");
+  int W = 0, H = 0;
+  std::vector<uint8_t> rgb;
+  if (!read_ppm(imgFile, W, H, rgb)) throw std::invalid_argument("gpb: cannot read binary PPM " + imgFile);
+  const size_t N = size_t(W) * H;
+  std::vector<uint32_t> packed(N);
+  gpb_pack_rgb(rgb.data(), N, packed.data());
+  const std::vector<float> cb = codebook_for(packed, W, H);
+  std::vector<float> orient(size_t(kOri) * N), nmax(N);
+  {
+    DeviceLease d = lease(current_device());
+    set_eigs(d.ctx, 17);
+    check(gpb_compute(d.ctx, W, H, packed.data(), cb.data(), nullptr, nullptr, orient.data(), nmax.data()), d.ctx);
+  }
+  gPb.assign(kOri, Mat());
+  for (int o = 0; o < kOri; o++) {
+    gPb[o] = Mat(H, W);
+    for (size_t i = 0; i < N; i++) gPb[o].data[i] = orient[size_t(o) * N + i];
+  }
+  if (!outFile.empty()) {
+    std::vector<std::pair<std::string, const Mat*>> out;
+    std::vector<std::string> names;
+    for (int o = 0; o < kOri; o++) out.emplace_back("gPb_" + std::to_string(o), &gPb[o]);
+    write_opencv_yaml(outFile + "_gPb.yml", out);
+    Mat m(H, W);
+    for (size_t i = 0; i < N; i++) m.data[i] = nmax[i];
+    write_opencv_yaml(outFile + "_mPb.yml", {{"mPb_rsz_mat", &m}});
+  }
+}
+
+}  // namespace gpb_ucm
+
+void computeGPb(unsigned int rank, unsigned int width, unsigned int height, unsigned int* data, float** hostGPb,
+                float** hostGPbAllConcat) {
+  using namespace gpb_ucm;
+  float* g = nullptr;
+  float* all = nullptr;
+  try {
+    if (!data || !hostGPb || !hostGPbAllConcat || width < 2 || height < 2)
+      throw std::invalid_argument("gpb: computeGPb: bad arguments");
+    const int ndev = gpb_device_count();
+    if (ndev < 1) throw std::runtime_error("gpb: no GPU");
+    const size_t N = size_t(width) * height;
+    std::vector<uint32_t> packed(data, data + N);
+    const std::vector<float> cb = codebook_for(packed, int(width), int(height));
+    g = static_cast<float*>(std::malloc(sizeof(float) * N));
+    all = static_cast<float*>(std::malloc(sizeof(float) * N * 8));
+    if (!g || !all) throw std::runtime_error("gpb: out of host memory");
+    DeviceLease d = lease(int(rank % unsigned(ndev)));
+    set_eigs(d.ctx, 9);
+    check(gpb_compute(d.ctx, int(width), int(height), packed.data(), cb.data(), g, all, nullptr, nullptr), d.ctx);
+    *hostGPb = g;
+    *hostGPbAllConcat = all;
+  } catch (const std::exception& e) {
+    std::free(g);
+    std::free(all);
+    std::fprintf(stderr, "%s\n", e.what());
+    throw int64_t(1);
+  }
+}
+
+int getCudaDeviceCount() { return gpb_device_count(); }

@@ -68,6 +68,8 @@ class Stats(C.Structure):
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
 _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_get_spmv_tile", C.c_int, [_ctxp, C.c_int, C.c_int, C.POINTER(C.c_int)])
 _sig("gpb_apply_laplacian", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, C.c_int, C.c_int, _f32p, _f32p])
 _sig("gpb_kernel_time", C.c_int, [_ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double)])
 KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb"]
@@ -129,6 +131,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()
+        self.nvec = 9
         rc = lib.gpb_ctx_create(device, C.byref(h))
         if rc != GPB_OK:
             raise GpbError(rc, f"cannot create a context on device {device}")
@@ -188,6 +191,7 @@ class Context:
 
     def set_eig_options(self, nvec=9, basis=48, keep=24, tol=1e-3, max_matvecs=20000):
         self._chk(lib.gpb_set_eig_options(self.h, nvec, basis, keep, tol, max_matvecs))
+        self.nvec = nvec
 
     def stats(self) -> dict:
         s = Stats()
@@ -205,17 +209,21 @@ class Context:
         self._chk(lib.gpb_build_w(self.h, W, H, m.reshape(-1), wu.reshape(-1), D.reshape(-1)))
         return wu, D
 
-    def eigs(self, mpb_nmax: np.ndarray, nvec: int = 9, with_vectors: bool = True):
+    def eigs(self, mpb_nmax: np.ndarray, with_vectors: bool = True):
+        """The ctx's nvec smallest generalised eigenpairs (set_eig_options)."""
         m = np.ascontiguousarray(mpb_nmax, np.float32)
         H, W = m.shape
+        nvec = self.nvec
         ev = np.empty(nvec, np.float64)
         vec = np.empty((nvec, H, W), np.float32) if with_vectors else None
         self._chk(lib.gpb_eigs(self.h, W, H, m.reshape(-1), ev, vec.ctypes.data if vec is not None else None))
         return ev, vec
 
-    def spectral_pb(self, mpb_nmax: np.ndarray, nvec: int = 9):
+    def spectral_pb(self, mpb_nmax: np.ndarray):
+        """sPb [8,H,W] and the ctx's nvec eigenvalues (set_eig_options)."""
         m = np.ascontiguousarray(mpb_nmax, np.float32)
         H, W = m.shape
+        nvec = self.nvec
         spb = np.empty((8, H, W), np.float32)
         ev = np.empty(nvec, np.float64)
         self._chk(lib.gpb_spectral_pb(self.h, W, H, m.reshape(-1), spb.reshape(-1), ev.ctypes.data))
@@ -259,6 +267,15 @@ class Context:
 
     def set_block_size(self, block: int):
         self._chk(lib.gpb_set_block_size(self.h, block))
+
+    def set_spmv_tile(self, rows: int):
+        """SpMV rows per thread (2, 4, 8), or 0 to autotune per image shape."""
+        self._chk(lib.gpb_set_spmv_tile(self.h, rows))
+
+    def spmv_tile(self, width: int, height: int) -> int:
+        r = C.c_int(0)
+        self._chk(lib.gpb_get_spmv_tile(self.h, width, height, C.byref(r)))
+        return r.value
 
     def apply_laplacian(self, mpb_nmax: np.ndarray, x: np.ndarray, block: int = 1) -> np.ndarray:
         m = np.ascontiguousarray(mpb_nmax, np.float32)
