@@ -50,6 +50,19 @@ def cpu_baseline(packed, codebook, crop: int, tol: float):
                        f"same image, eigen-residual <= {tol:g}; {dt:.2f} s")
 
 
+def measured_traffic(kernel_class: str, size: int):
+    """HBM bytes per launch of a kernel class, from the committed rocprofv3 PMC
+    summary (FETCH_SIZE x its copy-kernel calibration + WRITE_SIZE, per
+    dispatch, averaged over the class's dispatches) for this image size, or
+    None when no summary covers it."""
+    path = ROOT / "profiles" / f"traffic_{size}.json"
+    if not path.exists():
+        return None
+    d = json.loads(path.read_text())
+    c = d.get("classes", {}).get(kernel_class)
+    return round(c["bytes_per_launch"]) if c else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,8 +111,8 @@ def main():
     torch.cuda.synchronize()
 
     ctx = gpb.Context(0)
-    basis = args.basis or (48 if args.block == 1 else 96)
-    keep = args.keep or (24 if args.block == 1 else 48)
+    basis = args.basis or (40 if args.block == 1 else 192)
+    keep = args.keep or (20 if args.block == 1 else 64)
     ctx.set_eig_options(nvec=9, basis=basis, keep=keep, tol=tol, max_matvecs=40000)
     ctx.set_block_size(args.block)
     ctx.set_spmv_tile(args.spmv_tile)
@@ -131,8 +144,9 @@ def main():
     d = kt[dom]
     avg_ms = d["ms"] / max(1, d["launches"])
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    traffic = measured_traffic(dom, S)
     roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
                     bytes_per_launch=round(d["bytes"] / max(1, d["launches"])),
                     share_of_step=round(d["ms"] / (ms_step * args.steps), 4))

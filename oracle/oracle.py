@@ -236,3 +236,111 @@ def pipeline(packed: np.ndarray, codebook: np.ndarray, K: int = 9, tol: float = 
     gmax, gall = gpb_outputs(g, nm)
     return dict(textons=t, cues=cues, mpb_all=mpb_all, mpb_nmax=nm, Wd=Wd, D=D, evals=ev, evecs=vec, iters=it,
                 spb=s, gpb=g, gpb_max=gmax, gpb_all=gall)
+
+
+# ---- the reference's own buildW kernels (oracle/_ref, built by `make ref`) ----
+REF_BUILDW = Path(__file__).resolve().parent / "_ref" / "libref_buildw.so"
+
+
+def ref_build_w(mpb_nmax: np.ndarray):
+    """W (dense N x N) and its row sums from the reference's computeSupport +
+    computeAffinities2 (buildW/src/ic.cc, affinity.cc, smatrix.cc) on the
+    lattice spectralPb builds from mPb; None if _ref is not built."""
+    if not REF_BUILDW.exists():
+        return None
+    import ctypes as C
+    lib = C.CDLL(str(REF_BUILDW))
+    f = lib.ref_build_w
+    f.restype = C.c_int
+    m = np.ascontiguousarray(mpb_nmax, np.float64)
+    H, W = m.shape
+    N = H * W
+    cap = N * 81
+    rows = np.empty(cap, np.int32)
+    cols = np.empty(cap, np.int32)
+    vals = np.empty(cap, np.float64)
+    rsum = np.empty(N, np.float64)
+    nnz = C.c_int(cap)
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = f(C.c_int(W), C.c_int(H), P(m), C.byref(nnz), P(rows), P(cols), P(vals), P(rsum))
+    if rc != 0:
+        raise RuntimeError(f"ref_build_w failed ({rc})")
+    n = nnz.value
+    A = np.zeros((N, N))
+    A[rows[:n], cols[:n]] = vals[:n]
+    return A, rsum, (rows[:n].copy(), cols[:n].copy())
+
+
+# ---- the reference's own lib_image (oracle/_ref, built by `make ref`) ----
+REF_LIBIMAGE = Path(__file__).resolve().parent / "_ref" / "libref_libimage.so"
+_REFL = None
+
+
+def _refl():
+    global _REFL
+    if _REFL is None and REF_LIBIMAGE.exists():
+        _REFL = C.CDLL(str(REF_LIBIMAGE))
+    return _REFL
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def ref_gaussian(sigma: float) -> np.ndarray:
+    """lib_image::gaussian(sigma) (lib_image.hh:404)."""
+    lib = _refl()
+    buf = np.empty(4096, np.float64)
+    n = lib.ref_gaussian(C.c_double(sigma), _p(buf), C.c_int(buf.size))
+    return buf[:n].copy()
+
+
+def ref_lab_quantize(rgb: np.ndarray, nbins: int = 25) -> np.ndarray:
+    """Gamma 2.5 / Lab / normalise / quantise through lib_image; rgb [..., 3] u8 -> bins [3, ...]."""
+    lib = _refl()
+    a = np.ascontiguousarray(rgb, np.uint8)
+    n = a.size // 3
+    out = np.empty((3, n), np.uint8)
+    lib.ref_lab_quantize(_p(a), C.c_int(n), C.c_int(nbins), _p(out))
+    return out.reshape((3,) + a.shape[:-1])
+
+
+def ref_mirror(img: np.ndarray, border: int) -> np.ndarray:
+    lib = _refl()
+    m = np.ascontiguousarray(img, np.float64)
+    H, W = m.shape
+    out = np.empty((H + 2 * border, W + 2 * border), np.float64)
+    if lib.ref_mirror(_p(m), C.c_int(W), C.c_int(H), C.c_int(border), _p(out)) != 0:
+        raise ValueError("lib_image::border_mirror_2D rejected the arguments")
+    return out
+
+
+def ref_hist_gradient(labels: np.ndarray, r: int, ksigma: float) -> np.ndarray:
+    """lib_image::hist_gradient_2D(labels, r, 8, gaussian(ksigma) or none) -> [8, H, W] (untrimmed)."""
+    lib = _refl()
+    lm = np.ascontiguousarray(labels, np.uint8)
+    H, W = lm.shape
+    out = np.empty((8, H, W), np.float64)
+    lib.ref_hist_gradient(_p(lm), C.c_int(W), C.c_int(H), C.c_int(r), C.c_double(ksigma), _p(out))
+    return out
+
+
+def ref_nonmax(mpb_all: np.ndarray) -> np.ndarray:
+    """nonmax_channels + lib_image::nonmax_oriented_2D + multiscalePb's clip."""
+    lib = _refl()
+    m = np.ascontiguousarray(mpb_all, np.float64)
+    _, H, W = m.shape
+    out = np.empty((H, W), np.float64)
+    lib.ref_nonmax(_p(m), C.c_int(W), C.c_int(H), _p(out))
+    return out
+
+
+REF_FILTERS = Path(__file__).resolve().parent / "_ref" / "libref_filters.so"
+
+
+def ref_spb_filter(ori: float) -> np.ndarray:
+    """damascene gaussian_2D(1, 1, ori, deriv 1, support 3, 3) as 7 x 7 (x-major), FP32."""
+    lib = C.CDLL(str(REF_FILTERS))
+    f = np.empty(49, np.float32)
+    lib.ref_spb_filter(C.c_double(ori), _p(f))
+    return f.reshape(7, 7)
