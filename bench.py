@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--regions", type=int, default=400)
-    ap.add_argument("--cpu-crop", type=int, default=224)
+    ap.add_argument("--cpu-crop", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=1, help="Lanczos block size (1 = single-vector)")
     ap.add_argument("--basis", type=int, default=0)

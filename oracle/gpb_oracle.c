@@ -13,8 +13,8 @@
  *     mex_pb_parts_final_selected.cpp:104-119; the GPU path decodes R, G, B);
  *   - texton labels are the true argmin (SPFeatures.cpp:328-339 never updates
  *     minDistance, so every label is 31);
- *   - borders wider than the image reflect periodically (the reference reads
- *     out of bounds there);
+ *   - borders wider than the image reflect periodically (lib_image's
+ *     border_mirror_2D throws there, lib_image.cc:308-370);
  *   - nonmax_channels starts each pixel at -inf with its index reset
  *     (nonmax_channels.cpp:17-34 carries both across pixels);
  *   - the operator is the full symmetric D - W (spectralPb.cpp:199-209 keeps
