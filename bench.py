@@ -63,6 +63,51 @@ def measured_traffic(kernel_class: str, size: int):
     return round(c["bytes_per_launch"]) if c else None
 
 
+def bench_patches(gpb, ctx, tol: float, size: int = 5000, regions: int = 2500, grid: int = 4, overlap: int = 16,
+                  seed: int = 4):
+    """The batched form: a 5000x5000 seeded Voronoi image (2500 regions, sigma 8,
+    seed 4) cut into a 4x4 grid with seeded cut offsets and a 16-px overlap, the
+    16 patches' whole gPb computed in one batched Lanczos (mode 1) and, for
+    comparison, with one solve per patch (mode 0); stitched outputs in HBM.
+    Mpixel/s counts the image's pixels once (not the overlap)."""
+    import numpy as np
+    import torch
+
+    rgb = gpb.synth_voronoi(size, size, regions, 8.0, seed)
+    packed = gpb.pack_rgb(rgb)
+    del rgb
+    cb = gpb.make_codebook(packed, seed=seed, iters=10)
+    rects, cores = gpb.cut_patches(size, size, grid, overlap, seed)
+    dev = torch.device("cuda:0")
+    d_rgb = torch.from_numpy(packed.view(np.int32)).to(dev)
+    d_cb = torch.from_numpy(cb).to(dev)
+    d_gmax = torch.zeros((size, size), dtype=torch.float32, device=dev)
+    d_orient = torch.zeros((8, size, size), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    out = {}
+    for name, mode in (("batched", 1), ("one_by_one", 0)):
+        run = lambda: ctx.compute_patches_device(size, size, d_rgb.data_ptr(), d_cb.data_ptr(), rects, cores, mode,
+                                                 d_gmax.data_ptr(), d_orient.data_ptr())
+        run()  # warm-up (tile and workgroup tuning, workspace)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = [ctx.patch_stats(p) for p in range(len(rects))]
+        out[name] = dict(value=round(size * size / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3, 1),
+                         matvecs_total=int(sum(x["matvecs"] for x in st)),
+                         max_residual=max(x["max_residual"] for x in st),
+                         stage_ms=[round(x, 1) for x in st[0]["stage_ms"]])
+    out["speedup_batched_vs_one_by_one"] = round(out["one_by_one"]["ms"] / out["batched"]["ms"], 3)
+    out["workload"] = (f"{size}x{size} seeded Voronoi ({regions} regions, sigma 8, seed {seed}), {grid}x{grid} patches "
+                       f"{int(rects[:, 2].min())}..{int(rects[:, 2].max())} x {int(rects[:, 3].min())}.."
+                       f"{int(rects[:, 3].max())} px with {overlap}-px overlap, 9 eigenvectors each, residual <= {tol:g}")
+    del d_rgb, d_gmax, d_orient
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +121,9 @@ def main():
     ap.add_argument("--basis", type=int, default=0)
     ap.add_argument("--keep", type=int, default=0)
     ap.add_argument("--spmv-tile", type=int, default=0, help="SpMV rows per thread (2, 4, 8); 0 = autotune")
+    ap.add_argument("--spmv-fp32", action="store_true", help="SpMV arithmetic in FP32 (default FP64 arithmetic)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without per-kernel HIP events")
+    ap.add_argument("--forms", default="batched", help="comma list of extra forms to time: batched (C4), none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -116,6 +163,7 @@ def main():
     ctx.set_eig_options(nvec=9, basis=basis, keep=keep, tol=tol, max_matvecs=40000)
     ctx.set_block_size(args.block)
     ctx.set_spmv_tile(args.spmv_tile)
+    ctx.set_spmv_fp64(not args.spmv_fp32)
 
     def step():
         ctx.compute_device(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), d_gmax.data_ptr(), d_gall.data_ptr())
@@ -160,12 +208,15 @@ def main():
                             form="plain", lanczos_block=st["block"], lanczos_basis=st["basis"],
                             lanczos_matvecs=st["matvecs"],
                             lanczos_restarts=st["restarts"], reorth_second_passes=st["second_passes"],
-                            max_residual=st["max_residual"], spmv_tile_rows=ctx.spmv_tile(S, S),
+                            max_residual=st["max_residual"], spmv_tile_rows=ctx.spmv_tile(S, S), spmv_arith="f32" if args.spmv_fp32 else "f64",
                             stage_ms=[round(x, 2) for x in st["stage_ms"]]),
                 roofline=roofline,
                 kernels={k: dict(ms=round(v["ms"] / args.steps, 3), launches=v["launches"] // args.steps,
                                  gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0)
                          for k, v in kt.items()})
+    forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if "batched" in forms:
+        line["forms"] = dict(batched_patches=bench_patches(gpb, ctx, tol))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(packed, codebook, args.cpu_crop, tol)
     print(json.dumps(line), flush=True)

@@ -130,6 +130,15 @@ int gpb_set_block_size(gpb_ctx* ctx, int block);
 int gpb_set_spmv_tile(gpb_ctx* ctx, int rows);
 int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows);
 
+/* Arithmetic of the stencil SpMV for later solves on ctx: 1 (default) = the
+ * differences, products and row sums in FP64, 0 = in FP32.  Operands (W, x,
+ * D^-1/2) and results stay FP32 in HBM in both, so the memory traffic is the
+ * same; FP64 removes the FP32 rounding that limits the relative accuracy of
+ * eigenvalues near 1e-4 (smooth eigenvectors, whose row sums cancel).  The
+ * reference SpMVs are FP32 (damascene/src/stencilMVM/stencilMVM.cu:129-182)
+ * and FP64 (Spectra, src/spectralPb.cpp:221-244). */
+int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
+
 /* Kernel classes for gpb_kernel_time. */
 #define GPB_K_TEXTON 0
 #define GPB_K_CUES 1
@@ -192,6 +201,42 @@ int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, con
  * any may be NULL); returns after the stream is synchronised. */
 int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
                        float* d_gpb_max, float* d_gpb_all, float* d_gpb_orient);
+
+/* ---- patches: the batched form --------------------------------------------------- */
+
+/* Cuts a width x height image into grid x grid patches (grid <= 8): interior
+ * cut lines at i * size / grid, each moved by a seeded offset of up to 14% of
+ * a cell in x and 10% in y, so patch sizes differ.  cores[p] = the disjoint
+ * cells (x, y, w, h; they tile the image), rects[p] = each cell grown by
+ * `overlap` pixels on every side that is not the image border.  Patch p is
+ * row gy = p / grid, column gx = p % grid.  Replaces the equal-size split of
+ * Img2Pieces (src/utilities.cpp:462-556), whose overlap is 0 (src/main.cpp:75,
+ * 77). */
+int gpb_cut_patches(int width, int height, int grid, int overlap, uint64_t seed, int* rects, int* cores);
+
+/* Whole gPb of each patch rectangle of one image (rects[p] = x, y, w, h, up
+ * to 64 patches), each patch an independent problem as the reference's
+ * per-rank patches are (src/main.cpp:161-301).  mode 0 solves the patches'
+ * eigenproblems one after another; mode 1 runs them all in one batched
+ * Lanczos (the persistent batched kernels, each restart cycle replayed from a
+ * hipGraph; SpMV tile and workgroup count tuned per batch shape); mode 2 is
+ * mode 1 without the graph.  Outputs are stitched from each patch's core
+ * rectangle cores[p] (disjoint; Pieces2Gpb, src/utilities.cpp:180-212):
+ * gpb_max[H][W] (masked max over orientations) and gpb_orient[8][H][W]
+ * (oriented gPb before calibration); either may be NULL.  Host buffers;
+ * gpb_compute_patches_device takes device buffers (outputs left in HBM,
+ * pixels outside every core untouched).  No counterpart in the reference
+ * beyond the MPI patch loop it replaces. */
+int gpb_compute_patches(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, int npatch,
+                        const int* rects, const int* cores, int mode, float* gpb_max, float* gpb_orient);
+int gpb_compute_patches_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
+                               int npatch, const int* rects, const int* cores, int mode, float* d_gpb_max,
+                               float* d_gpb_orient);
+
+/* Solver statistics and eigenvalues of patch `patch` of the last patch call;
+ * gpb_get_stats then holds their sum (matvecs, converged) and maximum
+ * (residual, restarts). */
+int gpb_get_patch_stats(gpb_ctx* ctx, int patch, gpb_stats* out);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 

@@ -1,0 +1,86 @@
+"""Patch decomposition and the batched form (C4).
+
+CPU: the cutter's cores tile the image exactly, rects are the cores grown by
+the overlap on interior sides, the 5000x5000 / 4x4 / 16-px configuration gives
+cells between 900x1000 and 1600x1500, and cuts are seeded.  GPU: the batched
+Lanczos (with and without its hipGraph) against one solve per patch on the
+same patches (two independent FP32-SpMV solves to residual 1e-6 from
+different start vectors: eigenvalues agree to 5e-5 relative, which is twice the
+FP32 operator's own error on eigenvalues near 1e-4, every residual at target,
+stitched gPb to 1e-4), and each patch against the oracle's whole path on that patch's
+own crop (eigenvalues to 1e-5 relative, gPb to 1e-3 over the patch core).
+"""
+import numpy as np
+import pytest
+
+from conftest import synth_packed
+
+
+def test_cut_patches_tiles_the_image(gpb):
+    for (W, H, g, ov, seed) in [(5000, 5000, 4, 16, 4), (360, 300, 3, 8, 1), (200, 150, 2, 0, 7), (97, 83, 1, 5, 2)]:
+        rects, cores = gpb.cut_patches(W, H, g, ov, seed)
+        cover = np.zeros((H, W), np.int32)
+        for (x, y, w, h) in cores:
+            cover[y:y + h, x:x + w] += 1
+        assert np.all(cover == 1), (W, H, g)
+        for (rx, ry, rw, rh), (cx, cy, cw, ch) in zip(rects, cores):
+            assert rx == max(0, cx - ov) and ry == max(0, cy - ov)
+            assert rx + rw == min(W, cx + cw + ov) and ry + rh == min(H, cy + ch + ov)
+        r2, c2 = gpb.cut_patches(W, H, g, ov, seed)
+        assert np.array_equal(r2, rects) and np.array_equal(c2, cores)
+    _, cores = gpb.cut_patches(5000, 5000, 4, 16, 4)
+    assert cores[:, 2].min() >= 900 and cores[:, 2].max() <= 1600
+    assert cores[:, 3].min() >= 1000 and cores[:, 3].max() <= 1500
+    _, other = gpb.cut_patches(5000, 5000, 4, 16, 5)
+    assert not np.array_equal(other, cores)
+    with pytest.raises(gpb.GpbError):
+        gpb.cut_patches(40, 40, 4, 16, 1)  # cells too small for the overlap
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("W,H,grid,ov", [(360, 300, 3, 8), (250, 180, 2, 16)])
+def test_batched_matches_one_by_one_gpu(gpb, ctx, W, H, grid, ov):
+    print(f"patches {W}x{H} grid {grid} overlap {ov}", flush=True)
+    rgb, packed = synth_packed(gpb, W, H, regions=24, seed=21)
+    cb = gpb.make_codebook(packed, seed=21)
+    rects, cores = gpb.cut_patches(W, H, grid, ov, 3)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=1e-6)
+    out = {}
+    for mode in (0, 1, 2, 1):  # one by one, batched, batched without graph, batched again (graph replayed)
+        gmax, orient = ctx.compute_patches(packed, cb, rects, cores, mode)
+        ps = [ctx.patch_stats(p) for p in range(len(rects))]
+        out.setdefault(mode, []).append((gmax, orient, ps))
+        print(f"  mode {mode}: matvecs {[p['matvecs'] for p in ps]} residual max "
+              f"{max(p['max_residual'] for p in ps):.2e} stage ms {[round(x, 1) for x in ps[0]['stage_ms']]}",
+              flush=True)
+        assert all(p["max_residual"] <= 1e-6 for p in ps)
+    g0, o0, s0 = out[0][0]
+    for mode, runs in out.items():
+        for (g, o, s) in runs:
+            for p in range(len(rects)):
+                e0, e1 = np.array(s0[p]["evals"][1:9]), np.array(s[p]["evals"][1:9])
+                assert np.max(np.abs(e1 - e0) / e0) <= 5e-5, (mode, p)
+            assert np.abs(g - g0).max() <= 1e-4 and np.abs(o - o0).max() <= 1e-4, mode
+    # the graph replay gives the identical result as the first batched call
+    np.testing.assert_array_equal(out[1][0][0], out[1][1][0])
+
+
+@pytest.mark.gpu
+def test_patches_vs_oracle_gpu(gpb, orc, ctx):
+    W, H = 240, 200
+    print(f"patches vs oracle {W}x{H}", flush=True)
+    rgb, packed = synth_packed(gpb, W, H, regions=14, seed=31)
+    cb = gpb.make_codebook(packed, seed=31)
+    rects, cores = gpb.cut_patches(W, H, 2, 10, 9)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=1e-6)
+    _, orient = ctx.compute_patches(packed, cb, rects, cores, 1)
+    for p, ((rx, ry, rw, rh), (cx, cy, cw, ch)) in enumerate(zip(rects, cores)):
+        sub = np.ascontiguousarray(packed[ry:ry + rh, rx:rx + rw])
+        ref = orc.pipeline(sub, cb, K=9, tol=1e-10)
+        ev = np.array(ctx.patch_stats(p)["evals"][1:9])
+        rel = np.max(np.abs(ev - ref["evals"][1:9]) / ref["evals"][1:9])
+        core_ref = ref["gpb"][:, cy - ry:cy - ry + ch, cx - rx:cx - rx + cw]
+        core_gpu = orient[:, cy:cy + ch, cx:cx + cw]
+        err = np.abs(core_gpu - core_ref).max()
+        print(f"  patch {p} {rw}x{rh}: eigenvalue rel err {rel:.2e}, gPb abs err {err:.2e}", flush=True)
+        assert rel <= 1e-5 and err <= 1e-3

@@ -1,0 +1,126 @@
+// Device code shared by the eigensolvers (single-vector, block and batched
+// Lanczos): constants, the stencil SpMV tile and block reductions.
+#pragma once
+
+#include <vector>
+
+#include "gpb_pipeline.hpp"
+
+namespace gpb {
+
+void symeig_desc(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z);
+
+constexpr int kRedThreads = 256;
+constexpr int kMaxBasis = 128;
+
+constexpr int kProjectMaxN = 64;  // widest basis the fused projection handles
+
+struct SpmvOffsets {
+  int o[kStencilHalf];
+};
+
+// Normalised Laplacian in difference form: with u = s .* x (s = D^-1/2),
+// y_p = s_p * sum_q W_pq (u_p - u_q).  Differences of neighbouring values are
+// formed before the products, so eigenvalues near 0 keep their relative
+// accuracy in FP32 (the form I - D^-1/2 W D^-1/2 would lose it to cancellation
+// against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
+// W[k][p - e_k] for q = p - e_k.  A 64x16 tile of u (plus a 5-pixel halo) is
+// staged in LDS; W streams from HBM with coalesced row reads.
+// Tiles are 64 columns by 4 * ROWS rows (64x4 threads, ROWS rows each); ROWS
+// is picked per image shape by timing (tune_spmv).
+constexpr int kSpmvTX = 64;
+constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius;
+
+struct SpmvArgs {
+  int o[kStencilHalf];   // padded linear offsets
+  int lo[kStencilHalf];  // LDS tile offsets
+};
+
+// One 64 x 4*ROWS tile of y = L x at interior origin (x0, y0); returns this
+// thread's share of x . y over the tile.  AT is the arithmetic type of the
+// differences, products and sum (float, or double with the FP32 operands
+// widened exactly); W, x, s and y stay FP32 in HBM either way.  u: LDS tile
+// of (4*ROWS+10) x 74 values of s .* x in AT.
+// Pointers known to address device global memory (HBM).  Loads through
+// them are global_load instructions; through a generic pointer read from a
+// table they would be flat loads, which also count against the LDS wait
+// counter and serialise against the tile's LDS reads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<const T> as_global(const T* p) {
+  return (gptr<const T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global_mut(T* p) {
+  return (gptr<T>)p;
+}
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <int ROWS, typename AT>
+__device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, const float* __restrict__ s_,
+                                            const float* __restrict__ x_, float* __restrict__ y_, int W, int H, int Wp,
+                                            int Hp, size_t Np, const SpmvArgs A, int x0, int y0, AT* __restrict__ u) {
+  const gptr<const float> wup = as_global(wup_);
+  const gptr<const float> s = as_global(s_);
+  const gptr<const float> x = as_global(x_);
+  const gptr<float> y = as_global_mut(y_);
+  constexpr int kSpmvTY = 4 * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
+  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
+    const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
+    const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
+    AT v = 0;
+    if (py < Hp && px < Wp) {
+      const size_t q = size_t(py) * Wp + px;
+      v = AT(s[q]) * AT(x[q]);
+    }
+    u[i] = v;
+  }
+  __syncthreads();
+  const int xi = x0 + threadIdx.x;
+  double xy = 0;
+  if (xi < W) {
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int ty = threadIdx.y + 4 * r;
+      const int yi = y0 + ty;
+      if (yi >= H) break;
+      const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
+      const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
+      // all 80 weights of the row are loaded before the first FMA: the
+      // pointers may come from a table, where no no-alias fact lets the
+      // compiler move loads ahead of earlier stores on its own
+      float wa[kStencilHalf], wb[kStencilHalf];
+#pragma unroll
+      for (int j = 0; j < kStencilHalf; j++) {
+        wa[j] = wup[size_t(j) * Np + p];
+        wb[j] = wup[size_t(j) * Np + p - A.o[j]];
+      }
+      const AT up = u[c];
+      AT acc = 0;
+#pragma unroll
+      for (int j = 0; j < kStencilHalf; j++) {
+        acc = fma_t(AT(wa[j]), up - u[c + A.lo[j]], acc);
+        acc = fma_t(AT(wb[j]), up - u[c - A.lo[j]], acc);
+      }
+      const float yp = float(AT(s[p]) * acc);
+      y[p] = yp;
+      xy += double(x[p]) * double(yp);
+    }
+  }
+  return xy;
+}
+
+// Block sum of one double per thread (256 threads); result in thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* red4) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((tid & 63) == 0) red4[tid >> 6] = v;
+  __syncthreads();
+  return red4[0] + red4[1] + red4[2] + red4[3];
+}
+
+}  // namespace gpb

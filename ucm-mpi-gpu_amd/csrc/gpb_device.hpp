@@ -159,6 +159,11 @@ struct Context {
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
   int spmv_rows_forced = 0;
+  bool spmv_fp64 = true;   // SpMV arithmetic in FP64 (operands and results FP32 in HBM)
+  bool capturing = false;
+  // per-patch solver statistics and eigenvalues of the last patch call
+  std::vector<LanczosStats> patch_stats;
+  std::vector<double> patch_evals;  // a hipGraph capture is open on the stream (no event timing)
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {

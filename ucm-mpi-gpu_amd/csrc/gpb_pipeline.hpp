@@ -33,7 +33,8 @@ struct Stencil {
   int off_host[kStencilHalf] = {};
   size_t pad_index(int x, int y) const { return size_t(y + kICRadius) * Wp + (x + kICRadius); }
 };
-void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st);
+// Workspace buffers are keyed by tag (distinct tags keep several stencils alive).
+void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, const std::string& tag = "");
 void stencil_upper_offsets(int* dx, int* dy);
 void launch_fill(Context& c, float* p, size_t n, float v);
 

@@ -8,6 +8,7 @@
 
 #include "gpb_capi.h"
 #include "gpb_pipeline.hpp"
+#include "k_batch.hpp"
 
 namespace gpb {
 
@@ -119,6 +120,102 @@ static EigOptions ctx_eig_options(const Context& c) {
   return e;
 }
 
+// Copy a patch's core rectangle of its outputs into whole-image outputs
+// (the disjoint-rectangle stitch of Pieces2Gpb, src/utilities.cpp:180-212).
+__global__ __launch_bounds__(256) void stitch_kernel(const float* __restrict__ gmax_p,
+                                                     const float* __restrict__ orient_p, int pw, int ph, int rx,
+                                                     int ry, int cx, int cy, int cw, int ch, int W, int H,
+                                                     float* __restrict__ gmax, float* __restrict__ orient) {
+  const int x = cx + int(blockIdx.x * blockDim.x + threadIdx.x), y = cy + int(blockIdx.y);
+  if (x >= cx + cw || y >= cy + ch) return;
+  const size_t pi = size_t(y - ry) * pw + (x - rx), fi = size_t(y) * W + x;
+  const size_t Np = size_t(pw) * ph, N = size_t(W) * H;
+  if (gmax) gmax[fi] = gmax_p[pi];
+  if (orient)
+    for (int o = 0; o < kNumOri; o++) orient[o * N + fi] = orient_p[o * Np + pi];
+}
+
+// Whole gPb of every patch rectangle of one image (device buffers): stages
+// (a)-(c) per patch, the eigensolves batched (mode 1) or one after another
+// (mode 0), stages (e) per patch, and each patch's core rectangle stitched
+// into the whole-image outputs.  Stage times: [0] (a)-(c) for all patches,
+// [4] eigensolves, [5] (e) + stitch.
+static void run_patches(Context& c, int W, int H, const uint32_t* d_packed, const float* d_cb, int np,
+                        const int* rects, const int* cores, int mode, const BatchOptions& bo, float* d_gmax,
+                        float* d_orient) {
+  const EigOptions eo = ctx_eig_options(c);
+  const int K = eo.K;
+  std::vector<Stencil> sts(static_cast<size_t>(np));
+  std::vector<float*> evecs(static_cast<size_t>(np)), gcue(static_cast<size_t>(np)), nmax(static_cast<size_t>(np));
+  StageEvents se;
+  GPB_HIP(hipEventRecord(se.ev[0], c.stream));
+  for (int p = 0; p < np; p++) {
+    const int x0 = rects[4 * p], y0 = rects[4 * p + 1], w = rects[4 * p + 2], h = rects[4 * p + 3];
+    const std::string tag = "bp" + std::to_string(p) + ".";
+    const size_t N = size_t(w) * h;
+    uint32_t* rgb = c.ws<uint32_t>(tag + "rgb", N);
+    GPB_HIP(hipMemcpy2DAsync(rgb, size_t(w) * 4, d_packed + size_t(y0) * W + x0, size_t(W) * 4, size_t(w) * 4, h,
+                             hipMemcpyDeviceToDevice, c.stream));
+    uint8_t* text = c.ws<uint8_t>(tag + "text", N);
+    float* mpb_all = c.ws<float>(tag + "mpb_all", 8 * N);
+    gcue[size_t(p)] = c.ws<float>(tag + "gcue", 8 * N);
+    nmax[size_t(p)] = c.ws<float>(tag + "nmax", N);
+    launch_texton(c, rgb, w, h, d_cb, text);
+    launch_local_cues(c, rgb, text, w, h, mpb_all, gcue[size_t(p)], nullptr);
+    launch_nonmax(c, mpb_all, w, h, nmax[size_t(p)]);
+    build_stencil(c, nmax[size_t(p)], w, h, sts[size_t(p)], tag);
+    evecs[size_t(p)] = c.ws<float>(tag + "evecs", size_t(K) * sts[size_t(p)].Np);
+  }
+  for (int s = 1; s <= 4; s++) GPB_HIP(hipEventRecord(se.ev[s], c.stream));
+  std::vector<double> ev(size_t(np) * K);
+  if (mode == 1) {
+    lanczos_solve_batched(c, sts, eo, bo, ev, evecs, c.patch_stats);
+  } else {
+    c.patch_stats.assign(size_t(np), LanczosStats());
+    for (int p = 0; p < np; p++)
+      lanczos_solve(c, sts[size_t(p)], eo, &ev[size_t(p) * K], evecs[size_t(p)], c.patch_stats[size_t(p)]);
+  }
+  c.patch_evals = ev;
+  GPB_HIP(hipEventRecord(se.ev[5], c.stream));
+  for (int p = 0; p < np; p++) {
+    const int w = rects[4 * p + 2], h = rects[4 * p + 3];
+    const std::string tag = "bp" + std::to_string(p) + ".";
+    const size_t N = size_t(w) * h;
+    float* spb = c.ws<float>(tag + "spb", 8 * N);
+    float* orient = c.ws<float>(tag + "orient", 8 * N);
+    float* gmax = c.ws<float>(tag + "gmax", N);
+    launch_spb(c, sts[size_t(p)], evecs[size_t(p)], &ev[size_t(p) * K], K, spb);
+    launch_gpb(c, w, h, gcue[size_t(p)], spb, nmax[size_t(p)], orient, gmax, nullptr);
+    const int cw = cores[4 * p + 2], ch = cores[4 * p + 3];
+    if ((d_gmax || d_orient) && cw > 0 && ch > 0) {
+      const int cx = cores[4 * p], cy = cores[4 * p + 1];
+      hipLaunchKernelGGL(stitch_kernel, dim3(div_up(cw, 256), ch), dim3(256), 0, c.stream, gmax, orient, w, h,
+                         rects[4 * p], rects[4 * p + 1], cx, cy, cw, ch, W, H, d_gmax, d_orient);
+      GPB_LAUNCH_CHECK();
+    }
+  }
+  GPB_HIP(hipEventRecord(se.ev[6], c.stream));
+  GPB_HIP(hipStreamSynchronize(c.stream));
+  c.kt.fold();
+  for (int s = 0; s < kNumStages; s++) {
+    float ms = 0;
+    GPB_HIP(hipEventElapsedTime(&ms, se.ev[s], se.ev[s + 1]));
+    c.stage_ms[s] = ms;
+  }
+  // the context's last-solve statistics summarise the batch
+  LanczosStats sum;
+  for (const LanczosStats& st : c.patch_stats) {
+    sum.iterations += st.iterations;
+    sum.restarts = std::max(sum.restarts, st.restarts);
+    sum.basis = st.basis;
+    sum.block = st.block;
+    sum.converged += st.converged;
+    sum.max_residual = std::max(sum.max_residual, st.max_residual);
+    sum.second_passes = st.second_passes;
+  }
+  c.last_lanczos = sum;
+}
+
 }  // namespace gpb
 
 using namespace gpb;
@@ -185,6 +282,13 @@ int gpb_set_spmv_tile(gpb_ctx* ctx, int rows) {
   });
 }
 
+int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64) {
+  return guarded2(ctx, [&] {
+    if (fp64 != 0 && fp64 != 1) throw ArgError("SpMV precision must be 0 (FP32) or 1 (FP64 arithmetic)");
+    ctx->c.spmv_fp64 = fp64 != 0;
+  });
+}
+
 int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows) {
   return guarded2(ctx, [&] {
     if (!rows) throw ArgError("null pointer");
@@ -192,7 +296,8 @@ int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows) {
       *rows = ctx->c.spmv_rows_forced;
       return;
     }
-    auto it = ctx->c.spmv_rows.find((uint64_t(width) << 32) | uint32_t(height));
+    auto it = ctx->c.spmv_rows.find(((uint64_t(width) << 32) | uint32_t(height)) ^
+                                    (ctx->c.spmv_fp64 ? (1ull << 62) : 0));
     *rows = it == ctx->c.spmv_rows.end() ? 0 : it->second;
   });
 }
@@ -392,6 +497,124 @@ extern "C" int gpb_apply_laplacian(gpb_ctx* ctx, int width, int height, const fl
       debug_block_spmv(c, st, block, dx, dy);
     }
     unpad_to_host(c, dy, ncols, width, height, y);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+// ---- patches ---------------------------------------------------------------------------
+
+int gpb_cut_patches(int width, int height, int grid, int overlap, uint64_t seed, int* rects, int* cores) {
+  if (width < 2 || height < 2 || grid < 1 || grid > 8 || overlap < 0 || !rects || !cores) return GPB_ERR_ARG;
+  if (width / grid < 2 * overlap + 8 || height / grid < 2 * overlap + 8) return GPB_ERR_ARG;
+  // interior cuts at i * size / grid, moved by a seeded offset of up to 14% (x)
+  // and 10% (y) of the cell, so that cells differ in size
+  auto cuts = [&](int size, double frac, uint64_t stream, std::vector<int>& c) {
+    c.assign(size_t(grid + 1), 0);
+    c[size_t(grid)] = size;
+    const double cell = double(size) / grid;
+    for (int i = 1; i < grid; i++) {
+      uint64_t z = seed * 0x9E3779B97F4A7C15ull + stream * 0xD1B54A32D192ED03ull + uint64_t(i) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      const double u = double(z >> 11) * (1.0 / 9007199254740992.0);
+      c[size_t(i)] = int(std::lround(i * cell + (2 * u - 1) * frac * cell));
+    }
+  };
+  std::vector<int> cx, cy;
+  cuts(width, 0.14, 1, cx);
+  cuts(height, 0.10, 2, cy);
+  for (int gy = 0; gy < grid; gy++)
+    for (int gx = 0; gx < grid; gx++) {
+      const int p = gy * grid + gx;
+      int* c = cores + 4 * p;
+      c[0] = cx[size_t(gx)];
+      c[1] = cy[size_t(gy)];
+      c[2] = cx[size_t(gx + 1)] - cx[size_t(gx)];
+      c[3] = cy[size_t(gy + 1)] - cy[size_t(gy)];
+      const int x0 = std::max(0, c[0] - overlap), y0 = std::max(0, c[1] - overlap);
+      const int x1 = std::min(width, c[0] + c[2] + overlap), y1 = std::min(height, c[1] + c[3] + overlap);
+      int* r = rects + 4 * p;
+      r[0] = x0;
+      r[1] = y0;
+      r[2] = x1 - x0;
+      r[3] = y1 - y0;
+    }
+  return GPB_OK;
+}
+
+int gpb_compute_patches_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
+                        int npatch, const int* rects, const int* cores, int mode, float* d_gpb_max,
+                        float* d_gpb_orient) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!d_packed || !d_codebook || !rects || !cores) throw ArgError("null pointer");
+    if (npatch < 1 || npatch > kMaxPatches) throw ArgError("patch count out of range");
+    if (mode < 0 || mode > 2) throw ArgError("mode must be 0 (one solve per patch), 1 (batched) or 2 (batched, no graph)");
+    for (int p = 0; p < npatch; p++) {
+      const int* r = rects + 4 * p;
+      const int* c = cores + 4 * p;
+      if (r[2] < 2 || r[3] < 2 || r[0] < 0 || r[1] < 0 || r[0] + r[2] > width || r[1] + r[3] > height)
+        throw ArgError("patch rectangle outside the image");
+      if (c[2] < 0 || c[3] < 0 || c[0] < r[0] || c[1] < r[1] || c[0] + c[2] > r[0] + r[2] || c[1] + c[3] > r[1] + r[3])
+        throw ArgError("core rectangle outside its patch");
+    }
+    BatchOptions bo;
+    bo.use_graph = mode == 1;
+    run_patches(ctx->c, width, height, d_packed, d_codebook, npatch, rects, cores, mode == 0 ? 0 : 1, bo, d_gpb_max,
+                d_gpb_orient);
+  });
+}
+
+int gpb_get_patch_stats(gpb_ctx* ctx, int patch, gpb_stats* out) {
+  return guarded2(ctx, [&] {
+    Context& c = ctx->c;
+    if (!out || patch < 0 || patch >= int(c.patch_stats.size())) throw ArgError("no such patch");
+    std::memset(out, 0, sizeof(*out));
+    for (int s = 0; s < 6; s++) out->stage_ms[s] = c.stage_ms[s];
+    const LanczosStats& st = c.patch_stats[size_t(patch)];
+    out->lanczos_matvecs = st.iterations;
+    out->lanczos_restarts = st.restarts;
+    out->lanczos_basis = st.basis;
+    out->lanczos_block = st.block;
+    out->converged = st.converged;
+    out->max_residual = st.max_residual;
+    out->fallback = st.fallback;
+    out->reorth_second_passes = st.second_passes;
+    const int K = c.eig_K;
+    for (int i = 0; i < K && size_t(patch) * K + i < c.patch_evals.size(); i++)
+      out->evals[i] = c.patch_evals[size_t(patch) * K + i];
+  });
+}
+
+int gpb_compute_patches(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, int npatch,
+                        const int* rects, const int* cores, int mode, float* gpb_max, float* gpb_orient) {
+  if (!ctx) return GPB_ERR_ARG;
+  uint32_t* d_rgb = nullptr;
+  float *d_cb = nullptr, *d_gmax = nullptr, *d_orient = nullptr;
+  const int rc0 = guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!packed || !codebook) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    d_rgb = c.ws<uint32_t>("io.rgb", N);
+    d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    d_gmax = gpb_max ? c.ws<float>("io.gmax", N) : nullptr;
+    d_orient = gpb_orient ? c.ws<float>("io.orient", 8 * N) : nullptr;
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, sizeof(float) * kSchmidNF * kTextonK, hipMemcpyHostToDevice, c.stream));
+    if (d_gmax) launch_fill(c, d_gmax, N, 0.f);
+    if (d_orient) launch_fill(c, d_orient, 8 * N, 0.f);
+  });
+  if (rc0 != GPB_OK) return rc0;
+  const int rc = gpb_compute_patches_device(ctx, width, height, d_rgb, d_cb, npatch, rects, cores, mode, d_gmax,
+                                            d_orient);
+  if (rc != GPB_OK) return rc;
+  return guarded2(ctx, [&] {
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    if (gpb_max) GPB_HIP(hipMemcpyAsync(gpb_max, d_gmax, N * 4, hipMemcpyDeviceToHost, c.stream));
+    if (gpb_orient) GPB_HIP(hipMemcpyAsync(gpb_orient, d_orient, 8 * N * 4, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
   });
 }

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void degree_kernel(const float* __restrict__ w
   s[pp] = float(1.0 / sqrt(acc));
 }
 
-void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
+void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, const std::string& tag) {
   if (W < 2 || H < 2) throw ArgError("the affinity needs an image at least 2 pixels wide and high");
   AffTables& t = aff_tables(c);
   st.W = W;
@@ -170,10 +170,10 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st) {
   st.Wp = W + 2 * kICRadius;
   st.Hp = H + 2 * kICRadius;
   st.Np = size_t(st.Wp) * st.Hp;
-  st.wup = c.ws<float>("st.wup", kStencilHalf * st.Np);
-  st.D = c.ws<float>("st.D", st.Np);
-  st.s = c.ws<float>("st.s", st.Np);
-  st.offs = c.ws<int>("st.offs", kStencilHalf);
+  st.wup = c.ws<float>(tag + "st.wup", kStencilHalf * st.Np);
+  st.D = c.ws<float>(tag + "st.D", st.Np);
+  st.s = c.ws<float>(tag + "st.s", st.Np);
+  st.offs = c.ws<int>(tag + "st.offs", kStencilHalf);
   int offs[kStencilHalf];
   for (int j = 0; j < kStencilHalf; j++) offs[j] = t.updy[j] * st.Wp + t.updx[j];
   GPB_HIP(hipMemcpyAsync(st.offs, offs, sizeof(offs), hipMemcpyHostToDevice, c.stream));

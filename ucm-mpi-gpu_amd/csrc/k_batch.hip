@@ -1,0 +1,856 @@
+// Batched thick-restart Lanczos for many patches of uneven size at once.
+//
+// The reference decomposes a large image into patches and solves each one's
+// eigenproblem independently, one MPI rank per patch (src/main.cpp:161-301,
+// src/utilities.cpp:462-556 Img2Pieces; the solver is spectralPb.cpp:186-244).
+// Here every patch's Lanczos runs in lockstep inside the same launches:
+//   - a batched vector is the concatenation of the patches' padded frames;
+//   - the SpMV walks one combined tile list (XCD-aware), each tile reading its
+//     patch's own stencil;
+//   - the vector kernels are persistent: G workgroups each own one contiguous
+//     range of the batched vector, so uneven patches cost nothing extra in
+//     balance, and write one partial sum per (patch, workgroup) they touch;
+//   - every per-patch scalar (T, alpha, beta, projections, DGKS flags, the
+//     active mask) lives on the device, and converged patches drop out through
+//     the mask without changing the launch sequence;
+//   - each restart cycle after the first is captured once into a hipGraph and
+//     replayed, so a cycle of m - keep steps for all patches is one launch.
+// The host sees T only at restarts (Rayleigh-Ritz for all active patches).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eig_common.hpp"
+#include "k_batch.hpp"
+
+namespace gpb {
+
+namespace {
+
+constexpr int kHS = kMaxBasis + 2;  // stride of per-patch projection vectors
+constexpr int kPS = kProjectMaxN + 1;  // rows per patch of the projection partials
+
+struct BSpmvArgs {
+  int dx[kStencilHalf], dy[kStencilHalf], lo[kStencilHalf];
+};
+
+__device__ __forceinline__ int patch_of_tile(const BPatchDev* __restrict__ P, int np, int tile) {
+  int p = 0;
+  for (int i = 1; i < np; i++)
+    if (P[i].tile0 <= tile) p = i;
+  return p;
+}
+
+template <int ROWS, typename AT>
+__global__ __launch_bounds__(kSpmvTX * 4) void bspmv_kernel(const BPatchDev* __restrict__ P, int np, int ntiles,
+                                                            const float* __restrict__ x, float* __restrict__ y,
+                                                            double* __restrict__ xy_part,
+                                                            const int* __restrict__ active, BSpmvArgs B) {
+  constexpr int kTY = 4 * ROWS, kTileH = kTY + 2 * kICRadius;
+  __shared__ AT u[kTileH * kSpmvTileW];
+  __shared__ double red[4];
+  const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
+  const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
+  if (tile >= ntiles) return;
+  const int p = patch_of_tile(P, np, tile);
+  const BPatchDev q = P[p];
+  if (!active[p]) {
+    if (threadIdx.x == 0 && threadIdx.y == 0 && xy_part) xy_part[tile] = 0.0;
+    return;
+  }
+  SpmvArgs A;
+#pragma unroll
+  for (int j = 0; j < kStencilHalf; j++) {
+    A.o[j] = B.dy[j] * q.Wp + B.dx[j];
+    A.lo[j] = B.lo[j];
+  }
+  const int t = tile - q.tile0;
+  const int x0 = (t % q.tiles_x) * kSpmvTX, y0 = (t / q.tiles_x) * kTY;
+  const double xy = spmv_tile<ROWS, AT>(q.wup, q.s, x + q.voff, y + q.voff, q.W, q.H, q.Wp, q.Hp, q.Np, A, x0, y0, u);
+  if (!xy_part) return;
+  const double s = block_sum256(xy, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) xy_part[tile] = s;
+}
+
+// The range of batched-vector elements owned by workgroup blockIdx.x.
+struct Range {
+  size_t e0, e1;
+};
+__device__ __forceinline__ Range my_range(size_t Ntot, size_t chunk) {
+  const size_t e0 = size_t(blockIdx.x) * chunk;
+  return Range{e0, e0 + chunk < Ntot ? e0 + chunk : Ntot};
+}
+
+// [alpha: w <- w - alpha_p v_j - beta_p,j-1 v_{j-1}] then V_i . w (i < n) and
+// ||w||^2 per patch, as project_kernel does for one image.
+template <int NV>
+__global__ __launch_bounds__(256) void bproject_kernel(const BPatchDev* __restrict__ P, int np,
+                                                       const float* __restrict__ V, size_t ld, int n,
+                                                       float* __restrict__ w, const double* __restrict__ alpha,
+                                                       const double* __restrict__ beta, int j,
+                                                       const int* __restrict__ gate, const int* __restrict__ active,
+                                                       size_t Ntot, size_t chunk, double* __restrict__ partial) {
+  __shared__ double sh[4][NV + 1];
+  const Range r = my_range(Ntot, chunk);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p] || (gate && !gate[p])) continue;  // uniform
+    const double a = alpha ? alpha[p] : 0.0;
+    const double b = (alpha && j > 0) ? beta[size_t(p) * 2 * kMaxBasis + j - 1] : 0.0;
+    const float* vj = V + size_t(j) * ld;
+    const float* vjm = V + size_t(j > 0 ? j - 1 : 0) * ld;
+    double acc[NV + 1];
+#pragma unroll
+    for (int i = 0; i <= NV; i++) acc[i] = 0.0;
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+      double wn = w[e];
+      if (alpha) {
+        wn -= a * double(vj[e]) + b * double(vjm[e]);
+        const float f = float(wn);
+        w[e] = f;
+        wn = f;
+      }
+      acc[NV] += wn * wn;
+      const float* q = V + e;
+      float vv[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) vv[i] = q[size_t(i < n ? i : 0) * ld];
+#pragma unroll
+      for (int i = 0; i < NV; i++) acc[i] += double(vv[i]) * wn;
+    }
+#pragma unroll
+    for (int i = 0; i <= NV; i++) {
+      if (i < n || i == NV) {
+        double v = acc[i];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) sh[wv][i] = v;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t <= n; t += blockDim.x) {
+      const int i = t < n ? t : NV;
+      partial[(size_t(p) * kPS + t) * gridDim.x + blockIdx.x] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+    }
+    __syncthreads();
+  }
+}
+
+// w <- w - V h_p per patch; per-patch partial of ||w||^2 when nrm_part.  As
+// update_gs_kernel: NV >= n coefficients in registers, all NV loads of an
+// element issued before use.
+template <int NV>
+__global__ __launch_bounds__(256) void bupdate_kernel(const BPatchDev* __restrict__ P, int np,
+                                                      const float* __restrict__ V, size_t ld, int n,
+                                                      const double* __restrict__ h, float* __restrict__ w,
+                                                      double* __restrict__ nrm_part, const int* __restrict__ gate,
+                                                      const int* __restrict__ active, size_t Ntot, size_t chunk) {
+  __shared__ double red[4];
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p] || (gate && !gate[p])) continue;  // uniform
+    double hv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) hv[i] = i < n ? h[size_t(p) * kHS + i] : 0.0;
+    double nrm = 0;
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+      const float* q = V + e;
+      float vv[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) vv[i] = q[size_t(i < n ? i : 0) * ld];
+      double acc = w[e];
+#pragma unroll
+      for (int i = 0; i < NV; i++) acc -= hv[i] * double(vv[i]);
+      const float f = float(acc);
+      w[e] = f;
+      nrm += double(f) * double(f);
+    }
+    if (nrm_part) {
+      for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+      __syncthreads();
+      if (threadIdx.x == 0) nrm_part[size_t(p) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+      __syncthreads();
+    }
+  }
+}
+
+// out[p * ostride + i] = sum of partial rows over the workgroups that touched
+// patch p (brange[2p] .. brange[2p+1]); flat: rows are shared by all patches
+// (partial[i * G + b]), else per patch (partial[(p * pstride + i) * G + b]).
+__global__ __launch_bounds__(kRedThreads) void breduce_kernel(const double* __restrict__ partial, int G,
+                                                              const int* __restrict__ brange, int pstride, int flat,
+                                                              double* __restrict__ out, int ostride,
+                                                              const int* __restrict__ gate,
+                                                              const int* __restrict__ active) {
+  const int p = blockIdx.y, i = blockIdx.x;
+  if (!active[p] || (gate && !gate[p])) return;
+  __shared__ double red[kRedThreads / 64];
+  const double* row = flat ? partial + size_t(i) * G : partial + (size_t(p) * pstride + i) * G;
+  double a = 0;
+  for (int b = brange[2 * p] + int(threadIdx.x); b <= brange[2 * p + 1]; b += kRedThreads) a += row[b];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int k = 0; k < kRedThreads / 64; k++) s += red[k];
+    out[size_t(p) * ostride + i] = s;
+  }
+}
+
+__global__ void bdecide_kernel(const double* __restrict__ h1, int n, const double* __restrict__ nrm,
+                               int* __restrict__ flag, const int* __restrict__ active, int np,
+                               double* __restrict__ nsecond) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int f = active[p] && (nrm[2 * p] < 0.5 * h1[size_t(p) * kHS + n]);
+  flag[p] = f;
+  if (f) atomicAdd(nsecond, 1.0);
+}
+
+// Column j of each active patch's projected matrix (see tcol_kernel).
+__global__ void btcol_kernel(double* __restrict__ T, int m, int j, int couple_next, const double* __restrict__ alpha,
+                             const double* __restrict__ h1, const double* __restrict__ h2,
+                             const int* __restrict__ flag, const double* __restrict__ nrm1,
+                             const double* __restrict__ nrm2, double* __restrict__ beta,
+                             const int* __restrict__ active) {
+  const int p = blockIdx.x, i = threadIdx.x;
+  if (!active[p]) return;
+  double* Tp = T + size_t(p) * m * m;
+  const double* a1 = h1 + size_t(p) * kHS;
+  const double* a2 = h2 + size_t(p) * kHS;
+  const bool two = flag && flag[p];
+  if (i <= j) {
+    const double v = Tp[size_t(i) * m + j] + a1[i] + (two ? a2[i] : 0.0) + ((i == j && alpha) ? alpha[p] : 0.0);
+    Tp[size_t(i) * m + j] = v;
+    if (i != j) Tp[size_t(j) * m + i] = v;
+  }
+  if (i == 0) {
+    const double b = sqrt(two ? nrm2[2 * p] : nrm1[2 * p]);
+    double* bp = beta + size_t(p) * 2 * kMaxBasis;
+    bp[j] = b;
+    bp[kMaxBasis + j] = b > 1e-300 ? 1.0 / b : 0.0;
+    if (couple_next && j + 1 < m) {
+      Tp[size_t(j) * m + j + 1] = b;
+      Tp[size_t(j + 1) * m + j] = b;
+    }
+  }
+}
+
+// out = w / beta_p,j per patch (or * 1/sqrt(nrm2[p * nstride]) when nrm2).
+__global__ __launch_bounds__(256) void bscale_kernel(const BPatchDev* __restrict__ P, int np,
+                                                     const float* __restrict__ w, const double* __restrict__ beta,
+                                                     int j, const double* __restrict__ nrm2, int nstride,
+                                                     float* __restrict__ out, const int* __restrict__ active,
+                                                     size_t Ntot, size_t chunk) {
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p]) continue;
+    const double ib = nrm2 ? 1.0 / sqrt(nrm2[size_t(p) * nstride]) : beta[size_t(p) * 2 * kMaxBasis + kMaxBasis + j];
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) out[e] = float(double(w[e]) * ib);
+  }
+}
+
+// out_i = sum_j Y_p[j][i] V_j per patch, i < nk (24 at a time), as ritz_kernel.
+__global__ __launch_bounds__(256) void britz_kernel(const BPatchDev* __restrict__ P, int np,
+                                                    const float* __restrict__ V, size_t ld, int m, int nk,
+                                                    const double* __restrict__ Y /* [np][m][nk] */,
+                                                    float* __restrict__ out, const int* __restrict__ active,
+                                                    size_t Ntot, size_t chunk) {
+  constexpr int kC = 24;
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p]) continue;
+    const double* Yp = Y + size_t(p) * m * nk;
+    for (int i0 = 0; i0 < nk; i0 += kC) {
+      const int cnt = nk - i0 < kC ? nk - i0 : kC;
+      for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        double acc[kC];
+#pragma unroll
+        for (int i = 0; i < kC; i++) acc[i] = 0;
+        for (int j0 = 0; j0 < m; j0 += 8) {
+          float v[8];
+#pragma unroll
+          for (int t = 0; t < 8; t++) v[t] = V[size_t(j0 + t < m ? j0 + t : 0) * ld + e];
+#pragma unroll
+          for (int t = 0; t < 8; t++) {
+            if (j0 + t >= m) break;
+            const double* y = Yp + size_t(j0 + t) * nk + i0;
+            const double vt = double(v[t]);
+#pragma unroll
+            for (int i = 0; i < kC; i++)
+              if (i < cnt) acc[i] += y[i] * vt;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < kC; i++)
+          if (i < cnt) out[size_t(i0 + i) * ld + e] = float(acc[i]);
+      }
+    }
+  }
+}
+
+// Weighted squared norms sum_e D_e x_e^2 of rows rows[0..nr) of V, per patch.
+__global__ __launch_bounds__(256) void bwnorm_kernel(const BPatchDev* __restrict__ P, int np,
+                                                     const float* __restrict__ V, size_t ld, int nr,
+                                                     const int* __restrict__ rows, const int* __restrict__ active,
+                                                     size_t Ntot, size_t chunk, double* __restrict__ partial) {
+  __shared__ double red[4];
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p]) continue;
+    const float* D = P[p].D;
+    const size_t voff = P[p].voff;
+    for (int k = 0; k < nr; k++) {
+      const float* v = V + size_t(rows[k]) * ld;
+      double a = 0;
+      for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        const double x = v[e];
+        a += double(D[e - voff]) * x * x;
+      }
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+      __syncthreads();
+      if (threadIdx.x == 0) partial[(size_t(p) * kPS + k) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+      __syncthreads();
+    }
+  }
+}
+
+// Start vectors: row 0 = D^1/2 (padded frame), row 1 = seeded uniform noise on
+// the interior, per patch.
+__global__ __launch_bounds__(256) void bstart_kernel(const BPatchDev* __restrict__ P, int np, float* __restrict__ v0,
+                                                     float* __restrict__ v1, size_t Ntot, size_t chunk, uint64_t seed) {
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi) continue;
+    const BPatchDev q = P[p];
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+      const size_t i = e - q.voff;
+      v0[e] = sqrtf(q.D[i]);
+      const int y = int(i / q.Wp) - kICRadius, x = int(i % q.Wp) - kICRadius;
+      float rv = 0.f;
+      if (x >= 0 && x < q.W && y >= 0 && y < q.H) {
+        uint64_t z = seed + uint64_t(p) * 0xD1B54A32D192ED03ull + i * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        rv = float(double(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+      }
+      v1[e] = rv;
+    }
+  }
+}
+
+__global__ void bset_t00_kernel(double* __restrict__ T, int m, const double* __restrict__ alpha, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) T[size_t(p) * m * m] = alpha[p];
+}
+
+// Generalised eigenvectors v = s .* u of rows 0..K-1 into patch p's output.
+__global__ __launch_bounds__(256) void bto_general_kernel(const float* __restrict__ V, size_t ld, int K,
+                                                          const float* __restrict__ s, size_t voff, size_t Np,
+                                                          float* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < Np; i += size_t(gridDim.x) * blockDim.x)
+    for (int k = 0; k < K; k++) out[size_t(k) * Np + i] = V[size_t(k) * ld + voff + i] * s[i];
+}
+
+// ---- host ---------------------------------------------------------------------
+
+struct BatchState {
+  Context& c;
+  int np = 0, G = 0, ntiles = 0, rows = 2;
+  size_t Ntot = 0, chunk = 0;
+  BPatchDev* P = nullptr;
+  int* brange = nullptr;   // [np][2] workgroup range per patch (vector kernels)
+  int* trange = nullptr;   // [np][2] tile range per patch (SpMV)
+  int* active = nullptr;
+  double* partial = nullptr;
+  double* npart = nullptr;
+  double* xy = nullptr;
+  BSpmvArgs sa{};
+  double bytes_spmv = 0;   // algorithmic bytes of one batched SpMV
+  explicit BatchState(Context& cc) : c(cc) {}
+
+  void spmv(const float* x, float* y, double* xy_out) {
+    KScope ks(c, kKSpmv, bytes_spmv);
+    const dim3 g(unsigned(div_up(ntiles, kNumXcd) * kNumXcd)), b(kSpmvTX, 4);
+#define GPB_BSPMV(R, AT)                                                                                      \
+  hipLaunchKernelGGL((bspmv_kernel<R, AT>), g, b, 0, c.stream, P, np, ntiles, x, y, xy_out ? xy : nullptr, active, sa);
+    const bool f64 = c.spmv_fp64;
+    switch (rows) {
+      case 2:
+        if (f64) GPB_BSPMV(2, double) else GPB_BSPMV(2, float)
+        break;
+      case 4:
+        if (f64) GPB_BSPMV(4, double) else GPB_BSPMV(4, float)
+        break;
+      case 8:
+        if (f64) GPB_BSPMV(8, double) else GPB_BSPMV(8, float)
+        break;
+      default:
+        throw ArgError("unsupported SpMV tile");
+    }
+#undef GPB_BSPMV
+    GPB_LAUNCH_CHECK();
+    if (xy_out) {
+      hipLaunchKernelGGL(breduce_kernel, dim3(1, np), dim3(kRedThreads), 0, c.stream, xy, ntiles, trange, 1, 1, xy_out,
+                         1, nullptr, active);
+      GPB_LAUNCH_CHECK();
+    }
+  }
+  void project(const float* V, size_t ld, int n, float* w, const double* alpha, const double* beta, int j,
+               const int* gate, double* dst /* [np][kHS] */) {
+    if (n < 1 || n > kProjectMaxN) throw ArgError("projection width out of range");
+    {
+      KScope ks(c, kKDots, gate ? 0.0 : double(Ntot) * (4.0 * n + 4 + (alpha ? 12 : 0)));
+      const dim3 g(G), b(256);
+#define GPB_BPROJ(NV)                                                                                                \
+  case NV / 4:                                                                                                       \
+    hipLaunchKernelGGL(bproject_kernel<NV>, g, b, 0, c.stream, P, np, V, ld, n, w, alpha, beta, j, gate, active, Ntot, \
+                       chunk, partial);                                                                              \
+    break;
+      switch ((n + 3) / 4) {
+        GPB_BPROJ(4)
+        GPB_BPROJ(8)
+        GPB_BPROJ(12)
+        GPB_BPROJ(16)
+        GPB_BPROJ(20)
+        GPB_BPROJ(24)
+        GPB_BPROJ(28)
+        GPB_BPROJ(32)
+        GPB_BPROJ(36)
+        GPB_BPROJ(40)
+        GPB_BPROJ(44)
+        GPB_BPROJ(48)
+        GPB_BPROJ(52)
+        GPB_BPROJ(56)
+        GPB_BPROJ(60)
+        GPB_BPROJ(64)
+      }
+#undef GPB_BPROJ
+      GPB_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(breduce_kernel, dim3(n + 1, np), dim3(kRedThreads), 0, c.stream, partial, G, brange, kPS, 0, dst,
+                       kHS, gate, active);
+    GPB_LAUNCH_CHECK();
+  }
+  void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst /* [np][2] */,
+              const int* gate = nullptr) {
+    {
+      KScope ks(c, kKUpdate, gate ? 0.0 : double(Ntot) * (4.0 * n + 8));
+      if (n < 1 || n > kProjectMaxN) throw ArgError("update width out of range");
+      const dim3 g(G), b(256);
+      double* np_ = nrm_dst ? npart : nullptr;
+#define GPB_BUPD(NV)                                                                                                \
+  case NV / 4:                                                                                                      \
+    hipLaunchKernelGGL(bupdate_kernel<NV>, g, b, 0, c.stream, P, np, V, ld, n, h, w, np_, gate, active, Ntot, chunk); \
+    break;
+      switch ((n + 3) / 4) {
+        GPB_BUPD(4)
+        GPB_BUPD(8)
+        GPB_BUPD(12)
+        GPB_BUPD(16)
+        GPB_BUPD(20)
+        GPB_BUPD(24)
+        GPB_BUPD(28)
+        GPB_BUPD(32)
+        GPB_BUPD(36)
+        GPB_BUPD(40)
+        GPB_BUPD(44)
+        GPB_BUPD(48)
+        GPB_BUPD(52)
+        GPB_BUPD(56)
+        GPB_BUPD(60)
+        GPB_BUPD(64)
+      }
+#undef GPB_BUPD
+      GPB_LAUNCH_CHECK();
+    }
+    if (nrm_dst) {
+      hipLaunchKernelGGL(breduce_kernel, dim3(1, np), dim3(kRedThreads), 0, c.stream, npart, G, brange, 1, 0, nrm_dst,
+                         2, gate, active);
+      GPB_LAUNCH_CHECK();
+    }
+  }
+  // element-wise kernels (no per-patch reduction) use their own wide grid
+  int GW = 2048;
+  size_t chunk_w = 0;
+  void scale(const float* w, const double* beta, int j, const double* nrm2, int nstride, float* out) {
+    hipLaunchKernelGGL(bscale_kernel, dim3(GW), dim3(256), 0, c.stream, P, np, w, beta, j, nrm2, nstride, out, active,
+                       Ntot, chunk_w);
+    GPB_LAUNCH_CHECK();
+  }
+};
+
+}  // namespace
+
+void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const EigOptions& opt_in,
+                           const BatchOptions& bo, std::vector<double>& evals /* [np][K] */,
+                           const std::vector<float*>& d_evecs /* per patch [K][Np_p] */,
+                           std::vector<LanczosStats>& stats) {
+  const int np = int(sts.size());
+  if (np < 1 || np > kMaxPatches) throw ArgError("patch count out of range");
+  EigOptions opt = opt_in;
+  size_t Nmin = SIZE_MAX;
+  for (const Stencil& st : sts) Nmin = std::min(Nmin, size_t(st.W) * st.H);
+  if (opt.K < 2) throw ArgError("need at least 2 eigenpairs");
+  if (Nmin < size_t(opt.K) + 4) throw ArgError("patch too small for the requested eigenpairs");
+  opt.m = std::min<int>(std::min<int>(opt.m, kProjectMaxN - 1), int(Nmin) - 1);
+  if (opt.m < opt.K + 2) throw ArgError("basis too small");
+  opt.keep = std::max(opt.K + 1, std::min(opt.keep, opt.m - 2));
+  const int m = opt.m, K = opt.K, keep = opt.keep;
+
+  BatchState bs(c);
+  bs.np = np;
+  // layout: concatenated padded frames, SpMV tiles, workgroup ranges
+  std::vector<BPatchDev> ph(static_cast<size_t>(np));
+  size_t off = 0;
+  for (int p = 0; p < np; p++) {
+    const Stencil& st = sts[size_t(p)];
+    BPatchDev& d = ph[size_t(p)];
+    d = BPatchDev{};
+    d.wup = st.wup;
+    d.s = st.s;
+    d.D = st.D;
+    d.W = st.W;
+    d.H = st.H;
+    d.Wp = st.Wp;
+    d.Hp = st.Hp;
+    d.Np = st.Np;
+    d.voff = off;
+    d.tiles_x = int(div_up(st.W, kSpmvTX));
+    off += st.Np;
+    bs.bytes_spmv += double(st.W) * st.H * (4 * kStencilHalf + 12);
+  }
+  bs.Ntot = off;
+  constexpr int kMaxG = 2048;
+  bs.P = c.ws<BPatchDev>("bat.P", size_t(np));
+  bs.brange = c.ws<int>("bat.brange", size_t(2 * np));
+  bs.trange = c.ws<int>("bat.trange", size_t(2 * np));
+  bs.active = c.ws<int>("bat.active", size_t(np));
+  bs.partial = c.ws<double>("bat.partial", size_t(np) * kPS * kMaxG);
+  bs.npart = c.ws<double>("bat.npart", size_t(np) * kMaxG);
+  std::vector<int> act(size_t(np), 1);
+  GPB_HIP(hipMemcpyAsync(bs.active, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
+  {
+    int dx[kStencilHalf], dy[kStencilHalf];
+    stencil_upper_offsets(dx, dy);
+    for (int j = 0; j < kStencilHalf; j++) {
+      bs.sa.dx[j] = dx[j];
+      bs.sa.dy[j] = dy[j];
+      bs.sa.lo[j] = dy[j] * kSpmvTileW + dx[j];
+    }
+  }
+  // SpMV tile rows and vector-kernel workgroups (re-laid out per candidate)
+  auto set_tiles = [&](int rows) {
+    bs.rows = rows;
+    int tiles = 0;
+    std::vector<int> tr(size_t(2 * np));
+    for (int p = 0; p < np; p++) {
+      BPatchDev& d = ph[size_t(p)];
+      d.tile0 = tiles;
+      d.ntiles = d.tiles_x * int(div_up(d.H, 4 * rows));
+      tiles += d.ntiles;
+      tr[size_t(2 * p)] = d.tile0;
+      tr[size_t(2 * p + 1)] = d.tile0 + d.ntiles - 1;
+    }
+    bs.ntiles = tiles;
+    bs.xy = c.ws<double>("bat.xy", size_t(tiles));
+    GPB_HIP(hipMemcpyAsync(bs.P, ph.data(), sizeof(BPatchDev) * np, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(bs.trange, tr.data(), sizeof(int) * 2 * np, hipMemcpyHostToDevice, c.stream));
+  };
+  auto set_groups = [&](int G) {
+    bs.G = G;
+    bs.chunk = ((bs.Ntot + bs.G - 1) / bs.G + 255) / 256 * 256;
+    std::vector<int> br(size_t(2 * np));
+    for (int p = 0; p < np; p++) {
+      br[size_t(2 * p)] = int(ph[size_t(p)].voff / bs.chunk);
+      br[size_t(2 * p + 1)] = int((ph[size_t(p)].voff + ph[size_t(p)].Np - 1) / bs.chunk);
+    }
+    GPB_HIP(hipMemcpyAsync(bs.brange, br.data(), sizeof(int) * 2 * np, hipMemcpyHostToDevice, c.stream));
+  };
+  const size_t Nt = bs.Ntot;
+  bs.chunk_w = ((Nt + bs.GW - 1) / bs.GW + 255) / 256 * 256;
+  float* V = c.ws<float>("bat.V", size_t(m + 1) * Nt);
+  float* V2 = c.ws<float>("bat.V2", size_t(m + 1) * Nt);
+  float* w = c.ws<float>("bat.w", Nt);
+  double* h1 = c.ws<double>("bat.h1", size_t(np) * kHS);
+  {
+    // key: the batch's shapes (FNV-1a), flagged apart from single-image keys
+    uint64_t key = 1469598103934665603ull;
+    for (const Stencil& st : sts) key = (key ^ ((uint64_t(st.W) << 32) | uint32_t(st.H))) * 1099511628211ull;
+    key |= 1ull << 63;
+    key ^= c.spmv_fp64 ? (1ull << 62) : 0;
+    int best_rows = bo.spmv_rows, best_g = bo.workgroups;
+    auto it = c.spmv_rows.find(key);
+    if (it != c.spmv_rows.end()) {
+      if (!best_rows) best_rows = it->second & 0xff;
+      if (!best_g) best_g = it->second >> 8;
+    }
+    if (!best_rows || !best_g) {
+      // time each candidate (3 applications after one warm-up) on this batch
+      hipEvent_t e0, e1;
+      GPB_HIP(hipEventCreate(&e0));
+      GPB_HIP(hipEventCreate(&e1));
+      auto timed = [&](auto&& fn) {
+        fn();
+        GPB_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 3; r++) fn();
+        GPB_HIP(hipEventRecord(e1, c.stream));
+        GPB_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+      };
+      const bool kt = c.kt.on;
+      c.kt.on = false;
+      const int tn = std::min(24, m);  // basis width the projection is timed at
+      launch_fill(c, V, size_t(tn) * Nt, 1.0f);
+      launch_fill(c, w, Nt, 1.0f);
+      set_groups(1024);
+      if (!best_rows) {
+        float bt = 1e30f;
+        for (int rows : {2, 4, 8}) {
+          set_tiles(rows);
+          const float t = timed([&] { bs.spmv(V, w, nullptr); });
+          if (t < bt) {
+            bt = t;
+            best_rows = rows;
+          }
+        }
+      }
+      set_tiles(best_rows);
+      if (!best_g) {
+        float bt = 1e30f;
+        for (int G : {512, 1024, 2048}) {
+          set_groups(G);
+          const float t = timed([&] { bs.project(V, Nt, tn, w, nullptr, nullptr, 0, nullptr, h1); });
+          if (t < bt) {
+            bt = t;
+            best_g = G;
+          }
+        }
+      }
+      c.kt.on = kt;
+      GPB_HIP(hipEventDestroy(e0));
+      GPB_HIP(hipEventDestroy(e1));
+      if (!bo.spmv_rows && !bo.workgroups) c.spmv_rows[key] = best_rows | (best_g << 8);
+    }
+    set_tiles(best_rows);
+    set_groups(best_g);
+  }
+  double* T = c.ws<double>("bat.T", size_t(np) * m * m);
+  double* beta = c.ws<double>("bat.beta", size_t(np) * 2 * kMaxBasis);
+  double* alpha = c.ws<double>("bat.alpha", size_t(np));
+  double* h2 = c.ws<double>("bat.h2", size_t(np) * kHS);
+  double* nrm = c.ws<double>("bat.nrm", size_t(np) * 2);
+  int* flag = c.ws<int>("bat.flag", size_t(np));
+  int* ones = c.ws<int>("bat.ones", size_t(np));
+  double* nsecond = c.ws<double>("bat.nsecond", 1);
+  double* Ydev = c.ws<double>("bat.Y", size_t(np) * m * keep);
+  int* rows_dev = c.ws<int>("bat.rows", size_t(K + 1));
+  GPB_HIP(hipMemcpyAsync(ones, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
+  GPB_HIP(hipMemsetAsync(T, 0, sizeof(double) * np * m * m, c.stream));
+  GPB_HIP(hipMemsetAsync(nsecond, 0, sizeof(double), c.stream));
+  launch_fill(c, w, Nt, 0.f);
+  // start: V0 = D^1/2 / ||.||, T00 = V0' L V0; V1 random, orthogonal to V0, normalised
+  hipLaunchKernelGGL(bstart_kernel, dim3(bs.G), dim3(256), 0, c.stream, bs.P, np, V, V + Nt, Nt, bs.chunk,
+                     uint64_t(0x5eed1234abcdull));
+  GPB_LAUNCH_CHECK();
+  bs.project(V, Nt, 1, V, nullptr, nullptr, 0, nullptr, h1);  // h1[p][1] = ||V0||^2
+  bs.scale(V, nullptr, 0, h1 + 1, kHS, V);
+  bs.spmv(V, w, alpha);
+  hipLaunchKernelGGL(bset_t00_kernel, dim3(div_up(np, 64)), dim3(64), 0, c.stream, T, m, alpha, np);
+  GPB_LAUNCH_CHECK();
+  for (int pass = 0; pass < 2; pass++) {
+    bs.project(V, Nt, 1, V + Nt, nullptr, nullptr, 0, nullptr, h1);
+    bs.update(V, Nt, 1, h1, V + Nt, nullptr);
+  }
+  bs.project(V, Nt, 1, V + Nt, nullptr, nullptr, 0, nullptr, h1);  // h1[p][1] = ||V1||^2
+  bs.scale(V + Nt, nullptr, 0, h1 + 1, kHS, V + Nt);
+
+  stats.assign(size_t(np), LanczosStats());
+  std::vector<int> mv(size_t(np), 1);
+  for (auto& s : stats) {
+    s.basis = m;
+    s.block = 1;
+  }
+  evals.assign(size_t(np) * K, 0.0);
+  std::vector<double> Th(size_t(np) * m * m), betah(size_t(np) * 2 * kMaxBasis), nrmh(size_t(np) * kPS);
+  std::vector<double> Yall(size_t(np) * m * keep);
+  std::vector<std::vector<double>> th(static_cast<size_t>(np)), Y(static_cast<size_t>(np));
+  std::vector<int> rows(size_t(K + 1));
+  for (int i = 0; i < K; i++) rows[size_t(i)] = i;
+  rows[size_t(K)] = keep;
+  GPB_HIP(hipMemcpyAsync(rows_dev, rows.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, c.stream));
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  int parity = 0;  // which of V / V2 is current (graphs are per buffer)
+  int k = 1, nactive = np, cycles = 0;
+  const bool use_graph = bo.use_graph && !c.kt.on;
+
+  auto cycle_steps = [&](int k0, float* Vc) {
+    for (int j = k0; j < m; j++) {
+      float* vj = Vc + size_t(j) * Nt;
+      const int n = j + 1;
+      if (j == k0) {
+        bs.spmv(vj, w, nullptr);
+        bs.project(Vc, Nt, n, w, nullptr, nullptr, j, nullptr, h1);
+        bs.update(Vc, Nt, n, h1, w, nullptr);
+        bs.project(Vc, Nt, n, w, nullptr, nullptr, j, nullptr, h2);
+        bs.update(Vc, Nt, n, h2, w, nrm + 1);
+        hipLaunchKernelGGL(btcol_kernel, dim3(np), dim3(kMaxBasis), 0, c.stream, T, m, j, 1, nullptr, h1, h2, ones,
+                           nrm + 1, nrm + 1, beta, bs.active);
+        GPB_LAUNCH_CHECK();
+      } else {
+        bs.spmv(vj, w, alpha);
+        bs.project(Vc, Nt, n, w, alpha, beta, j, nullptr, h1);
+        bs.update(Vc, Nt, n, h1, w, nrm);
+        hipLaunchKernelGGL(bdecide_kernel, dim3(div_up(np, 64)), dim3(64), 0, c.stream, h1, n, nrm, flag, bs.active, np,
+                           nsecond);
+        GPB_LAUNCH_CHECK();
+        bs.project(Vc, Nt, n, w, nullptr, nullptr, j, flag, h2);
+        bs.update(Vc, Nt, n, h2, w, nrm + 1, flag);
+        hipLaunchKernelGGL(btcol_kernel, dim3(np), dim3(kMaxBasis), 0, c.stream, T, m, j, 1, alpha, h1, h2, flag, nrm,
+                           nrm + 1, beta, bs.active);
+        GPB_LAUNCH_CHECK();
+      }
+      bs.scale(w, beta, j, nullptr, 0, Vc + size_t(j + 1) * Nt);
+    }
+  };
+
+  while (true) {
+    if (use_graph && k == keep) {
+      hipGraphExec_t& ge = graphs[parity];
+      if (!ge) {
+        hipGraph_t g;
+        GPB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        c.capturing = true;
+        try {
+          cycle_steps(k, V);
+        } catch (...) {
+          c.capturing = false;
+          hipGraph_t dummy;
+          (void)hipStreamEndCapture(c.stream, &dummy);
+          throw;
+        }
+        c.capturing = false;
+        GPB_HIP(hipStreamEndCapture(c.stream, &g));
+        GPB_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        GPB_HIP(hipGraphDestroy(g));
+      }
+      GPB_HIP(hipGraphLaunch(ge, c.stream));
+    } else {
+      cycle_steps(k, V);
+    }
+    for (int p = 0; p < np; p++)
+      if (act[size_t(p)]) mv[size_t(p)] += m - k;
+    cycles++;
+    // Rayleigh-Ritz for every active patch on the host
+    GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * np * m * m, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * np * 2 * kMaxBasis, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < np; p++) {
+      if (!act[size_t(p)]) continue;
+      std::vector<double> A(Th.begin() + long(size_t(p) * m * m), Th.begin() + long(size_t(p + 1) * m * m)), thd, Yd;
+      symeig_desc(m, A, thd, Yd);
+      th[size_t(p)].assign(size_t(m), 0.0);
+      Y[size_t(p)].assign(size_t(m) * m, 0.0);
+      for (int i = 0; i < m; i++) {
+        th[size_t(p)][size_t(i)] = thd[size_t(m - 1 - i)];
+        for (int j = 0; j < m; j++) Y[size_t(p)][size_t(j) * m + i] = Yd[size_t(j) * m + (m - 1 - i)];
+      }
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < keep; i++) Yall[(size_t(p) * m + j) * keep + i] = Y[size_t(p)][size_t(j) * m + i];
+    }
+    GPB_HIP(hipMemcpyAsync(Ydev, Yall.data(), sizeof(double) * np * m * keep, hipMemcpyHostToDevice, c.stream));
+    {
+      KScope ks(c, kKRitz, double(Nt) * 4.0 * (m + keep));
+      hipLaunchKernelGGL(britz_kernel, dim3(bs.GW), dim3(256), 0, c.stream, bs.P, np, V, Nt, m, keep, Ydev, V2,
+                         bs.active, Nt, bs.chunk_w);
+      GPB_LAUNCH_CHECK();
+    }
+    GPB_HIP(hipMemcpyAsync(V2 + size_t(keep) * Nt, V + size_t(m) * Nt, Nt * sizeof(float), hipMemcpyDeviceToDevice,
+                           c.stream));
+    // weighted norms ||D^1/2 u_i|| (i < K) and ||D^1/2 v_m|| per patch
+    hipLaunchKernelGGL(bwnorm_kernel, dim3(bs.G), dim3(256), 0, c.stream, bs.P, np, V2, Nt, K + 1, rows_dev, bs.active,
+                       Nt, bs.chunk, bs.partial);
+    GPB_LAUNCH_CHECK();
+    double* wn = c.ws<double>("bat.wn", size_t(np) * kPS);
+    hipLaunchKernelGGL(breduce_kernel, dim3(K + 1, np), dim3(kRedThreads), 0, c.stream, bs.partial, bs.G, bs.brange,
+                       kPS, 0, wn, kPS, nullptr, bs.active);
+    GPB_LAUNCH_CHECK();
+    GPB_HIP(hipMemcpyAsync(nrmh.data(), wn, sizeof(double) * np * kPS, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    c.kt.fold();
+    std::swap(V, V2);
+    parity ^= 1;
+    int newly_done = 0;
+    for (int p = 0; p < np; p++) {
+      if (!act[size_t(p)]) continue;
+      const double bm = betah[size_t(p) * 2 * kMaxBasis + m - 1];
+      double worst = 0;
+      for (int i = 0; i < K; i++) {
+        const double r = std::fabs(bm * Y[size_t(p)][size_t(m - 1) * m + i]) * std::sqrt(nrmh[size_t(p) * kPS + K]) /
+                         std::sqrt(std::max(nrmh[size_t(p) * kPS + i], 1e-300));
+        worst = std::max(worst, r);
+      }
+      stats[size_t(p)].max_residual = worst;
+      stats[size_t(p)].restarts = cycles - 1;
+      stats[size_t(p)].iterations = mv[size_t(p)];
+      if (worst <= opt.tol || mv[size_t(p)] >= opt.max_matvecs) {
+        // done: eigenpairs out, patch leaves the batch
+        for (int i = 0; i < K; i++) evals[size_t(p) * K + i] = th[size_t(p)][size_t(i)];
+        stats[size_t(p)].converged = 0;
+        for (int i = 0; i < K; i++) {
+          const double r = std::fabs(bm * Y[size_t(p)][size_t(m - 1) * m + i]) * std::sqrt(nrmh[size_t(p) * kPS + K]) /
+                           std::sqrt(std::max(nrmh[size_t(p) * kPS + i], 1e-300));
+          stats[size_t(p)].converged += r <= opt.tol;
+        }
+        const Stencil& st = sts[size_t(p)];
+        hipLaunchKernelGGL(bto_general_kernel, dim3(std::min<unsigned>(2048, div_up(st.Np, 256))), dim3(256), 0,
+                           c.stream, V, Nt, K, st.s, ph[size_t(p)].voff, st.Np, d_evecs[size_t(p)]);
+        GPB_LAUNCH_CHECK();
+        act[size_t(p)] = 0;
+        nactive--;
+        newly_done++;
+      }
+    }
+    if (nactive == 0) break;
+    // thick restart for the rest: basis = Ritz vectors [0, keep) and v_m; T = diag(theta)
+    std::vector<double> T0(size_t(np) * m * m, 0.0);
+    for (int p = 0; p < np; p++)
+      if (act[size_t(p)])
+        for (int i = 0; i < keep; i++) T0[size_t(p) * m * m + size_t(i) * m + i] = th[size_t(p)][size_t(i)];
+    GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * np * m * m, hipMemcpyHostToDevice, c.stream));
+    if (newly_done) GPB_HIP(hipMemcpyAsync(bs.active, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
+    k = keep;
+  }
+  for (int g = 0; g < 2; g++)
+    if (graphs[g]) (void)hipGraphExecDestroy(graphs[g]);
+  double ns = 0;
+  GPB_HIP(hipMemcpyAsync(&ns, nsecond, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  GPB_HIP(hipStreamSynchronize(c.stream));
+  for (auto& s : stats) s.second_passes = int(ns);
+  for (int p = 0; p < np; p++)
+    if (stats[size_t(p)].max_residual > opt.tol)
+      throw NoConvergence("batched Lanczos: patch " + std::to_string(p) + " stopped after " +
+                          std::to_string(stats[size_t(p)].iterations) + " operator applications with residual " +
+                          std::to_string(stats[size_t(p)].max_residual));
+}
+
+}  // namespace gpb

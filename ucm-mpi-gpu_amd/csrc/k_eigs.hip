@@ -22,14 +22,10 @@
 #include <cstring>
 #include <vector>
 
-#include "gpb_pipeline.hpp"
+#include "eig_common.hpp"
 
 namespace gpb {
 
-void symeig_desc(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z);
-
-constexpr int kRedThreads = 256;
-constexpr int kMaxBasis = 128;
 
 __global__ void fill_int1_kernel(int* p, int v) { *p = v; }
 
@@ -43,34 +39,13 @@ void launch_fill(Context& c, float* p, size_t n, float v) {
   GPB_LAUNCH_CHECK();
 }
 
-struct SpmvOffsets {
-  int o[kStencilHalf];
-};
-
-// Normalised Laplacian in difference form: with u = s .* x (s = D^-1/2),
-// y_p = s_p * sum_q W_pq (u_p - u_q).  Differences of neighbouring values are
-// formed before the products, so eigenvalues near 0 keep their relative
-// accuracy in FP32 (the form I - D^-1/2 W D^-1/2 would lose it to cancellation
-// against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
-// W[k][p - e_k] for q = p - e_k.  A 64x16 tile of u (plus a 5-pixel halo) is
-// staged in LDS; W streams from HBM with coalesced row reads.
-// Tiles are 64 columns by 4 * ROWS rows (64x4 threads, ROWS rows each); ROWS
-// is picked per image shape by timing (tune_spmv).
-constexpr int kSpmvTX = 64;
-constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius;
-
-struct SpmvArgs {
-  int o[kStencilHalf];   // padded linear offsets
-  int lo[kStencilHalf];  // LDS tile offsets
-};
-
-template <int ROWS>
+template <int ROWS, typename AT>
 __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                            const float* __restrict__ x, float* __restrict__ y, int W,
                                                            int H, int Wp, int Hp, size_t Np, SpmvArgs A,
                                                            double* __restrict__ xy_partial) {
   constexpr int kSpmvTY = 4 * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
-  __shared__ float u[kSpmvTileH * kSpmvTileW];
+  __shared__ AT u[kSpmvTileH * kSpmvTileW];
   __shared__ double red[4];
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
   // workgroup b runs on XCD b % 8; each XCD takes a contiguous run of tiles
@@ -80,46 +55,11 @@ __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restri
   const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
   const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
   if (tile >= ntiles) return;  // whole workgroup
-  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
   const int x0 = (tile % tiles_x) * kSpmvTX, y0 = (tile / tiles_x) * kSpmvTY;  // interior coordinates of the tile origin
-  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
-    const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
-    const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
-    float v = 0.f;
-    if (py < Hp && px < Wp) {
-      const size_t q = size_t(py) * Wp + px;
-      v = s[q] * x[q];
-    }
-    u[i] = v;
-  }
-  __syncthreads();
-  const int xi = x0 + threadIdx.x;
-  double xy = 0;
-  if (xi < W) {
-#pragma unroll
-    for (int r = 0; r < ROWS; r++) {
-      const int ty = threadIdx.y + 4 * r;
-      const int yi = y0 + ty;
-      if (yi >= H) break;
-      const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
-      const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
-      const float up = u[c];
-      float acc = 0.f;
-#pragma unroll
-      for (int j = 0; j < kStencilHalf; j++) {
-        acc = __builtin_fmaf(wup[size_t(j) * Np + p], up - u[c + A.lo[j]], acc);
-        acc = __builtin_fmaf(wup[size_t(j) * Np + p - A.o[j]], up - u[c - A.lo[j]], acc);
-      }
-      const float yp = s[p] * acc;
-      y[p] = yp;
-      xy += double(x[p]) * double(yp);
-    }
-  }
+  const double xy = spmv_tile<ROWS, AT>(wup, s, x, y, W, H, Wp, Hp, Np, A, x0, y0, u);
   if (!xy_partial) return;  // uniform
-  for (int o = 32; o > 0; o >>= 1) xy += __shfl_xor(xy, o, 64);
-  if ((tid & 63) == 0) red[tid >> 6] = xy;
-  __syncthreads();
-  if (tid == 0) xy_partial[tile] = red[0] + red[1] + red[2] + red[3];
+  const double t = block_sum256(xy, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0) xy_partial[tile] = t;
 }
 
 int spmv_blocks(const Stencil& st, int rows) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, 4 * rows)); }
@@ -136,15 +76,20 @@ void spmv_dispatch(Context& c, const Stencil& st, int rows, const float* x, floa
   }
   const int ntiles = spmv_blocks(st, rows);
   dim3 block(kSpmvTX, 4), grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
-#define GPB_SPMV(R)                                                                                                   \
-  case R:                                                                                                             \
-    hipLaunchKernelGGL(spmv_kernel<R>, grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, st.Np, \
-                       A, xy_partial);                                                                                \
-    break;
+#define GPB_SPMV(R, AT)                                                                                        \
+  hipLaunchKernelGGL((spmv_kernel<R, AT>), grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, \
+                     st.Np, A, xy_partial);
+  const bool f64 = c.spmv_fp64;
   switch (rows) {
-    GPB_SPMV(2)
-    GPB_SPMV(4)
-    GPB_SPMV(8)
+    case 2:
+      if (f64) GPB_SPMV(2, double) else GPB_SPMV(2, float)
+      break;
+    case 4:
+      if (f64) GPB_SPMV(4, double) else GPB_SPMV(4, float)
+      break;
+    case 8:
+      if (f64) GPB_SPMV(8, double) else GPB_SPMV(8, float)
+      break;
     default:
       throw ArgError("unsupported SpMV tile");
   }
@@ -158,7 +103,7 @@ void spmv_dispatch(Context& c, const Stencil& st, int rows, const float* x, floa
 // the first time a shape is seen on the context; the fastest is cached.
 int tune_spmv(Context& c, const Stencil& st) {
   if (c.spmv_rows_forced) return c.spmv_rows_forced;
-  const uint64_t key = (uint64_t(st.W) << 32) | uint32_t(st.H);
+  const uint64_t key = ((uint64_t(st.W) << 32) | uint32_t(st.H)) ^ (c.spmv_fp64 ? (1ull << 62) : 0);
   auto it = c.spmv_rows.find(key);
   if (it != c.spmv_rows.end()) return it->second;
   float* x = c.ws<float>("tune.x", st.Np);
@@ -249,10 +194,43 @@ __global__ __launch_bounds__(256) void multidot2_kernel(const float* __restrict_
 }
 
 // w <- w - sum_i h_i V_i (FP64 accumulation), grid-stride; per-block partial
-// of ||w||^2 into nrm_partial[blk] when non-null.  Skipped when *gate == 0.
+// of ||w||^2 into nrm_partial[blk] when non-null.  NV >= n coefficients are
+// held in registers and all NV basis loads of an element are issued before
+// use (rows past n re-read row 0 with coefficient 0).  Skipped when
+// *gate == 0.
+template <int NV>
 __global__ __launch_bounds__(256) void update_gs_kernel(const float* __restrict__ V, size_t ld, int n,
                                                         const double* __restrict__ h, float* __restrict__ w, size_t Np,
                                                         double* __restrict__ nrm_partial, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ double sh[16];
+  double hv[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) hv[i] = i < n ? h[i] : 0.0;
+  double nrm = 0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    const float* q = V + p;
+    float vv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) vv[i] = q[size_t(i < n ? i : 0) * ld];
+    double acc = w[p];
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc -= hv[i] * double(vv[i]);
+    const float f = float(acc);
+    w[p] = f;
+    nrm += double(f) * double(f);
+  }
+  if (nrm_partial) {
+    const double r = block_sum(nrm, sh);
+    if (threadIdx.x == 0) nrm_partial[blockIdx.x] = r;
+  }
+}
+
+// The same update for bases wider than the register version (n > 64).
+__global__ __launch_bounds__(256) void update_wide_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                          const double* __restrict__ h, float* __restrict__ w,
+                                                          size_t Np, double* __restrict__ nrm_partial,
+                                                          const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double hs[kMaxBasis];
   __shared__ double sh[16];
@@ -270,6 +248,36 @@ __global__ __launch_bounds__(256) void update_gs_kernel(const float* __restrict_
     const double r = block_sum(nrm, sh);
     if (threadIdx.x == 0) nrm_partial[blockIdx.x] = r;
   }
+}
+
+void launch_update(Context& c, const float* V, size_t ld, int n, const double* h, float* w, size_t Np,
+                   double* nrm_partial, const int* gate, int G) {
+  if (n < 1 || n > kProjectMaxN) throw ArgError("update width out of range");
+  const dim3 g(G), b(256);
+#define GPB_UPD(NV)                                                                                               \
+  case NV / 4:                                                                                                    \
+    hipLaunchKernelGGL(update_gs_kernel<NV>, g, b, 0, c.stream, V, ld, n, h, w, Np, nrm_partial, gate);         \
+    break;
+  switch ((n + 3) / 4) {
+    GPB_UPD(4)
+    GPB_UPD(8)
+    GPB_UPD(12)
+    GPB_UPD(16)
+    GPB_UPD(20)
+    GPB_UPD(24)
+    GPB_UPD(28)
+    GPB_UPD(32)
+    GPB_UPD(36)
+    GPB_UPD(40)
+    GPB_UPD(44)
+    GPB_UPD(48)
+    GPB_UPD(52)
+    GPB_UPD(56)
+    GPB_UPD(60)
+    GPB_UPD(64)
+  }
+#undef GPB_UPD
+  GPB_LAUNCH_CHECK();
 }
 
 // Lanczos projection in one pass over the basis.  With alpha: first the
@@ -326,8 +334,6 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     partial[size_t(t) * gridDim.x + blockIdx.x] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
   }
 }
-
-constexpr int kProjectMaxN = 64;
 
 void launch_project(Context& c, const float* V, size_t ld, int n, float* w, const double* alpha, const double* beta,
                     int j, const int* gate, size_t Np, double* partial, int G) {
@@ -545,7 +551,7 @@ struct Reducer {
   Reducer(Context& cc, size_t np) : c(cc), Np(np) {
     NW = int((np + kDotChunk - 1) / kDotChunk);
     GU = int(std::min<size_t>(2048, std::max<size_t>(1, (np + 255) / 256)));
-    GP = int(std::min<size_t>(512, std::max<size_t>(1, (np + 255) / 256)));
+    GP = int(std::min<size_t>(1024, std::max<size_t>(1, (np + 255) / 256)));
     partial = c.ws<double>("eig.partial", std::max(size_t(kMaxBasis + 2) * NW, size_t(kProjectMaxN + 2) * GP));
     npart = c.ws<double>("eig.npart", size_t(GU));
   }
@@ -576,9 +582,13 @@ struct Reducer {
               const int* gate = nullptr) {
     {
       KScope ks(c, kKUpdate, gate ? 0.0 : double(Np) * (4.0 * n + 8));
-      hipLaunchKernelGGL(update_gs_kernel, dim3(GU), dim3(256), 0, c.stream, V, ld, n, h, w, Np,
-                         nrm_dst ? npart : nullptr, gate);
-      GPB_LAUNCH_CHECK();
+      if (n <= kProjectMaxN) {
+        launch_update(c, V, ld, n, h, w, Np, nrm_dst ? npart : nullptr, gate, GU);
+      } else {
+        hipLaunchKernelGGL(update_wide_kernel, dim3(GU), dim3(256), 0, c.stream, V, ld, n, h, w, Np,
+                           nrm_dst ? npart : nullptr, gate);
+        GPB_LAUNCH_CHECK();
+      }
     }
     if (nrm_dst) {
       hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, npart, GU, nrm_dst, gate);

@@ -69,6 +69,13 @@ _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
 _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_cut_patches", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _i32p, _i32p])
+_sig("gpb_compute_patches", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_int, _i32p, _i32p, C.c_int,
+                                      C.c_void_p, C.c_void_p])
+_sig("gpb_compute_patches_device", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _i32p, _i32p,
+                                             C.c_int, C.c_void_p, C.c_void_p])
+_sig("gpb_get_patch_stats", C.c_int, [_ctxp, C.c_int, C.c_void_p])
 _sig("gpb_get_spmv_tile", C.c_int, [_ctxp, C.c_int, C.c_int, C.POINTER(C.c_int)])
 _sig("gpb_apply_laplacian", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, C.c_int, C.c_int, _f32p, _f32p])
 _sig("gpb_kernel_time", C.c_int, [_ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double)])
@@ -115,6 +122,16 @@ def pack_rgb(rgb: np.ndarray) -> np.ndarray:
     out = np.empty(rgb.shape[:2], np.uint32)
     lib.gpb_pack_rgb(rgb.reshape(-1), rgb.shape[0] * rgb.shape[1], out.reshape(-1))
     return out
+
+
+def cut_patches(width: int, height: int, grid: int, overlap: int, seed: int):
+    """(rects, cores): [grid*grid, 4] int32 arrays of (x, y, w, h)."""
+    rects = np.empty((grid * grid, 4), np.int32)
+    cores = np.empty((grid * grid, 4), np.int32)
+    rc = lib.gpb_cut_patches(width, height, grid, overlap, seed, rects.reshape(-1), cores.reshape(-1))
+    if rc != GPB_OK:
+        raise GpbError(rc, "bad patch grid")
+    return rects, cores
 
 
 def make_codebook(packed: np.ndarray, seed: int = 1, iters: int = 10, max_samples: int = 65536) -> np.ndarray:
@@ -192,6 +209,38 @@ class Context:
     def set_eig_options(self, nvec=9, basis=48, keep=24, tol=1e-3, max_matvecs=20000):
         self._chk(lib.gpb_set_eig_options(self.h, nvec, basis, keep, tol, max_matvecs))
         self.nvec = nvec
+
+    @staticmethod
+    def _stats_dict(s) -> dict:
+        return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
+                    basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
+                    evals=list(s.evals), fallback=s.fallback, second_passes=s.reorth_second_passes)
+
+    def patch_stats(self, patch: int) -> dict:
+        s = Stats()
+        self._chk(lib.gpb_get_patch_stats(self.h, patch, C.byref(s)))
+        return self._stats_dict(s)
+
+    def compute_patches(self, packed: np.ndarray, codebook: np.ndarray, rects: np.ndarray, cores: np.ndarray,
+                        mode: int = 1):
+        """Patch-wise gPb (host buffers): returns stitched gpb_max [H,W] and gpb_orient [8,H,W]."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        r = np.ascontiguousarray(rects, np.int32)
+        c = np.ascontiguousarray(cores, np.int32)
+        gmax = np.zeros((H, W), np.float32)
+        orient = np.zeros((8, H, W), np.float32)
+        self._chk(lib.gpb_compute_patches(self.h, W, H, packed.reshape(-1), np.ascontiguousarray(codebook, np.float32),
+                                          r.shape[0], r.reshape(-1), c.reshape(-1), mode, gmax.ctypes.data,
+                                          orient.ctypes.data))
+        return gmax, orient
+
+    def compute_patches_device(self, W: int, H: int, d_packed: int, d_codebook: int, rects: np.ndarray,
+                               cores: np.ndarray, mode: int, d_gmax: int = 0, d_orient: int = 0):
+        r = np.ascontiguousarray(rects, np.int32)
+        c = np.ascontiguousarray(cores, np.int32)
+        self._chk(lib.gpb_compute_patches_device(self.h, W, H, d_packed, d_codebook, r.shape[0], r.reshape(-1),
+                                                 c.reshape(-1), mode, d_gmax or None, d_orient or None))
 
     def stats(self) -> dict:
         s = Stats()
@@ -271,6 +320,10 @@ class Context:
     def set_spmv_tile(self, rows: int):
         """SpMV rows per thread (2, 4, 8), or 0 to autotune per image shape."""
         self._chk(lib.gpb_set_spmv_tile(self.h, rows))
+
+    def set_spmv_fp64(self, on: bool):
+        """SpMV arithmetic in FP64 (default) or FP32; HBM operands are FP32 either way."""
+        self._chk(lib.gpb_set_spmv_precision(self.h, 1 if on else 0))
 
     def spmv_tile(self, width: int, height: int) -> int:
         r = C.c_int(0)
