@@ -139,6 +139,10 @@ int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows);
  * and FP64 (Spectra, src/spectralPb.cpp:221-244). */
 int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
 
+/* level > 0: the eigensolvers write one progress line per restart cycle to
+ * stderr (operator applications, residual); 0 (default): silent. */
+int gpb_set_verbose(gpb_ctx* ctx, int level);
+
 /* Kernel classes for gpb_kernel_time. */
 #define GPB_K_TEXTON 0
 #define GPB_K_CUES 1
@@ -237,6 +241,24 @@ int gpb_compute_patches_device(gpb_ctx* ctx, int width, int height, const uint32
  * gpb_get_stats then holds their sum (matvecs, converged) and maximum
  * (residual, restarts). */
 int gpb_get_patch_stats(gpb_ctx* ctx, int patch, gpb_stats* out);
+
+/* ---- the all-FP64 path (precision reference) ---------------------------------------- */
+
+/* Whole gPb with every stage in FP64 on the GPU: the same texton labels, then
+ * FP64 half-disc cues, mPb and nonmax, W stored in FP64 (the pair's two FP32
+ * directed affinities averaged in FP64, as buildW does), an FP64 Lanczos
+ * (FP64 vectors and stencil, Gram-Schmidt twice on every step) and FP64
+ * sPb / gPb.  Outputs (host, any may be NULL): mpb_nmax[N], gpb_orient[8][N]
+ * (before calibration), gpb_max[N] (masked max), evals[nvec], evecs[nvec][N]
+ * (v' D v = 1).  The reference for the mixed-precision study of the
+ * production path; the reference's own CPU path is FP64
+ * (src/multiscalePb.cpp, src/spectralPb.cpp). */
+int gpb_compute_f64(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                    double* mpb_nmax, double* gpb_orient, double* gpb_max, double* evals, double* evecs);
+
+/* The FP64 eigensolve alone on a given mpb_nmax (FP64 W and Lanczos):
+ * evals[nvec], evecs[nvec][N] (may be NULL). */
+int gpb_eigs_f64(gpb_ctx* ctx, int width, int height, const double* mpb_nmax, double* evals, double* evecs);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 

@@ -56,8 +56,6 @@ __device__ __forceinline__ gptr<T> as_global_mut(T* p) {
   return (gptr<T>)p;
 }
 
-__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 template <int ROWS, typename AT>
 __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, const float* __restrict__ s_,

@@ -161,6 +161,7 @@ struct Context {
   int spmv_rows_forced = 0;
   bool spmv_fp64 = true;   // SpMV arithmetic in FP64 (operands and results FP32 in HBM)
   bool capturing = false;
+  int verbose = 0;         // > 0: solvers report each restart on stderr
   // per-patch solver statistics and eigenvalues of the last patch call
   std::vector<LanczosStats> patch_stats;
   std::vector<double> patch_evals;  // a hipGraph capture is open on the stream (no event timing)
@@ -181,6 +182,11 @@ struct Context {
 };
 
 inline unsigned div_up(size_t a, size_t b) { return unsigned((a + b - 1) / b); }
+
+// Fused multiply-add in the operands' own precision (kernels templated on
+// their arithmetic type).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Records a start/stop event pair around the launches in its scope.
 struct KScope {

@@ -36,6 +36,28 @@ struct Stencil {
 // Workspace buffers are keyed by tag (distinct tags keep several stencils alive).
 void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, const std::string& tag = "");
 void stencil_upper_offsets(int* dx, int* dy);
+// The all-FP64 variant (precision reference): W, D, D^-1/2 stored in FP64.
+struct Stencil64 {
+  int W = 0, H = 0, Wp = 0, Hp = 0;
+  size_t Np = 0;
+  double* wup = nullptr;
+  double* D = nullptr;
+  double* s = nullptr;
+  int* offs = nullptr;
+  int off_host[kStencilHalf] = {};
+};
+void build_stencil_f64(Context& c, const double* d_mpb, int W, int H, Stencil64& st);
+void launch_fill_f64(Context& c, double* p, size_t n, double v);
+void launch_local_cues_f64(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, double* d_mpb_all,
+                           double* d_gcue, double* d_cues);
+void launch_nonmax_f64(Context& c, const double* d_mpb_all, int W, int H, double* d_nmax);
+void launch_spb_f64(Context& c, const Stencil64& st, const double* d_evecs, const double* evals, int K,
+                    double* d_spb);
+void launch_gpb_f64(Context& c, int W, int H, const double* d_gcue, const double* d_spb, const double* d_nmax,
+                    double* d_orient, double* d_gmax);
+struct LanczosStats;
+void lanczos_solve_f64(Context& c, const Stencil64& st, const struct EigOptions& opt, double* evals, double* d_evecs,
+                       LanczosStats& stats);
 void launch_fill(Context& c, float* p, size_t n, float v);
 
 // stage (d)

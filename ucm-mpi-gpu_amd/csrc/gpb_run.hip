@@ -302,6 +302,10 @@ int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows) {
   });
 }
 
+int gpb_set_verbose(gpb_ctx* ctx, int level) {
+  return guarded2(ctx, [&] { ctx->c.verbose = level; });
+}
+
 int gpb_set_block_size(gpb_ctx* ctx, int block) {
   return guarded2(ctx, [&] {
     if (!(block == 1 || block == 4 || block == 8 || block == 12 || block == 16))
@@ -616,5 +620,96 @@ int gpb_compute_patches(gpb_ctx* ctx, int width, int height, const uint32_t* pac
     if (gpb_max) GPB_HIP(hipMemcpyAsync(gpb_max, d_gmax, N * 4, hipMemcpyDeviceToHost, c.stream));
     if (gpb_orient) GPB_HIP(hipMemcpyAsync(gpb_orient, d_orient, 8 * N * 4, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+// ---- the all-FP64 path (precision reference) ---------------------------------------------
+
+namespace {
+
+// Unpadded [K][H][W] double host copy of a padded device block.
+void unpad64_to_host(Context& c, const double* d, int K, int W, int H, double* host) {
+  const int Wp = W + 2 * kICRadius;
+  const size_t Np = size_t(Wp) * (H + 2 * kICRadius);
+  for (int k = 0; k < K; k++)
+    GPB_HIP(hipMemcpy2DAsync(host + size_t(k) * W * H, size_t(W) * 8,
+                             d + size_t(k) * Np + size_t(kICRadius) * Wp + kICRadius, size_t(Wp) * 8, size_t(W) * 8, H,
+                             hipMemcpyDeviceToHost, c.stream));
+}
+
+}  // namespace
+
+int gpb_eigs_f64(gpb_ctx* ctx, int width, int height, const double* mpb_nmax, double* evals, double* evecs) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!mpb_nmax || !evals) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    const EigOptions eo = ctx_eig_options(c);
+    double* d_m = c.ws<double>("f64.mpb", N);
+    GPB_HIP(hipMemcpyAsync(d_m, mpb_nmax, N * 8, hipMemcpyHostToDevice, c.stream));
+    Stencil64 st;
+    build_stencil_f64(c, d_m, width, height, st);
+    double* d_ev = c.ws<double>("f64.evecs", size_t(eo.K) * st.Np);
+    lanczos_solve_f64(c, st, eo, evals, d_ev, c.last_lanczos);
+    if (evecs) unpad64_to_host(c, d_ev, eo.K, width, height, evecs);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < eo.K; i++) c.last_evals[i] = evals[i];
+  });
+}
+
+int gpb_compute_f64(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                    double* mpb_nmax, double* gpb_orient, double* gpb_max, double* evals, double* evecs) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!packed || !codebook) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const int W = width, H = height;
+    const size_t N = size_t(W) * H;
+    const EigOptions eo = ctx_eig_options(c);
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, sizeof(float) * kSchmidNF * kTextonK, hipMemcpyHostToDevice, c.stream));
+    uint8_t* text = c.ws<uint8_t>("run.text", N);
+    double* mpb_all = c.ws<double>("f64.mpb_all", 8 * N);
+    double* gcue = c.ws<double>("f64.gcue", 8 * N);
+    double* nmax = c.ws<double>("f64.nmax", N);
+    StageEvents se;
+    GPB_HIP(hipEventRecord(se.ev[0], c.stream));
+    launch_texton(c, d_rgb, W, H, d_cb, text);
+    GPB_HIP(hipEventRecord(se.ev[1], c.stream));
+    launch_local_cues_f64(c, d_rgb, text, W, H, mpb_all, gcue, nullptr);
+    GPB_HIP(hipEventRecord(se.ev[2], c.stream));
+    launch_nonmax_f64(c, mpb_all, W, H, nmax);
+    GPB_HIP(hipEventRecord(se.ev[3], c.stream));
+    Stencil64 st;
+    build_stencil_f64(c, nmax, W, H, st);
+    GPB_HIP(hipEventRecord(se.ev[4], c.stream));
+    double ev[kMaxNvec];
+    double* d_ev = c.ws<double>("f64.evecs", size_t(eo.K) * st.Np);
+    lanczos_solve_f64(c, st, eo, ev, d_ev, c.last_lanczos);
+    GPB_HIP(hipEventRecord(se.ev[5], c.stream));
+    double* spb = c.ws<double>("f64.spb", 8 * N);
+    double* orient = c.ws<double>("f64.orient", 8 * N);
+    double* gmax = c.ws<double>("f64.gmax", N);
+    launch_spb_f64(c, st, d_ev, ev, eo.K, spb);
+    launch_gpb_f64(c, W, H, gcue, spb, nmax, orient, gmax);
+    GPB_HIP(hipEventRecord(se.ev[6], c.stream));
+    if (mpb_nmax) GPB_HIP(hipMemcpyAsync(mpb_nmax, nmax, N * 8, hipMemcpyDeviceToHost, c.stream));
+    if (gpb_orient) GPB_HIP(hipMemcpyAsync(gpb_orient, orient, 8 * N * 8, hipMemcpyDeviceToHost, c.stream));
+    if (gpb_max) GPB_HIP(hipMemcpyAsync(gpb_max, gmax, N * 8, hipMemcpyDeviceToHost, c.stream));
+    if (evecs) unpad64_to_host(c, d_ev, eo.K, W, H, evecs);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    c.kt.fold();
+    for (int s2 = 0; s2 < kNumStages; s2++) {
+      float ms = 0;
+      GPB_HIP(hipEventElapsedTime(&ms, se.ev[s2], se.ev[s2 + 1]));
+      c.stage_ms[s2] = ms;
+    }
+    for (int i = 0; i < eo.K; i++) {
+      c.last_evals[i] = ev[i];
+      if (evals) evals[i] = ev[i];
+    }
   });
 }

@@ -90,10 +90,15 @@ struct AffParams {
   int updx[kStencilHalf], updy[kStencilHalf], upk[kStencilHalf];
   int W, H, Wp;          // Wp = W + 10 (padded row length)
   size_t Np;
-  float* wup;            // [40][Np] symmetric affinities, zero halo
+  void* wup;             // [40][Np] symmetric affinities, zero halo (float, or double in the FP64 variant)
 };
 
 __device__ __forceinline__ int clip_code(int lo_dist) { return lo_dist < kICRadius ? lo_dist : kICRadius; }
+
+__global__ __launch_bounds__(256) void to_f32_kernel(const double* __restrict__ in, size_t n, float* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    out[i] = float(in[i]);
+}
 
 // Affinity of a directed connection from the IC max (affinity.cc:40-52,
 // 121-143): pss = C_IC_SS(1 - icsim), W = exp(-(1 - pss) / 0.1).
@@ -111,6 +116,10 @@ __device__ __forceinline__ float directed_w(float maxpb, bool found) {
   return float(exp(double(e)));
 }
 
+// WT: storage of W.  The pair's two directed affinities are FP32 values (as
+// affinity.cc:121-143 computes them) averaged in FP64 (smatrix.cc:95-117);
+// float storage rounds that mean once more.
+template <typename WT>
 __global__ __launch_bounds__(kAffTX * kAffTY) void affinity_kernel(AffParams P) {
   __shared__ float tile[kAffTileH * kAffTileW];
   const int tid = threadIdx.y * kAffTX + threadIdx.x;
@@ -129,7 +138,7 @@ __global__ __launch_bounds__(kAffTX * kAffTY) void affinity_kernel(AffParams P) 
   for (int j = 0; j < kStencilHalf; j++) {
     const int dx = P.updx[j], dy = P.updy[j];
     const int qx = x + dx, qy = y + dy;
-    float w = 0.f;
+    double w = 0.0;
     if (qx >= 0 && qx < P.W && qy >= 0 && qy < P.H) {
       const ICConnDev a = P.conn[size_t(cfg_p) * kStencilFull + P.upk[j]];
       float m1 = 0.f;
@@ -140,16 +149,17 @@ __global__ __launch_bounds__(kAffTX * kAffTY) void affinity_kernel(AffParams P) 
       float m2 = 0.f;
       for (int s = 0; s < b.site_count; s++) m2 = fmaxf(m2, tile[cq + P.site[b.site_start + s]]);
       const float w1 = directed_w(m1, a.found), w2 = directed_w(m2, b.found);
-      w = float(0.5 * (double(w1) + double(w2)));
+      w = 0.5 * (double(w1) + double(w2));
     }
-    P.wup[size_t(j) * P.Np + pp] = w;
+    static_cast<WT*>(P.wup)[size_t(j) * P.Np + pp] = WT(w);
   }
 }
 
 // D_p = 1 + sum of the 80 neighbour affinities; s_p = D_p^-1/2; diag = s_p^2.
-__global__ __launch_bounds__(256) void degree_kernel(const float* __restrict__ wup, int W, int H, int Wp, size_t Np,
+template <typename WT>
+__global__ __launch_bounds__(256) void degree_kernel(const WT* __restrict__ wup, int W, int H, int Wp, size_t Np,
                                                      const int* __restrict__ offs /* 40 padded offsets */,
-                                                     float* __restrict__ D, float* __restrict__ s) {
+                                                     WT* __restrict__ D, WT* __restrict__ s) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= W) return;
   const size_t pp = size_t(y + kICRadius) * Wp + (x + kICRadius);
@@ -158,8 +168,8 @@ __global__ __launch_bounds__(256) void degree_kernel(const float* __restrict__ w
     acc += double(wup[size_t(j) * Np + pp]);
     acc += double(wup[size_t(j) * Np + pp - offs[j]]);
   }
-  D[pp] = float(acc);
-  s[pp] = float(1.0 / sqrt(acc));
+  D[pp] = WT(acc);
+  s[pp] = WT(1.0 / sqrt(acc));
 }
 
 void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, const std::string& tag) {
@@ -197,9 +207,68 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, co
   P.Np = st.Np;
   P.wup = st.wup;
   KScope ks(c, kKAffinity, double(W) * H * (4 + 4 * kStencilHalf));
-  hipLaunchKernelGGL(affinity_kernel, dim3(div_up(W, kAffTX), div_up(H, kAffTY)), dim3(kAffTX, kAffTY), 0, c.stream, P);
+  hipLaunchKernelGGL(affinity_kernel<float>, dim3(div_up(W, kAffTX), div_up(H, kAffTY)), dim3(kAffTX, kAffTY), 0,
+                     c.stream, P);
   GPB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(degree_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, st.wup, W, H, st.Wp, st.Np,
+  hipLaunchKernelGGL(degree_kernel<float>, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, st.wup, W, H, st.Wp, st.Np,
+                     st.offs, st.D, st.s);
+  GPB_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void fill_d_kernel(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+void launch_fill_f64(Context& c, double* p, size_t n, double v) {
+  if (!n) return;
+  hipLaunchKernelGGL(fill_d_kernel, dim3(std::min<unsigned>(4096, div_up(n, 256))), dim3(256), 0, c.stream, p, n, v);
+  GPB_LAUNCH_CHECK();
+}
+
+// The same stencil with W, D and D^-1/2 stored in FP64, from an FP64 mPb
+// rounded to FP32 for the walks (the lattice of buildW.cpp:57-117 is FP32).
+void build_stencil_f64(Context& c, const double* d_mpb, int W, int H, Stencil64& st) {
+  if (W < 2 || H < 2) throw ArgError("the affinity needs an image at least 2 pixels wide and high");
+  AffTables& t = aff_tables(c);
+  st.W = W;
+  st.H = H;
+  st.Wp = W + 2 * kICRadius;
+  st.Hp = H + 2 * kICRadius;
+  st.Np = size_t(st.Wp) * st.Hp;
+  st.wup = c.ws<double>("st64.wup", kStencilHalf * st.Np);
+  st.D = c.ws<double>("st64.D", st.Np);
+  st.s = c.ws<double>("st64.s", st.Np);
+  st.offs = c.ws<int>("st64.offs", kStencilHalf);
+  float* m32 = c.ws<float>("st64.mpb32", size_t(W) * H);
+  hipLaunchKernelGGL(to_f32_kernel, dim3(std::min<unsigned>(4096, div_up(size_t(W) * H, 256))), dim3(256), 0, c.stream,
+                     d_mpb, size_t(W) * H, m32);
+  GPB_LAUNCH_CHECK();
+  int offs[kStencilHalf];
+  for (int j = 0; j < kStencilHalf; j++) offs[j] = t.updy[j] * st.Wp + t.updx[j];
+  GPB_HIP(hipMemcpyAsync(st.offs, offs, sizeof(offs), hipMemcpyHostToDevice, c.stream));
+  for (int j = 0; j < kStencilHalf; j++) st.off_host[j] = offs[j];
+  launch_fill_f64(c, st.wup, kStencilHalf * st.Np, 0.0);
+  launch_fill_f64(c, st.D, st.Np, 0.0);
+  launch_fill_f64(c, st.s, st.Np, 0.0);
+  AffParams P{};
+  P.mpb = m32;
+  P.conn = t.conn.p;
+  P.site = t.site.p;
+  for (int j = 0; j < kStencilHalf; j++) {
+    P.updx[j] = t.updx[j];
+    P.updy[j] = t.updy[j];
+    P.upk[j] = t.upk[j];
+  }
+  P.W = W;
+  P.H = H;
+  P.Wp = st.Wp;
+  P.Np = st.Np;
+  P.wup = st.wup;
+  KScope ks(c, kKAffinity, double(W) * H * (8 + 8 * kStencilHalf));
+  hipLaunchKernelGGL(affinity_kernel<double>, dim3(div_up(W, kAffTX), div_up(H, kAffTY)), dim3(kAffTX, kAffTY), 0,
+                     c.stream, P);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(degree_kernel<double>, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, st.wup, W, H, st.Wp, st.Np,
                      st.offs, st.D, st.s);
   GPB_LAUNCH_CHECK();
 }

@@ -99,42 +99,59 @@ struct CueParams {
   int end_by_r[21];         // number of offsets with d^2 <= r^2
   float smooth[3][9];       // half kernels: [0] bg (sigma 2.5, 17 taps), [1] cg (sigma 1.25, 9 taps), [2] none
   float w_mpb[12], w_gpb[12];
-  float* mpb_all;           // [8][N]
-  float* gcue;              // [8][N] sum of gPb-weighted cues
-  float* cues;              // optional [12][8][N]
+  double smooth_d[3][9];    // the same in FP64 (the all-FP64 variant)
+  double w_mpb_d[12], w_gpb_d[12];
+  void* mpb_all;            // [8][N] (float, or double in the FP64 variant)
+  void* gcue;               // [8][N] sum of gPb-weighted cues
+  void* cues;               // optional [12][8][N]
   int W, H;
 };
+
+template <typename CT>
+__device__ __forceinline__ const CT* smooth_of(const CueParams& P, int k);
+template <>
+__device__ __forceinline__ const float* smooth_of<float>(const CueParams& P, int k) { return P.smooth[k]; }
+template <>
+__device__ __forceinline__ const double* smooth_of<double>(const CueParams& P, int k) { return P.smooth_d[k]; }
+template <typename CT>
+__device__ __forceinline__ CT wmpb_of(const CueParams& P, int c) {
+  if constexpr (sizeof(CT) == 8) return P.w_mpb_d[c]; else return P.w_mpb[c];
+}
+template <typename CT>
+__device__ __forceinline__ CT wgpb_of(const CueParams& P, int c) {
+  if constexpr (sizeof(CT) == 8) return P.w_gpb_d[c]; else return P.w_gpb[c];
+}
 
 // One snapshot: chi-squared of all orientations from the lane's histogram.
 // Histogram word [grp][bin][lane] holds the 8-bit counts of slices 4*grp..+3.
 // csum[b] (LDS, shared by the wave) = column sums of the truncated smoothing.
-template <int NB, int KH>
+template <int NB, int KH, typename CT>
 __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
-                                           const float* __restrict__ g, const float* __restrict__ csum,
-                                           float out[kNumOri]) {
+                                           const CT* __restrict__ g, const CT* __restrict__ csum,
+                                           CT out[kNumOri]) {
   // pass A: per-slice mass after smoothing, a_s = 1 / mass
-  float alpha[kNumSlices];
+  CT alpha[kNumSlices];
 #pragma unroll
-  for (int s = 0; s < kNumSlices; s++) alpha[s] = 0.f;
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = CT(0);
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     if (b < nb) {
-      const float cb = csum[b];
+      const CT cb = csum[b];
 #pragma unroll
       for (int grp = 0; grp < 4; grp++) {
         const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) alpha[4 * grp + q] = __builtin_fmaf(cb, float((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
+        for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(cb, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
       }
     }
   }
 #pragma unroll
-  for (int s = 0; s < kNumSlices; s++) alpha[s] = alpha[s] > 0.f ? 1.0f / alpha[s] : 0.f;
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = alpha[s] > CT(0) ? CT(1) / alpha[s] : CT(0);
   // pass B: U_T = sum_s a_s h_s, V_0 = right - left for orientation 0
-  float T[NB], V[NB];
+  CT T[NB], V[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    float u = 0.f, v = 0.f;
+    CT u = CT(0), v = CT(0);
     if (b < nb) {
 #pragma unroll
       for (int grp = 0; grp < 4; grp++) {
@@ -142,9 +159,9 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const int s = 4 * grp + q;
-          const float c = float((w >> (8 * q)) & 0xffu);
-          u = __builtin_fmaf(alpha[s], c, u);
-          v = __builtin_fmaf(s < kNumOri ? -alpha[s] : alpha[s], c, v);
+          const CT c = CT((w >> (8 * q)) & 0xffu);
+          u = fma_t(alpha[s], c, u);
+          v = fma_t(s < kNumOri ? -alpha[s] : alpha[s], c, v);
         }
       }
     }
@@ -152,66 +169,66 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
     V[b] = v;
   }
   if (KH > 0) {
-    float U[NB];
+    CT U[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) U[b] = T[b];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      float acc = 0.f;
+      CT acc = CT(0);
 #pragma unroll
       for (int k = -KH; k <= KH; k++) {
         const int n = b + k;
-        if (n >= 0 && n < NB) acc = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], U[n], acc) : acc;
+        if (n >= 0 && n < NB) acc = (n < nb) ? fma_t(g[k < 0 ? -k : k], U[n], acc) : acc;
       }
       T[b] = acc;
     }
   }
-  float rT[NB];
+  CT rT[NB];
 #pragma unroll
-  for (int b = 0; b < NB; b++) rT[b] = (b < nb && T[b] > 0.f) ? 1.0f / T[b] : 0.f;
+  for (int b = 0; b < NB; b++) rT[b] = (b < nb && T[b] > CT(0)) ? CT(1) / T[b] : CT(0);
   for (int o = 0; o < kNumOri; o++) {
-    float acc = 0.f;
+    CT acc = CT(0);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      float d = V[b];
+      CT d = V[b];
       if (KH > 0) {
-        d = 0.f;
+        d = CT(0);
 #pragma unroll
         for (int k = -KH; k <= KH; k++) {
           const int n = b + k;
-          if (n >= 0 && n < NB) d = (n < nb) ? __builtin_fmaf(g[k < 0 ? -k : k], V[n], d) : d;
+          if (n >= 0 && n < NB) d = (n < nb) ? fma_t(g[k < 0 ? -k : k], V[n], d) : d;
         }
       }
-      if (b < nb && d != 0.f) acc = __builtin_fmaf(d * d, rT[b], acc);
+      if (b < nb && d != CT(0)) acc = fma_t(d * d, rT[b], acc);
     }
-    out[o] = 0.5f * acc;
+    out[o] = CT(0.5) * acc;
     if (o + 1 < kNumOri) {
-      float a0 = 0.f, a8 = 0.f;
+      CT a0 = CT(0), a8 = CT(0);
 #pragma unroll
       for (int s = 0; s < kNumOri; s++) {
         a0 = (s == o) ? alpha[s] : a0;
         a8 = (s == o) ? alpha[s + kNumOri] : a8;
       }
-      a0 *= 2.f;
-      a8 *= -2.f;
+      a0 *= CT(2);
+      a8 *= -CT(2);
       const int sh = 8 * (o & 3);
       const uint32_t* h0 = hist + ((o >> 2) * kNbMax) * kCueLanes + lane;
       const uint32_t* h8 = hist + (((o >> 2) + 2) * kNbMax) * kCueLanes + lane;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         if (b < nb) {
-          const float c0 = float((h0[b * kCueLanes] >> sh) & 0xffu);
-          const float c8 = float((h8[b * kCueLanes] >> sh) & 0xffu);
-          V[b] = __builtin_fmaf(a8, c8, __builtin_fmaf(a0, c0, V[b]));
+          const CT c0 = CT((h0[b * kCueLanes] >> sh) & 0xffu);
+          const CT c8 = CT((h8[b * kCueLanes] >> sh) & 0xffu);
+          V[b] = fma_t(a8, c8, fma_t(a0, c0, V[b]));
         }
       }
     }
   }
 }
 
-template <int NB, int KH>
+template <int NB, int KH, typename CT>
 __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radii, uint8_t* tile, uint32_t* hist,
-                            float* csum, int lane, int px, int py, bool valid, float mpb[kNumOri], float gc[kNumOri]) {
+                            CT* csum, int lane, int px, int py, bool valid, CT mpb[kNumOri], CT gc[kNumOri]) {
   const int W = P.W, H = P.H;
   const int gx0 = int(blockIdx.x) * kCueTile - kCueRmax, gy0 = int(blockIdx.y) * kCueTile - kCueRmax;
   const uint8_t* plane = P.plane[ch];
@@ -225,14 +242,14 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
   }
   for (int i = 0; i < 4 * kNbMax; i++) hist[i * kCueLanes + lane] = 0u;
   const int nb = P.nbins[ch];
-  const float* g = P.smooth[KH == 8 ? 0 : (KH == 4 ? 1 : 2)];
+  const CT* g = smooth_of<CT>(P, KH == 8 ? 0 : (KH == 4 ? 1 : 2));
   if (lane < kNbMax) {
-    float s = 0.f;
+    CT s = 0;
     for (int k = -KH; k <= KH; k++) {
       const int n = lane - k;
       if (n >= 0 && n < nb) s += g[k < 0 ? -k : k];
     }
-    csum[lane] = (lane < nb) ? s : 0.f;
+    csum[lane] = (lane < nb) ? s : CT(0);
   }
   __syncthreads();
   const int ty = lane / kCueTile, tx = lane - ty * kCueTile;
@@ -246,63 +263,65 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
       const uint32_t lab = centre[P.off[i]];
       atomicAdd(&hist[((s >> 2) * kNbMax + lab) * kCueLanes + lane], 1u << (8 * (s & 3)));
     }
-    float chi[kNumOri];
-    chi2_snapshot<NB, KH>(hist, lane, nb, g, csum, chi);
+    CT chi[kNumOri];
+    chi2_snapshot<NB, KH, CT>(hist, lane, nb, g, csum, chi);
     const int cue = ri0 + ri;
     if (valid) {
       const size_t N = size_t(W) * H, p = size_t(py) * W + px;
 #pragma unroll
       for (int o = 0; o < kNumOri; o++) {
-        mpb[o] = __builtin_fmaf(P.w_mpb[cue], chi[o], mpb[o]);
-        gc[o] = __builtin_fmaf(P.w_gpb[cue], chi[o], gc[o]);
-        if (P.cues) P.cues[(size_t(cue) * kNumOri + o) * N + p] = chi[o];
+        mpb[o] = fma_t(wmpb_of<CT>(P, cue), chi[o], mpb[o]);
+        gc[o] = fma_t(wgpb_of<CT>(P, cue), chi[o], gc[o]);
+        if (P.cues) static_cast<CT*>(P.cues)[(size_t(cue) * kNumOri + o) * N + p] = chi[o];
       }
     }
   }
   __syncthreads();
 }
 
+template <typename CT>
 __global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
   __shared__ uint8_t tile[kCueTileW * kCueTileW];
   __shared__ uint32_t hist[4 * kNbMax * kCueLanes];
-  __shared__ float csum[kNbMax];
+  __shared__ CT csum[kNbMax];
   const int lane = threadIdx.x;
   const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = blockIdx.y * kCueTile + (lane / kCueTile);
   const bool valid = px < P.W && py < P.H;
-  float mpb[kNumOri], gc[kNumOri];
+  CT mpb[kNumOri], gc[kNumOri];
 #pragma unroll
-  for (int o = 0; o < kNumOri; o++) mpb[o] = gc[o] = 0.f;
+  for (int o = 0; o < kNumOri; o++) mpb[o] = gc[o] = CT(0);
   static constexpr int rbg[3] = {3, 5, 10}, rcg[3] = {5, 10, 20};
-  cue_channel<kLabBins, 8>(P, 0, 0, rbg, tile, hist, csum, lane, px, py, valid, mpb, gc);
-  cue_channel<kLabBins, 4>(P, 1, 3, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
-  cue_channel<kLabBins, 4>(P, 2, 6, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
-  cue_channel<kTextonK, 0>(P, 3, 9, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 8, CT>(P, 0, 0, rbg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4, CT>(P, 1, 3, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kLabBins, 4, CT>(P, 2, 6, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
+  cue_channel<kTextonK, 0, CT>(P, 3, 9, rcg, tile, hist, csum, lane, px, py, valid, mpb, gc);
   if (valid) {
     const size_t N = size_t(P.W) * P.H, p = size_t(py) * P.W + px;
 #pragma unroll
     for (int o = 0; o < kNumOri; o++) {
-      P.mpb_all[o * N + p] = mpb[o];
-      P.gcue[o * N + p] = gc[o];
+      static_cast<CT*>(P.mpb_all)[o * N + p] = mpb[o];
+      static_cast<CT*>(P.gcue)[o * N + p] = gc[o];
     }
   }
 }
 
 // ---- oriented nonmax ---------------------------------------------------------
-__global__ __launch_bounds__(256) void max_ori_kernel(const float* __restrict__ mpb_all, size_t n,
-                                                      float* __restrict__ a, uint8_t* __restrict__ k) {
+template <typename VT>
+__global__ __launch_bounds__(256) void max_ori_kernel(const VT* __restrict__ mpb_all, size_t n,
+                                                      VT* __restrict__ a, uint8_t* __restrict__ k) {
   const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  float mv = -__builtin_huge_valf();
+  VT mv = -VT(__builtin_huge_val());
   int mi = 0;
 #pragma unroll
   for (int o = 0; o < kNumOri; o++) {
-    const float v = mpb_all[o * n + p];
+    const VT v = mpb_all[o * n + p];
     if (v > mv) {
       mv = v;
       mi = o;
     }
   }
-  a[p] = mv < 0.f ? 0.f : mv;
+  a[p] = mv < VT(0) ? VT(0) : mv;
   k[p] = uint8_t(mi);
 }
 
@@ -312,9 +331,10 @@ struct NmsTable {
   double cosd[kNumOri][kNumOri];
 };
 
-// nonmax_oriented_2D in FP64 on the FP32 inputs, then the multiscalePb clip.
-__global__ __launch_bounds__(256) void nonmax_kernel(const float* __restrict__ a, const uint8_t* __restrict__ k, int W,
-                                                     int H, NmsTable T, float* __restrict__ nmax) {
+// nonmax_oriented_2D in FP64 on the VT inputs, then the multiscalePb clip.
+template <typename VT>
+__global__ __launch_bounds__(256) void nonmax_kernel(const VT* __restrict__ a, const uint8_t* __restrict__ k, int W,
+                                                     int H, NmsTable T, VT* __restrict__ nmax) {
 #pragma clang fp contract(off)
   const int x = blockIdx.x * blockDim.x + threadIdx.x;  // column (lib_image y)
   const int y = blockIdx.y;                             // row (lib_image x)
@@ -361,7 +381,7 @@ __global__ __launch_bounds__(256) void nonmax_kernel(const float* __restrict__ a
   }
   if (1.2 * res > 1) res = 1;
   if (res < 0) res = 0;
-  nmax[n] = float(res);
+  nmax[n] = VT(res);
 }
 
 // ---- host side -------------------------------------------------------------------
@@ -372,6 +392,7 @@ struct CueTables {
   DevBuf<double> gamma;
   int end_by_r[21];
   float smooth[3][9];
+  double smooth_d[3][9];
   NmsTable nms;
 };
 std::map<int, std::unique_ptr<CueTables>> g_cue_tables;  // per device
@@ -413,6 +434,9 @@ static CueTables& cue_tables(Context& c) {
     t->smooth[0][k] = float(gbg[size_t(8 + k)]);
     t->smooth[1][k] = k < 5 ? float(gcg[size_t(4 + k)]) : 0.f;
     t->smooth[2][k] = k == 0 ? 1.f : 0.f;
+    t->smooth_d[0][k] = gbg[size_t(8 + k)];
+    t->smooth_d[1][k] = k < 5 ? gcg[size_t(4 + k)] : 0.0;
+    t->smooth_d[2][k] = k == 0 ? 1.0 : 0.0;
   }
   const NonmaxTable& nt = nonmax_table();
   for (int k = 0; k < kNumOri; k++) {
@@ -424,8 +448,9 @@ static CueTables& cue_tables(Context& c) {
   return *slot;
 }
 
-void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, float* d_mpb_all,
-                       float* d_gcue, float* d_cues) {
+template <typename CT>
+static void local_cues_t(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, CT* d_mpb_all,
+                         CT* d_gcue, CT* d_cues) {
   CueTables& t = cue_tables(c);
   const size_t N = size_t(W) * H;
   uint8_t* q = c.ws<uint8_t>("cue.q", 3 * N);
@@ -448,19 +473,34 @@ void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text,
   P.slice = t.slice.p;
   for (int r = 0; r <= kCueRmax; r++) P.end_by_r[r] = t.end_by_r[r];
   for (int a = 0; a < 3; a++)
-    for (int k = 0; k < 9; k++) P.smooth[a][k] = t.smooth[a][k];
+    for (int k = 0; k < 9; k++) {
+      P.smooth[a][k] = t.smooth[a][k];
+      P.smooth_d[a][k] = t.smooth_d[a][k];
+    }
   for (int i = 0; i < kNumCues; i++) {
     P.w_mpb[i] = float(kMPbWeights[i]);
     P.w_gpb[i] = float(kGPbWeights[i]);
+    P.w_mpb_d[i] = kMPbWeights[i];
+    P.w_gpb_d[i] = kGPbWeights[i];
   }
   P.mpb_all = d_mpb_all;
   P.gcue = d_gcue;
   P.cues = d_cues;
   P.W = W;
   P.H = H;
-  KScope ks(c, kKCues, double(N) * (4 + 64 + (d_cues ? 384 : 0)));
-  hipLaunchKernelGGL(cues_kernel, dim3(div_up(W, kCueTile), div_up(H, kCueTile)), dim3(kCueLanes), 0, c.stream, P);
+  KScope ks(c, kKCues, double(N) * (4 + 16 * sizeof(CT) + (d_cues ? 96 * sizeof(CT) : 0)));
+  hipLaunchKernelGGL(cues_kernel<CT>, dim3(div_up(W, kCueTile), div_up(H, kCueTile)), dim3(kCueLanes), 0, c.stream, P);
   GPB_LAUNCH_CHECK();
+}
+
+void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, float* d_mpb_all,
+                       float* d_gcue, float* d_cues) {
+  local_cues_t<float>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues);
+}
+
+void launch_local_cues_f64(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, double* d_mpb_all,
+                           double* d_gcue, double* d_cues) {
+  local_cues_t<double>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues);
 }
 
 void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d_q /* 3N */, int* d_nb /* 3 */) {
@@ -472,16 +512,25 @@ void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d
   GPB_LAUNCH_CHECK();
 }
 
-void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nmax) {
+template <typename VT>
+static void nonmax_t(Context& c, const VT* d_mpb_all, int W, int H, VT* d_nmax) {
   CueTables& t = cue_tables(c);
   const size_t N = size_t(W) * H;
-  float* a = c.ws<float>("nms.a", N);
+  VT* a = c.ws<VT>(sizeof(VT) == 8 ? "nms.a64" : "nms.a", N);
   uint8_t* k = c.ws<uint8_t>("nms.k", N);
-  KScope ks(c, kKNonmax, double(N) * 46);
-  hipLaunchKernelGGL(max_ori_kernel, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_mpb_all, N, a, k);
+  KScope ks(c, kKNonmax, double(N) * (10 * sizeof(VT) + 6));
+  hipLaunchKernelGGL(max_ori_kernel<VT>, dim3(div_up(N, 256)), dim3(256), 0, c.stream, d_mpb_all, N, a, k);
   GPB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(nonmax_kernel, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, a, k, W, H, t.nms, d_nmax);
+  hipLaunchKernelGGL(nonmax_kernel<VT>, dim3(div_up(W, 256), H), dim3(256), 0, c.stream, a, k, W, H, t.nms, d_nmax);
   GPB_LAUNCH_CHECK();
+}
+
+void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nmax) {
+  nonmax_t<float>(c, d_mpb_all, W, H, d_nmax);
+}
+
+void launch_nonmax_f64(Context& c, const double* d_mpb_all, int W, int H, double* d_nmax) {
+  nonmax_t<double>(c, d_mpb_all, W, H, d_nmax);
 }
 
 }  // namespace gpb

@@ -19,6 +19,7 @@
 // residual ||(D - W) v - lambda D v|| / ||D v||, computed exactly from the
 // Krylov relation (the residual of every Ritz vector is parallel to v_m).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -551,7 +552,7 @@ struct Reducer {
   Reducer(Context& cc, size_t np) : c(cc), Np(np) {
     NW = int((np + kDotChunk - 1) / kDotChunk);
     GU = int(std::min<size_t>(2048, std::max<size_t>(1, (np + 255) / 256)));
-    GP = int(std::min<size_t>(1024, std::max<size_t>(1, (np + 255) / 256)));
+    GP = int(std::min<size_t>(512, std::max<size_t>(1, (np + 255) / 256)));
     partial = c.ws<double>("eig.partial", std::max(size_t(kMaxBasis + 2) * NW, size_t(kProjectMaxN + 2) * GP));
     npart = c.ws<double>("eig.npart", size_t(GU));
   }
@@ -760,6 +761,8 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       worst = std::max(worst, resid[i]);
     }
     stats.max_residual = worst;
+    if (c.verbose)
+      std::fprintf(stderr, "gpb: Lanczos %dx%d: %d operator applications, residual %.3e\n", st.W, st.H, matvecs, worst);
     converged = worst <= opt.tol;
     if (converged || matvecs >= opt.max_matvecs) break;
     // thick restart: basis = Ritz vectors [0, keep) and v_m; T = diag(theta)

@@ -14,34 +14,36 @@ namespace gpb {
 constexpr int kSpbT = 16;
 constexpr int kSpbTile = kSpbT + 2 * kSPbRadius;  // 22
 
+template <typename VT>
 struct SpbParams {
-  float filt[kNumOri][kSPbSize * kSPbSize];
-  const float* evecs;      // [K][Np] padded
-  const float* mnmx;       // [K][2] min, max over the image
-  float inv_sqrt_lam[kMaxNvec];  // 0 => vector skipped
+  VT filt[kNumOri][kSPbSize * kSPbSize];
+  const VT* evecs;         // [K][Np] padded
+  const VT* mnmx;          // [K][2] min, max over the image
+  VT inv_sqrt_lam[kMaxNvec];  // 0 => vector skipped
   int K, W, H, Wp;
   size_t Np;
-  float* spb;              // [8][N]
+  VT* spb;                 // [8][N]
 };
 
-__global__ __launch_bounds__(256) void minmax_kernel(const float* __restrict__ v, int W, int H, int Wp,
-                                                     float* __restrict__ partial /* [2][G] */) {
-  __shared__ float smn[256], smx[256];
-  float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+template <typename VT>
+__global__ __launch_bounds__(256) void minmax_kernel(const VT* __restrict__ v, int W, int H, int Wp,
+                                                     VT* __restrict__ partial /* [2][G] */) {
+  __shared__ VT smn[256], smx[256];
+  VT mn = VT(__builtin_huge_val()), mx = -VT(__builtin_huge_val());
   const size_t N = size_t(W) * H;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += size_t(gridDim.x) * blockDim.x) {
     const int y = int(i / W), x = int(i % W);
-    const float a = v[size_t(y + kICRadius) * Wp + x + kICRadius];
-    mn = fminf(mn, a);
-    mx = fmaxf(mx, a);
+    const VT a = v[size_t(y + kICRadius) * Wp + x + kICRadius];
+    mn = a < mn ? a : mn;
+    mx = a > mx ? a : mx;
   }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (int(threadIdx.x) < s) {
-      smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + s]);
-      smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + s]);
+      smn[threadIdx.x] = smn[threadIdx.x + s] < smn[threadIdx.x] ? smn[threadIdx.x + s] : smn[threadIdx.x];
+      smx[threadIdx.x] = smx[threadIdx.x + s] > smx[threadIdx.x] ? smx[threadIdx.x + s] : smx[threadIdx.x];
     }
     __syncthreads();
   }
@@ -51,21 +53,22 @@ __global__ __launch_bounds__(256) void minmax_kernel(const float* __restrict__ v
   }
 }
 
-__global__ __launch_bounds__(256) void minmax_final_kernel(const float* __restrict__ partial, int G,
-                                                           float* __restrict__ out) {
-  __shared__ float smn[256], smx[256];
-  float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+template <typename VT>
+__global__ __launch_bounds__(256) void minmax_final_kernel(const VT* __restrict__ partial, int G,
+                                                           VT* __restrict__ out) {
+  __shared__ VT smn[256], smx[256];
+  VT mn = VT(__builtin_huge_val()), mx = -VT(__builtin_huge_val());
   for (int i = threadIdx.x; i < G; i += 256) {
-    mn = fminf(mn, partial[i]);
-    mx = fmaxf(mx, partial[G + i]);
+    mn = partial[i] < mn ? partial[i] : mn;
+    mx = partial[G + i] > mx ? partial[G + i] : mx;
   }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (int(threadIdx.x) < s) {
-      smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + s]);
-      smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + s]);
+      smn[threadIdx.x] = smn[threadIdx.x + s] < smn[threadIdx.x] ? smn[threadIdx.x + s] : smn[threadIdx.x];
+      smx[threadIdx.x] = smx[threadIdx.x + s] > smx[threadIdx.x] ? smx[threadIdx.x + s] : smx[threadIdx.x];
     }
     __syncthreads();
   }
@@ -83,39 +86,40 @@ __device__ __forceinline__ int d_refl101(int i, int n) {
   return i < n ? i : p - i;
 }
 
-__global__ __launch_bounds__(kSpbT * kSpbT) void spb_kernel(SpbParams P) {
-  __shared__ float tile[kSpbTile * kSpbTile];
+template <typename VT>
+__global__ __launch_bounds__(kSpbT * kSpbT) void spb_kernel(SpbParams<VT> P) {
+  __shared__ VT tile[kSpbTile * kSpbTile];
   const int tid = threadIdx.y * kSpbT + threadIdx.x;
   const int x0 = blockIdx.x * kSpbT - kSPbRadius, y0 = blockIdx.y * kSpbT - kSPbRadius;
   const int x = blockIdx.x * kSpbT + threadIdx.x, y = blockIdx.y * kSpbT + threadIdx.y;
-  float acc[kNumOri];
+  VT acc[kNumOri];
 #pragma unroll
-  for (int o = 0; o < kNumOri; o++) acc[o] = 0.f;
+  for (int o = 0; o < kNumOri; o++) acc[o] = VT(0);
   for (int i = 1; i < P.K; i++) {
-    const float isl = P.inv_sqrt_lam[i];
-    if (isl == 0.f) continue;  // uniform across the grid
-    const float mn = P.mnmx[2 * i], mx = P.mnmx[2 * i + 1];
+    const VT isl = P.inv_sqrt_lam[i];
+    if (isl == VT(0)) continue;  // uniform across the grid
+    const VT mn = P.mnmx[2 * i], mx = P.mnmx[2 * i + 1];
     const bool norm = mx > mn;
-    const float inv = norm ? 1.0f / (mx - mn) : 1.0f;
-    const float* v = P.evecs + size_t(i) * P.Np;
+    const VT inv = norm ? VT(1) / (mx - mn) : VT(1);
+    const VT* v = P.evecs + size_t(i) * P.Np;
     __syncthreads();
     for (int t = tid; t < kSpbTile * kSpbTile; t += kSpbT * kSpbT) {
       const int ty = t / kSpbTile, tx = t - ty * kSpbTile;
       const int gx = d_refl101(x0 + tx, P.W), gy = d_refl101(y0 + ty, P.H);
-      float a = v[size_t(gy + kICRadius) * P.Wp + gx + kICRadius];
+      VT a = v[size_t(gy + kICRadius) * P.Wp + gx + kICRadius];
       if (norm) a = (a - mn) * inv;
       tile[t] = a * isl;
     }
     __syncthreads();
-    const float* c = tile + threadIdx.y * kSpbTile + threadIdx.x;
+    const VT* c = tile + threadIdx.y * kSpbTile + threadIdx.x;
 #pragma unroll
     for (int o = 0; o < kNumOri; o++) {
-      float r = 0.f;
+      VT r = VT(0);
 #pragma unroll
       for (int a = 0; a < kSPbSize; a++)
 #pragma unroll
-        for (int b = 0; b < kSPbSize; b++) r = __builtin_fmaf(P.filt[o][a * kSPbSize + b], c[a * kSpbTile + b], r);
-      acc[o] += fabsf(r);
+        for (int b = 0; b < kSPbSize; b++) r = fma_t(P.filt[o][a * kSPbSize + b], c[a * kSpbTile + b], r);
+      acc[o] += r < VT(0) ? -r : r;
     }
   }
   if (x < P.W && y < P.H) {
@@ -125,35 +129,45 @@ __global__ __launch_bounds__(kSpbT * kSpbT) void spb_kernel(SpbParams P) {
   }
 }
 
-void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb) {
+template <typename VT>
+static void spb_t(Context& c, int W, int H, int Wp, size_t Np, const VT* d_evecs, const double* evals, int K,
+                  VT* d_spb) {
   if (K > kMaxNvec) throw ArgError("too many eigenvectors");
   const int G = 256;
-  float* part = c.ws<float>("spb.part", 2 * G);
-  float* mnmx = c.ws<float>("spb.mnmx", 2 * K);
+  VT* part = c.ws<VT>(sizeof(VT) == 8 ? "spb.part64" : "spb.part", 2 * G);
+  VT* mnmx = c.ws<VT>(sizeof(VT) == 8 ? "spb.mnmx64" : "spb.mnmx", 2 * K);
   for (int i = 1; i < K; i++) {
-    hipLaunchKernelGGL(minmax_kernel, dim3(G), dim3(256), 0, c.stream, d_evecs + size_t(i) * st.Np, st.W, st.H, st.Wp,
-                       part);
+    hipLaunchKernelGGL(minmax_kernel<VT>, dim3(G), dim3(256), 0, c.stream, d_evecs + size_t(i) * Np, W, H, Wp, part);
     GPB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, c.stream, part, G, mnmx + 2 * i);
+    hipLaunchKernelGGL(minmax_final_kernel<VT>, dim3(1), dim3(256), 0, c.stream, part, G, mnmx + 2 * i);
     GPB_LAUNCH_CHECK();
   }
-  SpbParams P{};
+  SpbParams<VT> P{};
   const std::vector<double>& f = spb_filters();
   for (int o = 0; o < kNumOri; o++)
-    for (int t = 0; t < kSPbSize * kSPbSize; t++) P.filt[o][t] = float(f[size_t(o) * kSPbSize * kSPbSize + t]);
+    for (int t = 0; t < kSPbSize * kSPbSize; t++) P.filt[o][t] = VT(f[size_t(o) * kSPbSize * kSPbSize + t]);
   P.evecs = d_evecs;
   P.mnmx = mnmx;
   for (int i = 0; i < kMaxNvec; i++)
-    P.inv_sqrt_lam[i] = (i > 0 && i < K && evals[i] > 2.220446049250313e-16) ? float(1.0 / std::sqrt(evals[i])) : 0.f;
+    P.inv_sqrt_lam[i] = (i > 0 && i < K && evals[i] > 2.220446049250313e-16) ? VT(1.0 / std::sqrt(evals[i])) : VT(0);
   P.K = K;
-  P.W = st.W;
-  P.H = st.H;
-  P.Wp = st.Wp;
-  P.Np = st.Np;
+  P.W = W;
+  P.H = H;
+  P.Wp = Wp;
+  P.Np = Np;
   P.spb = d_spb;
-  KScope ks(c, kKSpb, double(st.W) * st.H * (4.0 * (K - 1) + 32));
-  hipLaunchKernelGGL(spb_kernel, dim3(div_up(st.W, kSpbT), div_up(st.H, kSpbT)), dim3(kSpbT, kSpbT), 0, c.stream, P);
+  KScope ks(c, kKSpb, double(W) * H * sizeof(VT) * (K - 1 + 8));
+  hipLaunchKernelGGL(spb_kernel<VT>, dim3(div_up(W, kSpbT), div_up(H, kSpbT)), dim3(kSpbT, kSpbT), 0, c.stream, P);
   GPB_LAUNCH_CHECK();
+}
+
+void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb) {
+  spb_t<float>(c, st.W, st.H, st.Wp, st.Np, d_evecs, evals, K, d_spb);
+}
+
+void launch_spb_f64(Context& c, const Stencil64& st, const double* d_evecs, const double* evals, int K,
+                    double* d_spb) {
+  spb_t<double>(c, st.W, st.H, st.Wp, st.Np, d_evecs, evals, K, d_spb);
 }
 
 // gPb_o = gcue_o + w_sPb * sPb_o; max masked by mPb_nmax > 0.05; sigmoid maps
@@ -203,6 +217,32 @@ void launch_gpb(Context& c, int W, int H, const float* d_gcue, const float* d_sp
   KScope ks(c, kKGpb, double(W) * H * 104);
   hipLaunchKernelGGL(gpb_kernel, dim3(div_up(W, kGT), div_up(H, kGT)), dim3(kGT, 8), 0, c.stream, d_gcue, d_spb, d_nmax,
                      W, H, float(kGPbWeights[kNumCues]), d_orient, d_gmax, d_gall);
+  GPB_LAUNCH_CHECK();
+}
+
+// The all-FP64 combine: orient[o] = gcue[o] + w_sPb * sPb[o]; gmax = max_o
+// orient where mPb_nmax > 0.05, else 0 (the calibration sigmoid and the
+// column-major transpose are fixed transforms and are left to gpb_kernel).
+__global__ __launch_bounds__(256) void gpb64_kernel(const double* __restrict__ gcue, const double* __restrict__ spb,
+                                                    const double* __restrict__ nmax, size_t N, double wspb,
+                                                    double* __restrict__ orient, double* __restrict__ gmax) {
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < N; p += size_t(gridDim.x) * blockDim.x) {
+    double mv = -200.0;
+    for (int o = 0; o < kNumOri; o++) {
+      const double g = fma(wspb, spb[o * N + p], gcue[o * N + p]);
+      if (orient) orient[o * N + p] = g;
+      mv = g > mv ? g : mv;
+    }
+    if (gmax) gmax[p] = nmax[p] > 0.05 ? mv : 0.0;
+  }
+}
+
+void launch_gpb_f64(Context& c, int W, int H, const double* d_gcue, const double* d_spb, const double* d_nmax,
+                    double* d_orient, double* d_gmax) {
+  const size_t N = size_t(W) * H;
+  KScope ks(c, kKGpb, double(N) * 8 * 25);
+  hipLaunchKernelGGL(gpb64_kernel, dim3(std::min<unsigned>(4096, div_up(N, 256))), dim3(256), 0, c.stream, d_gcue,
+                     d_spb, d_nmax, N, kGPbWeights[kNumCues], d_orient, d_gmax);
   GPB_LAUNCH_CHECK();
 }
 

@@ -70,6 +70,10 @@ _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_verbose", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_compute_f64", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p])
+_sig("gpb_eigs_f64", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p])
 _sig("gpb_cut_patches", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _i32p, _i32p])
 _sig("gpb_compute_patches", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_int, _i32p, _i32p, C.c_int,
                                       C.c_void_p, C.c_void_p])
@@ -320,6 +324,31 @@ class Context:
     def set_spmv_tile(self, rows: int):
         """SpMV rows per thread (2, 4, 8), or 0 to autotune per image shape."""
         self._chk(lib.gpb_set_spmv_tile(self.h, rows))
+
+    def compute_f64(self, packed: np.ndarray, codebook: np.ndarray, with_evecs: bool = True) -> dict:
+        """The all-FP64 GPU path: mpb_nmax, gpb_orient [8,H,W], gpb_max, evals, evecs [nvec,H,W] (float64)."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        K = self.nvec
+        out = dict(mpb_nmax=np.empty((H, W)), gpb_orient=np.empty((8, H, W)), gpb_max=np.empty((H, W)),
+                   evals=np.empty(K), evecs=np.empty((K, H, W)) if with_evecs else None)
+        self._chk(lib.gpb_compute_f64(self.h, W, H, packed.reshape(-1), np.ascontiguousarray(codebook, np.float32),
+                                      out["mpb_nmax"].ctypes.data, out["gpb_orient"].ctypes.data,
+                                      out["gpb_max"].ctypes.data, out["evals"].ctypes.data,
+                                      out["evecs"].ctypes.data if with_evecs else None))
+        return out
+
+    def eigs_f64(self, mpb_nmax: np.ndarray, with_vectors: bool = True):
+        m = np.ascontiguousarray(mpb_nmax, np.float64)
+        H, W = m.shape
+        ev = np.empty(self.nvec)
+        vec = np.empty((self.nvec, H, W)) if with_vectors else None
+        self._chk(lib.gpb_eigs_f64(self.h, W, H, m.ctypes.data, ev.ctypes.data,
+                                   vec.ctypes.data if vec is not None else None))
+        return ev, vec
+
+    def set_verbose(self, level: int):
+        self._chk(lib.gpb_set_verbose(self.h, level))
 
     def set_spmv_fp64(self, on: bool):
         """SpMV arithmetic in FP64 (default) or FP32; HBM operands are FP32 either way."""

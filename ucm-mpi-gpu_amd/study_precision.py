@@ -1,0 +1,127 @@
+"""Mixed-precision study of the gPb path on one GPU.
+
+Workload: a seeded three-class Voronoi image (default 4096x4096, 1600 regions,
+sigma 8, seed 3).  Variants, all on the GPU:
+
+  prod      the production path: FP32 storage (cues, W, Lanczos vectors),
+            FP64 SpMV arithmetic, FP64 reorthogonalisation / tridiagonal /
+            Ritz;
+  spmv32    the same eigensolve with FP32 SpMV arithmetic;
+  f64       every stage in FP64 (cues, nonmax, W, Lanczos vectors and
+            SpMV, sPb, gPb) -- the reference;
+  f64solve  the FP64 eigensolve on the production mPb (isolates the solver).
+
+Per output it reports the error of prod against f64: mPb_nmax max abs error
+(and pixels whose nonmax survival flips), eigenvalue relative error of pairs
+1..8, the largest principal angle between the eigenvector subspaces (D inner
+product), and gPb max abs error (oriented, and the masked maximum where both
+masks agree).  The eigensolves run at the residual target given (the bench's
+1e-3) and at 1e-6, where the solver's convergence error is below the
+precision effects.  Prints one JSON object.
+
+    python ucm-mpi-gpu_amd/study_precision.py [--size 4096] [--regions 1600] [--seed 3]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import scipy.linalg as sl
+
+PKG = Path(__file__).resolve().parent
+
+
+def _binding():
+    name = "gpb_amd_binding"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, PKG / "gpb.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def subspace_angle(Va: np.ndarray, Vb: np.ndarray, D: np.ndarray) -> float:
+    """Largest principal angle between span(Va) and span(Vb) in the D inner product."""
+    s = np.sqrt(D.reshape(-1).astype(np.float64))
+    A = (Va.reshape(Va.shape[0], -1).astype(np.float64) * s).T
+    B = (Vb.reshape(Vb.shape[0], -1).astype(np.float64) * s).T
+    return float(np.max(sl.subspace_angles(A, B)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--regions", type=int, default=1600)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--tols", default="1e-3,1e-6", help="residual targets of the production solves")
+    ap.add_argument("--ref-tol", type=float, default=1e-6, help="residual target of the FP64 reference solves")
+    args = ap.parse_args()
+    tols = sorted({float(t) for t in args.tols.split(",")}, reverse=True)
+    gpb = _binding()
+    S = args.size
+    t0 = time.perf_counter()
+    rgb = gpb.synth_voronoi(S, S, args.regions, 8.0, args.seed, classes=3)
+    packed = gpb.pack_rgb(rgb)
+    del rgb
+    cb = gpb.make_codebook(packed, seed=args.seed, iters=10)
+    print(f"image and codebook ready ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx = gpb.Context(0)
+    ctx.set_verbose(1)  # progress on stderr during the long FP64 solves
+    res = dict(workload=f"{S}x{S} seeded three-class Voronoi ({args.regions} regions, sigma 8, seed {args.seed}), "
+                        f"9 eigenvectors")
+    # the all-FP64 reference (tight residual)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=args.ref_tol, max_matvecs=40000)
+    t = time.perf_counter()
+    f64 = ctx.compute_f64(packed, cb)
+    res["f64_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+    res["f64_matvecs"] = ctx.stats()["matvecs"]
+    print(f"f64 path done ({res['f64_ms']} ms)", file=sys.stderr, flush=True)
+    _, D = ctx.build_w(f64["mpb_nmax"].astype(np.float32))
+    res["ref_tol"] = args.ref_tol
+    for tol in tols:
+        key = f"tol_{tol:g}"
+        ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol, max_matvecs=40000)
+        ctx.set_spmv_fp64(True)
+        t = time.perf_counter()
+        out = ctx.compute(packed, cb)
+        prod_ms = (time.perf_counter() - t) * 1e3
+        ev, vec = ctx.eigs(out["mpb_nmax"])
+        matvecs = ctx.stats()["matvecs"]
+        ctx.set_spmv_fp64(False)
+        ev_b, vec_b = ctx.eigs(out["mpb_nmax"])
+        ctx.set_spmv_fp64(True)
+        print(f"{key}: production and FP32-SpMV solves done", file=sys.stderr, flush=True)
+        ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=args.ref_tol, max_matvecs=40000)
+        ev_s, vec_s = ctx.eigs_f64(out["mpb_nmax"].astype(np.float64))
+        print(f"{key}: FP64 solve on the production mPb done", file=sys.stderr, flush=True)
+        same = (out["mpb_nmax"] > 0.05) == (f64["mpb_nmax"] > 0.05)
+        r = dict(prod_ms=round(prod_ms, 1), prod_matvecs=matvecs)
+        r["mpb_nmax_max_abs_err"] = float(np.abs(out["mpb_nmax"] - f64["mpb_nmax"]).max())
+        flip = (out["mpb_nmax"] > 0) != (f64["mpb_nmax"] > 0)
+        r["mpb_nmax_flips"] = int(np.count_nonzero(flip))
+        r["mpb_nmax_max_abs_err_no_flip"] = float(np.abs(out["mpb_nmax"] - f64["mpb_nmax"])[~flip].max())
+        r["eig_rel_err_end_to_end"] = float(np.max(np.abs(ev[1:] - f64["evals"][1:]) / f64["evals"][1:]))
+        r["subspace_angle_end_to_end"] = subspace_angle(vec[1:], f64["evecs"][1:], D)
+        r["eig_rel_err_solver"] = float(np.max(np.abs(ev[1:] - ev_s[1:]) / ev_s[1:]))
+        r["subspace_angle_solver"] = subspace_angle(vec[1:], vec_s[1:], D)
+        r["eig_rel_err_solver_spmv32"] = float(np.max(np.abs(ev_b[1:] - ev_s[1:]) / ev_s[1:]))
+        r["subspace_angle_solver_spmv32"] = subspace_angle(vec_b[1:], vec_s[1:], D)
+        r["gpb_orient_max_abs_err"] = float(np.abs(out["gpb_orient"] - f64["gpb_orient"]).max())
+        r["gpb_max_max_abs_err"] = float(np.abs(out["gpb_max"] - f64["gpb_max"])[same].max())
+        r["gpb_mask_flips"] = int(np.count_nonzero(~same))
+        res[key] = r
+        print(f"{key}: {json.dumps(r)}", file=sys.stderr, flush=True)
+    res["evals_f64"] = [float(x) for x in f64["evals"]]
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
