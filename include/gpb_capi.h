@@ -43,6 +43,8 @@ typedef struct gpb_stats {
   double evals[32];     /* generalised eigenvalues, ascending (first nvec valid) */
   int fallback;         /* 1 if a warm start was rejected and plain Lanczos ran */
   int reorth_second_passes; /* Gram-Schmidt steps that needed a second pass */
+  int warm_mode;        /* eigensolve start: 0 plain, 1 learned block, 2 random block */
+  double warm_ms;       /* warm-start network inference time (mode 1) */
 } gpb_stats;
 
 /* Context lifetime.  Replaces the per-call device selection and
@@ -120,6 +122,59 @@ int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol,
  * in the reference (its solvers are single-vector, lanczos.cu:192-282,
  * SymEigsSolver.h:200-280). */
 int gpb_set_block_size(gpb_ctx* ctx, int block);
+
+/* ---- learned warm start (no counterpart in the reference) ------------------
+ * The reference's solvers start from a fixed or random vector
+ * (spectralPb.cpp:221-244, damascene/src/noReorthog/lanczos.cu:393-752).
+ * Here a convolutional encoder-decoder (12 3x3 convolutions, 32..128
+ * channels, BF16 on MFMA with FP32 accumulation) maps the cue features of
+ * 4x4 pixel cells to 8 vectors per cell; upsampled, they start block Lanczos
+ * (8 columns plus the locked D^1/2 1).  Plain Lanczos is the fallback and the
+ * judge: it runs whenever the block solve fails or a returned pair's measured
+ * residual exceeds the target. */
+
+/* Eigensolve start for later gpb_compute calls: mode 0 = plain (default),
+ * 1 = learned block (needs initialised parameters), 2 = block Lanczos from a
+ * random block (the ablation without the network).  basis / keep: block
+ * solver basis and restart sizes (<= 0: 64 / 24).  Needs nvec = 9. */
+int gpb_set_warm_start(gpb_ctx* ctx, int mode, int basis, int keep);
+
+/* Parameter vector: per layer l = 0..11, W[9][cout][cin] (tap = 3 dy + dx)
+ * then bias[cout], FP32; offsets from gpb_warm_layer_info. */
+size_t gpb_warm_param_count(void);
+int gpb_warm_layer_info(int layer, int* cin, int* cout, int* relu, size_t* w_offset, size_t* b_offset);
+/* Seeded He initialisation; resets the optimiser state. */
+int gpb_warm_init(gpb_ctx* ctx, uint64_t seed);
+int gpb_warm_set_params(gpb_ctx* ctx, const float* params);
+int gpb_warm_get_params(gpb_ctx* ctx, float* params);
+
+/* Training set on the device: capacity images of width x height.  _add runs
+ * the path to the eigenvectors (plain Lanczos with the context's eigensolver
+ * options, nvec >= 9) and stores the image's cue features and the D-weighted
+ * cell sums of eigenvectors 1..8 (evals, optional: its eigenvalues). */
+int gpb_warm_dataset_reset(gpb_ctx* ctx, int width, int height, int capacity);
+int gpb_warm_dataset_add(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                         double* evals);
+int gpb_warm_dataset_size(gpb_ctx* ctx, int* n);
+/* The network's cue features of an image: feat [32][ceil(H/4)][ceil(W/4)],
+ * channel 8 c + o = the mean over the 4x4 cell of sum over scales s of
+ * w_mpb[3 c + s] * chi2[3 c + s][o] (c: L, a, b, texton; o: orientation). */
+int gpb_warm_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                      float* feat);
+/* Adam (lr, betas 0.9 / 0.999, gradient norm clipped at 1) over shuffled
+ * mini-batches of training-set entries [first, first + count); epoch_loss
+ * (optional, epochs entries): mean training loss per epoch. */
+int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed,
+                   float* epoch_loss);
+/* Mean loss (mean squared sine of the principal angles, D inner product)
+ * over training-set entries [first, first + count). */
+int gpb_warm_eval_loss(gpb_ctx* ctx, int first, int count, double* loss);
+/* Loss and gradient for host data at cell resolution (hq, wq multiples of 8):
+ * x [nimg][hq][wq][34] (32 cue features + 2 coordinates), targets
+ * [nimg][hq][wq][8], dweights [nimg][hq][wq]; grad (optional, param_count),
+ * y (optional, [nimg][hq][wq][8] network output). */
+int gpb_warm_loss_grad(gpb_ctx* ctx, int nimg, int hq, int wq, const float* x, const float* targets,
+                       const float* dweights, double* loss, float* grad, float* y);
 
 /* SpMV tile for later solves on ctx: rows = 2, 4 or 8 rows per thread
  * (tiles of 64 x 4*rows pixels), or 0 (default) to time the three variants on

@@ -93,7 +93,8 @@ struct LanczosStats {
 // Per-kernel-class timing with HIP events on the context stream (enabled by
 // gpb_set_profiling); durations are folded in when the stream is synchronised.
 enum KernelClass {
-  kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKOther, kNumKClasses
+  kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKWarm, kKOther,
+  kNumKClasses
 };
 struct KernelTimes {
   bool on = false;
@@ -132,6 +133,7 @@ struct KernelTimes {
 
 struct Context;
 void upload_tables(Context& c);
+struct WarmState;  // learned warm start (k_warm.hip)
 
 struct Context {
   int device = 0;
@@ -165,6 +167,12 @@ struct Context {
   // per-patch solver statistics and eigenvalues of the last patch call
   std::vector<LanczosStats> patch_stats;
   std::vector<double> patch_evals;  // a hipGraph capture is open on the stream (no event timing)
+  // learned warm start: network, optimiser and training set; the mode used
+  // by the eigensolve (0 plain, 1 learned block, 2 block from a random start)
+  std::shared_ptr<WarmState> warm;
+  int warm_mode = 0;
+  int warm_basis = 64, warm_keep = 24;  // block-solver basis and restart size
+  double warm_ms = 0;  // network inference time of the last warm-started solve
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {

@@ -19,7 +19,8 @@ void launch_texton(Context& c, const uint32_t* d_rgb, int W, int H, const float*
 // stage (b)
 void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d_q /* 3N */, int* d_nb /* 3 */);
 void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, float* d_mpb_all,
-                       float* d_gcue, float* d_cues /* optional [12][8][N] */);
+                       float* d_gcue, float* d_cues /* optional [12][8][N] */,
+                       float* d_feat = nullptr /* optional [32][ceil(H/4)][ceil(W/4)] cell-mean cue features */);
 void launch_nonmax(Context& c, const float* d_mpb_all, int W, int H, float* d_nmax);
 
 // stage (c): radius-5 IC stencil in a zero-padded layout (5-pixel halo)
@@ -82,6 +83,10 @@ struct BlockOptions {
 void block_lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, const BlockOptions& bo, double* evals,
                          float* d_evecs, LanczosStats& stats);
 void debug_block_spmv(Context& c, const Stencil& st, int B, const float* X, float* Y);
+// Measured relative generalised residuals of K <= 9 returned pairs; returns
+// the largest (resid, optional, receives all K).
+double measured_max_residual(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K,
+                             double* resid = nullptr);
 // helpers shared by the solvers
 void launch_reduce(Context& c, const double* partial, int G, int n, double* dst);
 void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np);

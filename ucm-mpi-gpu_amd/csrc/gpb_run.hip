@@ -4,11 +4,13 @@
 // Reference call chain: computeGPb (damascene/src/damascene.cu:249-661) on the
 // GPU side; GlobalPb -> multiscalePb -> spectralPb (src/globalPb.cpp:15-126,
 // src/multiscalePb.cpp:16-283, src/spectralPb.cpp:33-343) on the CPU side.
+#include <chrono>
 #include <cstring>
 
 #include "gpb_capi.h"
 #include "gpb_pipeline.hpp"
 #include "k_batch.hpp"
+#include "k_warm.hpp"
 
 namespace gpb {
 
@@ -39,6 +41,7 @@ struct PipelineBuffers {
   float* nmax;
   float* evecs;
   float* spb;
+  float* feat;  // cue features at cell resolution (warm start only)
 };
 
 static PipelineBuffers pipeline_buffers(Context& c, int W, int H, int K, bool want_cues, size_t Np) {
@@ -51,10 +54,18 @@ static PipelineBuffers pipeline_buffers(Context& c, int W, int H, int K, bool wa
   b.nmax = c.ws<float>("run.nmax", N);
   b.evecs = c.ws<float>("run.evecs", size_t(K) * Np);
   b.spb = c.ws<float>("run.spb", 8 * N);
+  b.feat = nullptr;
+  if (c.warm_mode == 1) {
+    int hq, wq, h0, w0;
+    warm_cell_grid(W, H, &hq, &wq, &h0, &w0);
+    b.feat = c.ws<float>("run.feat", size_t(kWarmFeat) * hq * wq);
+  }
   return b;
 }
 
 void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* evals, float* d_evecs);
+void solve_eigs_warm(Context& c, const Stencil& st, const EigOptions& eo, const float* d_feat, double* evals,
+                     float* d_evecs);
 
 // Runs the whole path; outputs are device pointers (any may be null).
 static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo, bool want_cues, float* d_gmax,
@@ -66,14 +77,17 @@ static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo
   GPB_HIP(hipEventRecord(se.ev[0], c.stream));
   launch_texton(c, im.rgb, W, H, im.codebook, b.text);
   GPB_HIP(hipEventRecord(se.ev[1], c.stream));
-  launch_local_cues(c, im.rgb, b.text, W, H, b.mpb_all, b.gcue, b.cues);
+  launch_local_cues(c, im.rgb, b.text, W, H, b.mpb_all, b.gcue, b.cues, b.feat);
   GPB_HIP(hipEventRecord(se.ev[2], c.stream));
   launch_nonmax(c, b.mpb_all, W, H, b.nmax);
   GPB_HIP(hipEventRecord(se.ev[3], c.stream));
   Stencil st;
   build_stencil(c, b.nmax, W, H, st);
   GPB_HIP(hipEventRecord(se.ev[4], c.stream));
-  solve_eigs(c, st, eo, evals, b.evecs);
+  if (c.warm_mode)
+    solve_eigs_warm(c, st, eo, b.feat, evals, b.evecs);
+  else
+    solve_eigs(c, st, eo, evals, b.evecs);
   GPB_HIP(hipEventRecord(se.ev[5], c.stream));
   launch_spb(c, st, b.evecs, evals, eo.K, b.spb);
   launch_gpb(c, W, H, b.gcue, b.spb, b.nmax, d_orient, d_gmax, d_gall);
@@ -106,6 +120,60 @@ void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* eva
     const int spent = c.last_lanczos.iterations;
     lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
     c.last_lanczos.fallback = 2;
+    c.last_lanczos.iterations += spent;
+  }
+}
+
+// Warm-started eigensolve: block Lanczos with 8 columns (plus the locked
+// D^1/2 1, 9 vectors) from the network's predicted block (mode 1) or from a
+// random block (mode 2).  Plain single-vector Lanczos from its own start is
+// the fallback whenever the block solve breaks down or stops short, or the
+// measured residual ||(D - W)v - lambda D v|| / ||D v|| of a returned pair
+// exceeds the target (stats.fallback = 1; the operator applications of both
+// solves are counted).
+void solve_eigs_warm(Context& c, const Stencil& st, const EigOptions& eo, const float* d_feat, double* evals,
+                     float* d_evecs) {
+  if (eo.K != 9) throw ArgError("the warm start computes 9 eigenpairs (nvec = 9)");
+  BlockOptions bo;
+  bo.block = kWarmOut;
+  EigOptions be = eo;
+  be.m = c.warm_basis;
+  be.keep = c.warm_keep;
+  be.tol = 0.7 * eo.tol;  // margin for the measured check below
+  c.warm_ms = 0;
+  if (c.warm_mode == 1) {
+    if (!d_feat) throw std::logic_error("warm start: cue features missing");
+    float* X0 = c.ws<float>("warm.block", size_t(kWarmOut) * st.Np);
+    hipEvent_t e0, e1;
+    GPB_HIP(hipEventCreate(&e0));
+    GPB_HIP(hipEventCreate(&e1));
+    GPB_HIP(hipEventRecord(e0, c.stream));
+    warm_predict_block(c, d_feat, st.W, st.H, X0, st.Np);
+    GPB_HIP(hipEventRecord(e1, c.stream));
+    GPB_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.warm_ms = ms;
+    bo.init_block = X0;
+  }
+  bool ok = false;
+  int spent = 0;
+  try {
+    block_lanczos_solve(c, st, be, bo, evals, d_evecs, c.last_lanczos);
+    spent = c.last_lanczos.iterations;
+    const double r = measured_max_residual(c, st, d_evecs, evals, eo.K);
+    c.last_lanczos.max_residual = r;
+    ok = r <= eo.tol;
+  } catch (const BlockBreakdown&) {
+    spent = c.last_lanczos.iterations;
+  } catch (const NoConvergence&) {
+    spent = c.last_lanczos.iterations;
+  }
+  if (!ok) {
+    lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
+    c.last_lanczos.fallback = 1;
     c.last_lanczos.iterations += spent;
   }
 }
@@ -314,6 +382,147 @@ int gpb_set_block_size(gpb_ctx* ctx, int block) {
   });
 }
 
+int gpb_set_warm_start(gpb_ctx* ctx, int mode, int basis, int keep) {
+  return guarded2(ctx, [&] {
+    if (mode < 0 || mode > 2) throw ArgError("warm-start mode must be 0, 1 or 2");
+    if (mode == 1 && !warm_ready(ctx->c)) throw ArgError("warm-start network not initialised");
+    const int b = basis > 0 ? basis : 64, k = keep > 0 ? keep : 24;
+    if (b < 9 + 2 * kWarmOut || k < 10 || k > b - kWarmOut) throw ArgError("bad warm-start basis / keep");
+    ctx->c.warm_mode = mode;
+    ctx->c.warm_basis = b;
+    ctx->c.warm_keep = k;
+  });
+}
+
+size_t gpb_warm_param_count(void) { return warm_param_count(); }
+
+int gpb_warm_layer_info(int layer, int* cin, int* cout, int* relu, size_t* w_offset, size_t* b_offset) {
+  try {
+    if (!cin || !cout || !relu || !w_offset || !b_offset) return GPB_ERR_ARG;
+    warm_layer_info(layer, cin, cout, relu, w_offset, b_offset);
+    return GPB_OK;
+  } catch (...) {
+    return GPB_ERR_ARG;
+  }
+}
+
+int gpb_warm_init(gpb_ctx* ctx, uint64_t seed) {
+  return guarded2(ctx, [&] { warm_init(ctx->c, seed); });
+}
+
+int gpb_warm_set_params(gpb_ctx* ctx, const float* params) {
+  return guarded2(ctx, [&] {
+    if (!params) throw ArgError("null pointer");
+    warm_set_params(ctx->c, params);
+  });
+}
+
+int gpb_warm_get_params(gpb_ctx* ctx, float* params) {
+  return guarded2(ctx, [&] {
+    if (!params) throw ArgError("null pointer");
+    warm_get_params(ctx->c, params);
+  });
+}
+
+int gpb_warm_dataset_reset(gpb_ctx* ctx, int width, int height, int capacity) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    warm_dataset_reset(ctx->c, width, height, capacity);
+  });
+}
+
+int gpb_warm_dataset_add(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                         double* evals) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!packed || !codebook) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const int W = width, H = height;
+    const size_t N = size_t(W) * H;
+    const EigOptions eo = ctx_eig_options(c);
+    if (eo.K < 1 + kWarmOut) throw ArgError("training targets need nvec >= 9");
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
+    int hq, wq, h0, w0;
+    warm_cell_grid(W, H, &hq, &wq, &h0, &w0);
+    uint8_t* text = c.ws<uint8_t>("run.text", N);
+    float* mpb_all = c.ws<float>("run.mpb_all", 8 * N);
+    float* gcue = c.ws<float>("run.gcue", 8 * N);
+    float* nmax = c.ws<float>("run.nmax", N);
+    float* feat = c.ws<float>("run.feat", size_t(kWarmFeat) * hq * wq);
+    launch_texton(c, d_rgb, W, H, d_cb, text);
+    launch_local_cues(c, d_rgb, text, W, H, mpb_all, gcue, nullptr, feat);
+    launch_nonmax(c, mpb_all, W, H, nmax);
+    Stencil st;
+    build_stencil(c, nmax, W, H, st);
+    float* d_ev = c.ws<float>("run.evecs", size_t(eo.K) * st.Np);
+    double ev[kMaxNvec];
+    lanczos_solve(c, st, eo, ev, d_ev, c.last_lanczos);
+    warm_dataset_add(c, feat, st, d_ev);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    c.kt.fold();
+    for (int i = 0; i < eo.K; i++) {
+      c.last_evals[i] = ev[i];
+      if (evals) evals[i] = ev[i];
+    }
+  });
+}
+
+int gpb_warm_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                      float* feat) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (!packed || !codebook || !feat) throw ArgError("null pointer");
+    Context& c = ctx->c;
+    const int W = width, H = height;
+    const size_t N = size_t(W) * H;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
+    int hq, wq, h0, w0;
+    warm_cell_grid(W, H, &hq, &wq, &h0, &w0);
+    uint8_t* text = c.ws<uint8_t>("run.text", N);
+    float* mpb_all = c.ws<float>("run.mpb_all", 8 * N);
+    float* gcue = c.ws<float>("run.gcue", 8 * N);
+    float* d_feat = c.ws<float>("run.feat", size_t(kWarmFeat) * hq * wq);
+    launch_texton(c, d_rgb, W, H, d_cb, text);
+    launch_local_cues(c, d_rgb, text, W, H, mpb_all, gcue, nullptr, d_feat);
+    GPB_HIP(hipMemcpyAsync(feat, d_feat, sizeof(float) * kWarmFeat * hq * wq, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    c.kt.fold();
+  });
+}
+
+int gpb_warm_dataset_size(gpb_ctx* ctx, int* n) {
+  return guarded2(ctx, [&] {
+    if (!n) throw ArgError("null pointer");
+    *n = warm_dataset_size(ctx->c);
+  });
+}
+
+int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed,
+                   float* epoch_loss) {
+  return guarded2(ctx, [&] { warm_train(ctx->c, first, count, epochs, batch, lr, seed, epoch_loss); });
+}
+
+int gpb_warm_eval_loss(gpb_ctx* ctx, int first, int count, double* loss) {
+  return guarded2(ctx, [&] {
+    if (!loss) throw ArgError("null pointer");
+    *loss = warm_eval_loss(ctx->c, first, count);
+  });
+}
+
+int gpb_warm_loss_grad(gpb_ctx* ctx, int nimg, int hq, int wq, const float* x, const float* targets,
+                       const float* dweights, double* loss, float* grad, float* y) {
+  return guarded2(ctx, [&] {
+    if (!x || !targets || !dweights || !loss) throw ArgError("null pointer");
+    *loss = warm_loss_grad_host(ctx->c, nimg, hq, wq, x, targets, dweights, grad, y);
+  });
+}
+
 int gpb_set_profiling(gpb_ctx* ctx, int on) {
   return guarded2(ctx, [&] {
     GPB_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -347,6 +556,8 @@ int gpb_get_stats(gpb_ctx* ctx, gpb_stats* out) {
     out->max_residual = ctx->c.last_lanczos.max_residual;
     out->fallback = ctx->c.last_lanczos.fallback;
     out->reorth_second_passes = ctx->c.last_lanczos.second_passes;
+    out->warm_mode = ctx->c.warm_mode;
+    out->warm_ms = ctx->c.warm_ms;
     for (int i = 0; i < kMaxNvec; i++) out->evals[i] = ctx->c.last_evals[i];
   });
 }
@@ -676,16 +887,27 @@ int gpb_compute_f64(gpb_ctx* ctx, int width, int height, const uint32_t* packed,
     double* gcue = c.ws<double>("f64.gcue", 8 * N);
     double* nmax = c.ws<double>("f64.nmax", N);
     StageEvents se;
+    // with verbose on, each stage ends with a synchronised progress line
+    const auto t_start = std::chrono::steady_clock::now();
+    auto note = [&](const char* what) {
+      if (!c.verbose) return;
+      GPB_HIP(hipStreamSynchronize(c.stream));
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+      std::fprintf(stderr, "gpb: FP64 path %dx%d: %s done at %.1f s\n", W, H, what, s);
+    };
     GPB_HIP(hipEventRecord(se.ev[0], c.stream));
     launch_texton(c, d_rgb, W, H, d_cb, text);
     GPB_HIP(hipEventRecord(se.ev[1], c.stream));
+    note("texton");
     launch_local_cues_f64(c, d_rgb, text, W, H, mpb_all, gcue, nullptr);
     GPB_HIP(hipEventRecord(se.ev[2], c.stream));
+    note("local cues");
     launch_nonmax_f64(c, mpb_all, W, H, nmax);
     GPB_HIP(hipEventRecord(se.ev[3], c.stream));
     Stencil64 st;
     build_stencil_f64(c, nmax, W, H, st);
     GPB_HIP(hipEventRecord(se.ev[4], c.stream));
+    note("nonmax and W");
     double ev[kMaxNvec];
     double* d_ev = c.ws<double>("f64.evecs", size_t(eo.K) * st.Np);
     lanczos_solve_f64(c, st, eo, ev, d_ev, c.last_lanczos);

@@ -24,7 +24,6 @@ void symeig_desc(int n, std::vector<double>& a, std::vector<double>& w, std::vec
 constexpr int kBMax = 16;
 constexpr int kBTX = 32, kBTY = 8;
 constexpr int kBTileW = kBTX + 2 * kICRadius, kBTileH = kBTY + 2 * kICRadius;
-constexpr int kBDotChunk = 4096;
 constexpr int kBMaxBasis = 336;
 
 struct BSpmvArgs {
@@ -33,7 +32,10 @@ struct BSpmvArgs {
 };
 
 // Y_r = L X_r for r < B; X, Y columns are ldx / ldy apart (padded layout).
-template <int B>
+// u = s x is staged in LDS in FP32 (LDS bandwidth bounds this kernel); AT is
+// the accumulation type: with AT = double each product of an FP32 weight and
+// an FP32 difference is exact and the 80-term sums carry no FP32 rounding.
+template <int B, typename AT>
 __global__ __launch_bounds__(kBTX * kBTY) void bspmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                             const float* __restrict__ X, size_t ldx,
                                                             float* __restrict__ Y, size_t ldy, int W, int H, int Wp,
@@ -59,86 +61,217 @@ __global__ __launch_bounds__(kBTX * kBTY) void bspmv_kernel(const float* __restr
   if (xi >= W || yi >= H) return;
   const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
   const int c = (threadIdx.y + kICRadius) * kBTileW + threadIdx.x + kICRadius;
-  float up[B], acc[B];
+  float up[B];
+  AT acc[B];
 #pragma unroll
   for (int r = 0; r < B; r++) {
     up[r] = u[c * B + r];
-    acc[r] = 0.f;
+    acc[r] = AT(0);
   }
 #pragma unroll 4
   for (int j = 0; j < kStencilHalf; j++) {
-    const float w1 = wup[size_t(j) * Np + p], w2 = wup[size_t(j) * Np + p - A.o[j]];
+    const AT w1 = wup[size_t(j) * Np + p], w2 = wup[size_t(j) * Np + p - A.o[j]];
     const float* u1 = u + (c + A.lo[j]) * B;
     const float* u2 = u + (c - A.lo[j]) * B;
 #pragma unroll
     for (int r = 0; r < B; r++) {
-      acc[r] = __builtin_fmaf(w1, up[r] - u1[r], acc[r]);
-      acc[r] = __builtin_fmaf(w2, up[r] - u2[r], acc[r]);
+      acc[r] = fma_t(w1, AT(up[r] - u1[r]), acc[r]);
+      acc[r] = fma_t(w2, AT(up[r] - u2[r]), acc[r]);
     }
   }
-  const float sp = s[p];
+  const AT sp = s[p];
 #pragma unroll
-  for (int r = 0; r < B; r++) Y[size_t(r) * ldy + p] = sp * acc[r];
+  for (int r = 0; r < B; r++) Y[size_t(r) * ldy + p] = float(sp * acc[r]);
 }
 
-__device__ __forceinline__ double bwave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
+// Wave-level transpose-reduce of K (a power of two, <= 64) values: every
+// step exchanges half of the remaining values with the partner lane, so the
+// 64 lanes finish with lane l holding the wave total of value l >> (6 - log2 K)
+// after K - 1 + (6 - log2 K) shuffles instead of 6 K.
+template <int K, int S>
+struct TransposeReduce {
+  static __device__ __forceinline__ void run(double* a, int lane) {
+    constexpr int h = K / 2;
+    const bool up = (lane & S) != 0;
+#pragma unroll
+    for (int j = 0; j < h; j++) {
+      const double send = up ? a[j] : a[j + h];
+      const double keep = up ? a[j + h] : a[j];
+      a[j] = keep + __shfl_xor(send, S, 64);
+    }
+    TransposeReduce<h, S / 2>::run(a, lane);
+  }
+};
+template <int S>
+struct TransposeReduce<1, S> {
+  static __device__ __forceinline__ void run(double* a, int lane) {
+#pragma unroll
+    for (int s = S; s >= 1; s >>= 1) a[0] += __shfl_xor(a[0], s, 64);
+  }
+};
 
-// partial[(i * B + r)][chunk] = sum over the chunk of V_i . (D) W_r, i < n.
-template <int B>
-__global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V, size_t ld, const float* __restrict__ Wm,
-                                                    size_t ldw, const float* __restrict__ D, size_t Np,
+constexpr int pow2_ceil(int v) { return v <= 1 ? 1 : 2 * pow2_ceil((v + 1) / 2); }
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// Panel dots H[i][r] = sum_p V_i[p] (D[p]) W_r[p] for i < n, r < B.  Each
+// thread keeps the panel values of PX pixels in registers and streams the
+// basis past them in groups of NV vectors (each basis vector is read once per
+// launch, the panel once); group totals are transpose-reduced per wave and
+// accumulated per workgroup in LDS.  Workgroup g covers chunks g, g + G, ...
+// of 256 * PX pixels and writes partial[(i * B + r) * G + g].
+template <int B, int NV, int PX>
+__global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                    const float* __restrict__ Wm, size_t ldw,
+                                                    const float* __restrict__ D, size_t Np,
                                                     double* __restrict__ partial) {
-  __shared__ double sh[4][B];
-  const int i = blockIdx.y;
-  const size_t lo = size_t(blockIdx.x) * kBDotChunk;
-  const float* v = V + size_t(i) * ld;
-  double acc[B];
+  constexpr int K = pow2_ceil(NV * B);
+  static_assert(K <= 64, "group too wide");
+  constexpr int SH = 6 - ilog2(K);
+  __shared__ double red[4][K];
+  __shared__ double tot[kBMaxBasis * B];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int G = gridDim.x;
+  for (int t = tid; t < n * B; t += 256) tot[t] = 0.0;
+  __syncthreads();
+  const size_t nchunk = (Np + 256 * PX - 1) / (256 * PX);
+  for (size_t ch = blockIdx.x; ch < nchunk; ch += G) {
+    size_t pp[PX];
+    float w[PX][B];
+    double dd[PX];
 #pragma unroll
-  for (int r = 0; r < B; r++) acc[r] = 0;
-  for (int e = threadIdx.x; e < kBDotChunk; e += 256) {
-    const size_t p = lo + e;
-    if (p < Np) {
-      double x = v[p];
-      if (D) x *= double(D[p]);
+    for (int k = 0; k < PX; k++) {
+      const size_t p = ch * (256 * PX) + size_t(k) * 256 + tid;
+      const bool in = p < Np;
+      pp[k] = in ? p : Np - 1;
+      dd[k] = in ? (D ? double(D[pp[k]]) : 1.0) : 0.0;
 #pragma unroll
-      for (int r = 0; r < B; r++) acc[r] += x * double(Wm[size_t(r) * ldw + p]);
+      for (int r = 0; r < B; r++) w[k][r] = Wm[size_t(r) * ldw + pp[k]];
+    }
+    for (int i0 = 0; i0 < n; i0 += NV) {
+      double a[K];
+#pragma unroll
+      for (int t = 0; t < K; t++) a[t] = 0.0;
+      float x[PX][NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const size_t iv = size_t(min(i0 + v, n - 1));
+#pragma unroll
+        for (int k = 0; k < PX; k++) x[k][v] = V[iv * ld + pp[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < PX; k++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double xv = double(x[k][v]) * dd[k];
+#pragma unroll
+          for (int r = 0; r < B; r++) a[v * B + r] = fma(xv, double(w[k][r]), a[v * B + r]);
+        }
+      TransposeReduce<K, 32>::run(a, lane);
+      if ((lane & ((1 << SH) - 1)) == 0) red[wv][lane >> SH] = a[0];
+      __syncthreads();
+      if (tid < NV * B && i0 + tid / B < n) tot[i0 * B + tid] += red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      __syncthreads();
     }
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int r = 0; r < B; r++) {
-    const double t = bwave_sum(acc[r]);
-    if (lane == 0) sh[wv][r] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < B) {
-    const int r = threadIdx.x;
-    partial[(size_t(i) * B + r) * gridDim.x + blockIdx.x] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
-  }
+  for (int t = tid; t < n * B; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
 }
 
-// W_r <- W_r - sum_i H[i][r] V_i (FP64 accumulation), grid-stride.
-template <int B>
+// W_r <- W_r - sum_i H[i][r] V_i (FP64 accumulation); PX pixels per thread
+// share each coefficient read from LDS.
+template <int B, int PX>
 __global__ __launch_bounds__(256) void bupdate_kernel(const float* __restrict__ V, size_t ld, int n,
                                                       const double* __restrict__ Hm, float* __restrict__ Wm, size_t ldw,
                                                       size_t Np) {
   __shared__ double hs[kBMaxBasis * B];
   for (int t = threadIdx.x; t < n * B; t += blockDim.x) hs[t] = Hm[t];
   __syncthreads();
-  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
-    double acc[B];
+  const size_t step = size_t(gridDim.x) * 256 * PX;
+  for (size_t p0 = size_t(blockIdx.x) * 256 * PX + threadIdx.x; p0 < Np; p0 += step) {
+    size_t pp[PX];
+    double acc[PX][B];
 #pragma unroll
-    for (int r = 0; r < B; r++) acc[r] = Wm[size_t(r) * ldw + p];
+    for (int k = 0; k < PX; k++) {
+      const size_t p = p0 + size_t(k) * 256;
+      pp[k] = p < Np ? p : Np - 1;
+#pragma unroll
+      for (int r = 0; r < B; r++) acc[k][r] = Wm[size_t(r) * ldw + pp[k]];
+    }
     for (int i = 0; i < n; i++) {
-      const double v = V[size_t(i) * ld + p];
+      double h[B];
 #pragma unroll
-      for (int r = 0; r < B; r++) acc[r] -= hs[i * B + r] * v;
+      for (int r = 0; r < B; r++) h[r] = hs[i * B + r];
+#pragma unroll
+      for (int k = 0; k < PX; k++) {
+        const double v = V[size_t(i) * ld + pp[k]];
+#pragma unroll
+        for (int r = 0; r < B; r++) acc[k][r] = fma(-h[r], v, acc[k][r]);
+      }
     }
 #pragma unroll
-    for (int r = 0; r < B; r++) Wm[size_t(r) * ldw + p] = float(acc[r]);
+    for (int k = 0; k < PX; k++) {
+      if (p0 + size_t(k) * 256 < Np) {
+#pragma unroll
+        for (int r = 0; r < B; r++) Wm[size_t(r) * ldw + pp[k]] = float(acc[k][r]);
+      }
+    }
+  }
+}
+
+// partial[i][g] = sum over workgroup g's pixels of D[p] X_i[p]^2, i < n.
+__global__ __launch_bounds__(256) void bwnorm_kernel(const float* __restrict__ X, size_t ld, int n,
+                                                     const float* __restrict__ D, size_t Np,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.y;
+  const float* x = X + size_t(i) * ld;
+  double a = 0;
+  for (size_t p = size_t(blockIdx.x) * 256 + threadIdx.x; p < Np; p += size_t(gridDim.x) * 256) {
+    const double v = x[p];
+    a = fma(double(D[p]) * v, v, a);
+  }
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[size_t(i) * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// out_c = sum_i Y[i][c] V_i for c < nk (Ritz recombination, FP64
+// accumulation): Y (m x nk) sits in LDS; each pass over V produces NC output
+// columns, so V is read ceil(nk / NC) times.
+template <int NC>
+__global__ __launch_bounds__(256) void britz_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
+                                                    const double* __restrict__ Y, float* __restrict__ out, size_t ldo,
+                                                    size_t Np) {
+  extern __shared__ double ys[];  // [m][nk]
+  for (int t = threadIdx.x; t < m * nk; t += blockDim.x) ys[t] = Y[t];
+  __syncthreads();
+  for (int c0 = 0; c0 < nk; c0 += NC) {
+    for (size_t p = size_t(blockIdx.x) * 256 + threadIdx.x; p < Np; p += size_t(gridDim.x) * 256) {
+      double acc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) acc[c] = 0.0;
+      int i = 0;
+      for (; i + 4 <= m; i += 4) {  // four loads in flight before the FMAs
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = V[size_t(i + q) * ld + p];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double* yr = ys + size_t(i + q) * nk + c0;
+#pragma unroll
+          for (int c = 0; c < NC; c++) acc[c] = fma(yr[c0 + c < nk ? c : 0], v[q], acc[c]);
+        }
+      }
+      for (; i < m; i++) {
+        const double v = V[size_t(i) * ld + p];
+        const double* yr = ys + size_t(i) * nk + c0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[c] = fma(yr[c0 + c < nk ? c : 0], v, acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        if (c0 + c < nk) out[size_t(c0 + c) * ldo + p] = float(acc[c]);
+    }
   }
 }
 
@@ -254,6 +387,16 @@ __global__ __launch_bounds__(256) void to_uspace_kernel(const float* __restrict_
 
 namespace {
 
+// Launch shapes of the panel kernels per block size: NV basis vectors per
+// dots group, PX pixels per thread in the dots and update kernels.
+template <int B>
+struct BlockShape {
+  static constexpr int NV = B <= 4 ? 8 : (B <= 8 ? 4 : 2);
+  static constexpr int PXD = B <= 8 ? 4 : 2;
+  static constexpr int PXU = B <= 8 ? 2 : 1;
+};
+constexpr int kBDotsGrid = 1024;
+
 template <int B>
 struct BlockOps {
   Context& c;
@@ -264,8 +407,8 @@ struct BlockOps {
   BSpmvArgs A;
   unsigned vg;
   BlockOps(Context& cc, const Stencil& s_) : c(cc), st(s_), Np(s_.Np) {
-    NC = int((Np + kBDotChunk - 1) / kBDotChunk);
-    partial = c.ws<double>("beig.partial", size_t(kBMaxBasis + kBMax) * kBMax * NC);
+    NC = int(std::min<size_t>(kBDotsGrid, div_up(Np, size_t(256) * BlockShape<B>::PXD)));
+    partial = c.ws<double>("beig.partial", size_t(kBMaxBasis) * kBMax * kBDotsGrid);
     int dx[kStencilHalf], dy[kStencilHalf];
     stencil_upper_offsets(dx, dy);
     for (int j = 0; j < kStencilHalf; j++) {
@@ -273,26 +416,63 @@ struct BlockOps {
       A.lo[j] = dy[j] * kBTileW + dx[j];
     }
     vg = std::min<unsigned>(2048, div_up(Np, 256));
+    ug = std::min<unsigned>(2048, div_up(Np, size_t(256) * BlockShape<B>::PXU));
   }
+  unsigned ug;
   void spmv(const float* X, float* Y) {
     KScope ks(c, kKSpmv, double(st.W) * st.H * (4.0 * kStencilHalf + 4 + 8.0 * B));
-    hipLaunchKernelGGL(bspmv_kernel<B>, dim3(div_up(st.W, kBTX), div_up(st.H, kBTY)), dim3(kBTX, kBTY), 0, c.stream,
-                       st.wup, st.s, X, Np, Y, Np, st.W, st.H, st.Wp, st.Hp, Np, A);
+    const dim3 grid(div_up(st.W, kBTX), div_up(st.H, kBTY)), block(kBTX, kBTY);
+    if (c.spmv_fp64)
+      hipLaunchKernelGGL((bspmv_kernel<B, double>), grid, block, 0, c.stream, st.wup, st.s, X, Np, Y, Np, st.W, st.H,
+                         st.Wp, st.Hp, Np, A);
+    else
+      hipLaunchKernelGGL((bspmv_kernel<B, float>), grid, block, 0, c.stream, st.wup, st.s, X, Np, Y, Np, st.W, st.H,
+                         st.Wp, st.Hp, Np, A);
     GPB_LAUNCH_CHECK();
   }
   // dst[i * B + r] = V_i . (D) W_r, i < n
   void dots(const float* V, int n, const float* Wm, const float* D, double* dst) {
     {
       KScope ks(c, kKDots, double(Np) * (4.0 * n + 4.0 * B + (D ? 4 : 0)));
-      hipLaunchKernelGGL(bdots_kernel<B>, dim3(NC, n), dim3(256), 0, c.stream, V, Np, Wm, Np, D, Np, partial);
+      if (n > kBMaxBasis) throw ArgError("block Lanczos: basis above the dots kernel's capacity");
+      hipLaunchKernelGGL((bdots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD>), dim3(NC), dim3(256), 0, c.stream, V,
+                         Np, n, Wm, Np, D, Np, partial);
       GPB_LAUNCH_CHECK();
     }
     launch_reduce(c, partial, NC, n * B, dst);
   }
   void update(const float* V, int n, const double* Hm, float* Wm) {
     KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
-    hipLaunchKernelGGL(bupdate_kernel<B>, dim3(vg), dim3(256), 0, c.stream, V, Np, n, Hm, Wm, Np, Np);
+    hipLaunchKernelGGL((bupdate_kernel<B, BlockShape<B>::PXU>), dim3(ug), dim3(256), 0, c.stream, V, Np, n, Hm, Wm, Np,
+                       Np);
     GPB_LAUNCH_CHECK();
+  }
+  // out_c = sum_i Y[i][c] V_i, c < nk
+  void ritz(const float* V, int m, int nk, const double* Ydev, float* out) {
+    const size_t lds = sizeof(double) * size_t(m) * nk;
+    if (lds > 64 * 1024 || nk > 64) {  // Y does not fit the LDS budget: the general kernel
+      launch_ritz(c, V, Np, m, nk, Ydev, out, Np);
+      return;
+    }
+    const int nc = nk <= 24 ? 24 : (nk <= 48 ? 48 : 64);
+    KScope ks(c, kKRitz, double(Np) * 4.0 * (double(m) * div_up(nk, nc) + nk));
+    const dim3 grid(std::min<unsigned>(2048, div_up(Np, 256)));
+    if (nc == 24)
+      hipLaunchKernelGGL(britz_kernel<24>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
+    else if (nc == 48)
+      hipLaunchKernelGGL(britz_kernel<48>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
+    else
+      hipLaunchKernelGGL(britz_kernel<64>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
+    GPB_LAUNCH_CHECK();
+  }
+  // dst[i] = ||D^1/2 X_i||^2, i < n
+  void wnorms(const float* X, int n, double* dst) {
+    {
+      KScope ks(c, kKDots, double(Np) * (4.0 * n + 4));
+      hipLaunchKernelGGL(bwnorm_kernel, dim3(NC, n), dim3(256), 0, c.stream, X, Np, n, st.D, Np, partial);
+      GPB_LAUNCH_CHECK();
+    }
+    launch_reduce(c, partial, NC, n, dst);
   }
   // CholQR2 of the panel X (in place); Rtot = R of X_in = X_out Rtot.
   void cholqr2(float* X, double* G, double* Rinv, double* R1, double* Rtot, int* brk) {
@@ -339,6 +519,10 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   hipLaunchKernelGGL(fill_f64_kernel, dim3(64), dim3(256), 0, c.stream, T, size_t(M) * M, 0.0);
   GPB_LAUNCH_CHECK();
   launch_fill(c, Wb, size_t(B) * Np, 0.f);
+  // panels are built in place and the SpMV writes interior pixels only: the
+  // halo entries of every column must read zero
+  launch_fill(c, V, size_t(M + B) * Np, 0.f);
+  launch_fill(c, V2, size_t(M + B) * Np, 0.f);
 
   // locked u0 = D^1/2 1 / ||.||, starting panel at [1, 1 + B)
   launch_locked_start(c, st, V, V + Np, B, Np, 0x5eed1234abcdull);
@@ -377,16 +561,16 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     while (k + B <= M) {
       float* Pk = V + size_t(k) * Np;
       const int n = k + B;
-      ops.spmv(Pk, Wb);
+      float* Pn = V + size_t(n) * Np;  // the next panel is built in place
+      ops.spmv(Pk, Pn);
       matvecs += B;
-      ops.dots(V, n, Wb, nullptr, H1);
-      ops.update(V, n, H1, Wb);
-      ops.dots(V, n, Wb, nullptr, H2);
-      ops.update(V, n, H2, Wb);
+      ops.dots(V, n, Pn, nullptr, H1);
+      ops.update(V, n, H1, Pn);
+      ops.dots(V, n, Pn, nullptr, H2);
+      ops.update(V, n, H2, Pn);
       hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, H1, H2);
       GPB_LAUNCH_CHECK();
-      ops.cholqr2(Wb, G, Rinv, R1, Rtot, brk);
-      GPB_HIP(hipMemcpyAsync(V + size_t(n) * Np, Wb, size_t(B) * Np * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+      ops.cholqr2(Pn, G, Rinv, R1, Rtot, brk);
       k += B;
     }
     kcur = k;  // projected size
@@ -425,13 +609,13 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     for (int j = 0; j < kcur; j++)
       for (int i = 0; i < nk; i++) Yk[size_t(j) * nk + i] = Y[size_t(j) * kcur + i];
     GPB_HIP(hipMemcpyAsync(Ydev, Yk.data(), sizeof(double) * kcur * nk, hipMemcpyHostToDevice, c.stream));
-    launch_ritz(c, V, Np, kcur, nk, Ydev, V2, Np);
+    ops.ritz(V, kcur, nk, Ydev, V2);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(nk) * Np, V + size_t(kcur) * Np, size_t(B) * Np * sizeof(float),
                            hipMemcpyDeviceToDevice, c.stream));
     // ||D^1/2 u_i|| for the wanted Ritz vectors
-    for (int i = 0; i < K; i++) ops.dots(V2 + size_t(i) * Np, 1, V2 + size_t(i) * Np, st.D, nrm + i * B);
-    std::vector<double> nh(size_t(K) * B);
-    GPB_HIP(hipMemcpyAsync(nh.data(), nrm, sizeof(double) * K * B, hipMemcpyDeviceToHost, c.stream));
+    ops.wnorms(V2, K, nrm);
+    std::vector<double> nh(static_cast<size_t>(K));
+    GPB_HIP(hipMemcpyAsync(nh.data(), nrm, sizeof(double) * K, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     std::swap(V, V2);
     double worst = 0;
@@ -446,7 +630,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
       double rr = 0;
       for (int a = 0; a < B; a++)
         for (int b2 = 0; b2 < B; b2++) rr += cvec[a] * Gd[size_t(a) * B + b2] * cvec[b2];
-      const double un = nh[size_t(i) * B];
+      const double un = nh[size_t(i)];
       resid[size_t(i)] = std::sqrt(std::max(rr, 0.0)) / std::sqrt(std::max(un, 1e-300));
       worst = std::max(worst, resid[size_t(i)]);
     }
@@ -472,6 +656,77 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
 }
 
 }  // namespace
+
+// partial[(2 i) G + g] = sum D (y_i - lam_i u_i)^2, partial[(2 i + 1) G + g] =
+// sum D u_i^2 over workgroup g's pixels.
+__global__ __launch_bounds__(256) void resid_parts_kernel(const float* __restrict__ U, const float* __restrict__ Y,
+                                                          const float* __restrict__ D, const double* __restrict__ lam,
+                                                          size_t Np, double* __restrict__ partial) {
+  __shared__ double sh[2][4];
+  const int i = blockIdx.y;
+  const double l = lam[i];
+  double a = 0, b = 0;
+  for (size_t p = size_t(blockIdx.x) * 256 + threadIdx.x; p < Np; p += size_t(gridDim.x) * 256) {
+    const double u = U[size_t(i) * Np + p], d = D[p];
+    const double r = double(Y[size_t(i) * Np + p]) - l * u;
+    a = fma(d * r, r, a);
+    b = fma(d * u, u, b);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[size_t(2 * i) * gridDim.x + blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    partial[size_t(2 * i + 1) * gridDim.x + blockIdx.x] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// Measured relative residuals ||(D - W) v_i - lam_i D v_i|| / ||D v_i|| of
+// K <= 9 pairs (v generalised, padded layout): u = D^1/2 v, y = L u, and
+// (D - W) v - lam D v = D^1/2 (y - lam u), D v = D^1/2 u.
+double measured_max_residual(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K,
+                             double* resid) {
+  if (K < 1 || K > 9) throw ArgError("residual check takes 1 to 9 pairs");
+  const size_t Np = st.Np;
+  float* U = c.ws<float>("res.U", size_t(9) * Np);
+  float* Y = c.ws<float>("res.Y", size_t(9) * Np);
+  launch_fill(c, U, size_t(9) * Np, 0.f);
+  const unsigned vg = std::min<unsigned>(2048, div_up(Np, 256));
+  for (int i = 0; i < K; i++) {
+    hipLaunchKernelGGL(to_uspace_kernel, dim3(vg), dim3(256), 0, c.stream, d_evecs + size_t(i) * Np, st.D,
+                       U + size_t(i) * Np, Np);
+    GPB_LAUNCH_CHECK();
+  }
+  launch_spmv(c, st, U, Y);
+  if (K > 1) {
+    BlockOps<8> ops(c, st);
+    ops.spmv(U + Np, Y + Np);  // columns 1..8 (beyond K: zero vectors)
+  }
+  double* lam = c.ws<double>("res.lam", 9);
+  double* part = c.ws<double>("res.part", size_t(18) * 1024);
+  double* sums = c.ws<double>("res.sums", 18);
+  GPB_HIP(hipMemcpyAsync(lam, evals, sizeof(double) * K, hipMemcpyHostToDevice, c.stream));
+  const unsigned G = std::min<unsigned>(1024, div_up(Np, 256));
+  hipLaunchKernelGGL(resid_parts_kernel, dim3(G, K), dim3(256), 0, c.stream, U, Y, st.D, lam, Np, part);
+  GPB_LAUNCH_CHECK();
+  launch_reduce(c, part, int(G), 2 * K, sums);
+  std::vector<double> h(static_cast<size_t>(2 * K));
+  GPB_HIP(hipMemcpyAsync(h.data(), sums, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, c.stream));
+  GPB_HIP(hipStreamSynchronize(c.stream));
+  double worst = 0;
+  for (int i = 0; i < K; i++) {
+    const double r = std::sqrt(std::max(h[size_t(2 * i)], 0.0)) / std::sqrt(std::max(h[size_t(2 * i + 1)], 1e-300));
+    if (resid) resid[i] = r;
+    worst = std::max(worst, r);
+  }
+  return worst;
+}
 
 // Diagnostic: Y = L X for B columns with the block kernel, or the
 // single-vector kernel column by column when B == 1.

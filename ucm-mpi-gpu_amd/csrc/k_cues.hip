@@ -16,6 +16,7 @@
 // with a_s = 1 / (c . h_s) (c = column sums of the truncated smoothing
 // matrix G), sum = G * sum_s a_s h_s and diff_o = G * V_o, where
 // V_{o+1} = V_o + 2 a_o h_o - 2 a_{o+8} h_{o+8}.
+#include <cstdio>
 #include <array>
 #include <mutex>
 
@@ -104,7 +105,10 @@ struct CueParams {
   void* mpb_all;            // [8][N] (float, or double in the FP64 variant)
   void* gcue;               // [8][N] sum of gPb-weighted cues
   void* cues;               // optional [12][8][N]
+  float* feat;              // optional [32][ceil(H/4)][ceil(W/4)]: per channel and orientation, the
+                            // scale-weighted mPb contribution averaged over 4x4 cells (FP32 kernel only)
   int W, H;
+  int by0;                  // first tile row of this launch (launches may cover row bands)
 };
 
 template <typename CT>
@@ -230,7 +234,7 @@ template <int NB, int KH, typename CT>
 __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radii, uint8_t* tile, uint32_t* hist,
                             CT* csum, int lane, int px, int py, bool valid, CT mpb[kNumOri], CT gc[kNumOri]) {
   const int W = P.W, H = P.H;
-  const int gx0 = int(blockIdx.x) * kCueTile - kCueRmax, gy0 = int(blockIdx.y) * kCueTile - kCueRmax;
+  const int gx0 = int(blockIdx.x) * kCueTile - kCueRmax, gy0 = (int(blockIdx.y) + P.by0) * kCueTile - kCueRmax;
   const uint8_t* plane = P.plane[ch];
   for (int i = lane; i < kCueTileW * kCueTileW; i += kCueLanes) {
     const int ty = i / kCueTileW, tx = i - ty * kCueTileW;
@@ -254,6 +258,9 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
   __syncthreads();
   const int ty = lane / kCueTile, tx = lane - ty * kCueTile;
   const uint8_t* centre = tile + (ty + kCueRmax) * kCueTileW + tx + kCueRmax;
+  CT chf[kNumOri];
+#pragma unroll
+  for (int o = 0; o < kNumOri; o++) chf[o] = CT(0);
   int i = 0;
   for (int ri = 0; ri < 3; ri++) {
     const int end = P.end_by_r[radii[ri]];
@@ -272,7 +279,29 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
       for (int o = 0; o < kNumOri; o++) {
         mpb[o] = fma_t(wmpb_of<CT>(P, cue), chi[o], mpb[o]);
         gc[o] = fma_t(wgpb_of<CT>(P, cue), chi[o], gc[o]);
+        chf[o] = fma_t(wmpb_of<CT>(P, cue), chi[o], chf[o]);
         if (P.cues) static_cast<CT*>(P.cues)[(size_t(cue) * kNumOri + o) * N + p] = chi[o];
+      }
+    }
+  }
+  if constexpr (sizeof(CT) == 4) {
+    if (P.feat) {
+      // 4x4 cell mean: the cell's 16 lanes differ in lane bits 0, 1 (x) and 3, 4 (y)
+      const int Wq = (W + 3) / 4, Hq = (H + 3) / 4;
+      float cnt = valid ? 1.f : 0.f;
+      for (int m = 1; m <= 16; m <<= 1) {
+        if (m == 4) continue;
+        cnt += __shfl_xor(cnt, m, 64);
+      }
+#pragma unroll
+      for (int o = 0; o < kNumOri; o++) {
+        float v = valid ? float(chf[o]) : 0.f;
+        for (int m = 1; m <= 16; m <<= 1) {
+          if (m == 4) continue;
+          v += __shfl_xor(v, m, 64);
+        }
+        if ((lane & 0x1b) == 0 && cnt > 0.f)
+          P.feat[(size_t(ch) * kNumOri + o) * size_t(Hq) * Wq + size_t(py >> 2) * Wq + (px >> 2)] = v / cnt;
       }
     }
   }
@@ -285,7 +314,7 @@ __global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
   __shared__ uint32_t hist[4 * kNbMax * kCueLanes];
   __shared__ CT csum[kNbMax];
   const int lane = threadIdx.x;
-  const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = blockIdx.y * kCueTile + (lane / kCueTile);
+  const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = (blockIdx.y + P.by0) * kCueTile + (lane / kCueTile);
   const bool valid = px < P.W && py < P.H;
   CT mpb[kNumOri], gc[kNumOri];
 #pragma unroll
@@ -450,7 +479,7 @@ static CueTables& cue_tables(Context& c) {
 
 template <typename CT>
 static void local_cues_t(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, CT* d_mpb_all,
-                         CT* d_gcue, CT* d_cues) {
+                         CT* d_gcue, CT* d_cues, float* d_feat) {
   CueTables& t = cue_tables(c);
   const size_t N = size_t(W) * H;
   uint8_t* q = c.ws<uint8_t>("cue.q", 3 * N);
@@ -486,21 +515,34 @@ static void local_cues_t(Context& c, const uint32_t* d_rgb, const uint8_t* d_tex
   P.mpb_all = d_mpb_all;
   P.gcue = d_gcue;
   P.cues = d_cues;
+  P.feat = d_feat;
   P.W = W;
   P.H = H;
-  KScope ks(c, kKCues, double(N) * (4 + 16 * sizeof(CT) + (d_cues ? 96 * sizeof(CT) : 0)));
-  hipLaunchKernelGGL(cues_kernel<CT>, dim3(div_up(W, kCueTile), div_up(H, kCueTile)), dim3(kCueLanes), 0, c.stream, P);
-  GPB_LAUNCH_CHECK();
+  KScope ks(c, kKCues, double(N) * (4 + 16 * sizeof(CT) + (d_cues ? 96 * sizeof(CT) : 0)) + (d_feat ? 8.0 * N : 0.0));
+  const int ty = int(div_up(H, kCueTile));
+  // verbose runs launch row bands of 64 tile rows, each followed by a progress line
+  const int band = c.verbose ? 64 : ty;
+  for (int y0 = 0; y0 < ty; y0 += band) {
+    P.by0 = y0;
+    hipLaunchKernelGGL(cues_kernel<CT>, dim3(div_up(W, kCueTile), std::min(band, ty - y0)), dim3(kCueLanes), 0,
+                       c.stream, P);
+    GPB_LAUNCH_CHECK();
+    if (c.verbose && ty > band) {
+      GPB_HIP(hipStreamSynchronize(c.stream));
+      std::fprintf(stderr, "gpb: local cues (%s) %dx%d: rows %d of %d\n", sizeof(CT) == 8 ? "FP64" : "FP32", W, H,
+                   std::min(H, (y0 + band) * kCueTile), H);
+    }
+  }
 }
 
 void launch_local_cues(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, float* d_mpb_all,
-                       float* d_gcue, float* d_cues) {
-  local_cues_t<float>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues);
+                       float* d_gcue, float* d_cues, float* d_feat) {
+  local_cues_t<float>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues, d_feat);
 }
 
 void launch_local_cues_f64(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, double* d_mpb_all,
                            double* d_gcue, double* d_cues) {
-  local_cues_t<double>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues);
+  local_cues_t<double>(c, d_rgb, d_text, W, H, d_mpb_all, d_gcue, d_cues, nullptr);
 }
 
 void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d_q /* 3N */, int* d_nb /* 3 */) {

@@ -62,7 +62,7 @@ class Stats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 6), ("lanczos_matvecs", C.c_int), ("lanczos_restarts", C.c_int),
                 ("lanczos_basis", C.c_int), ("lanczos_block", C.c_int), ("converged", C.c_int),
                 ("max_residual", C.c_double), ("evals", C.c_double * 32), ("fallback", C.c_int),
-                ("reorth_second_passes", C.c_int)]
+                ("reorth_second_passes", C.c_int), ("warm_mode", C.c_int), ("warm_ms", C.c_double)]
 
 
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
@@ -83,7 +83,38 @@ _sig("gpb_get_patch_stats", C.c_int, [_ctxp, C.c_int, C.c_void_p])
 _sig("gpb_get_spmv_tile", C.c_int, [_ctxp, C.c_int, C.c_int, C.POINTER(C.c_int)])
 _sig("gpb_apply_laplacian", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, C.c_int, C.c_int, _f32p, _f32p])
 _sig("gpb_kernel_time", C.c_int, [_ctxp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long), C.POINTER(C.c_double)])
-KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb"]
+_sig("gpb_set_warm_start", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int])
+_sig("gpb_warm_param_count", C.c_size_t, [])
+_sig("gpb_warm_layer_info", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)])
+_sig("gpb_warm_init", C.c_int, [_ctxp, C.c_uint64])
+_sig("gpb_warm_set_params", C.c_int, [_ctxp, _f32p])
+_sig("gpb_warm_get_params", C.c_int, [_ctxp, _f32p])
+_sig("gpb_warm_dataset_reset", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int])
+_sig("gpb_warm_dataset_add", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p])
+_sig("gpb_warm_dataset_size", C.c_int, [_ctxp, C.POINTER(C.c_int)])
+_sig("gpb_warm_features", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, _f32p])
+_sig("gpb_warm_train", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_void_p])
+_sig("gpb_warm_eval_loss", C.c_int, [_ctxp, C.c_int, C.c_int, C.POINTER(C.c_double)])
+_sig("gpb_warm_loss_grad", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, C.POINTER(C.c_double),
+                                     C.c_void_p, C.c_void_p])
+
+
+def warm_param_count() -> int:
+    return int(lib.gpb_warm_param_count())
+
+
+def warm_layers() -> list:
+    """Per layer: (cin, cout, relu, weight offset, bias offset) in the parameter vector."""
+    out = []
+    for l in range(12):
+        ci, co, r = C.c_int(), C.c_int(), C.c_int()
+        wo, bo = C.c_size_t(), C.c_size_t()
+        _check(lib.gpb_warm_layer_info(l, C.byref(ci), C.byref(co), C.byref(r), C.byref(wo), C.byref(bo)),
+               what="warm_layer_info")
+        out.append((ci.value, co.value, r.value, wo.value, bo.value))
+    return out
+KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb", "warm"]
 
 
 def ic_program(left: int, right: int, top: int, bottom: int):
@@ -252,7 +283,7 @@ class Context:
         return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
                     basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
                     evals=list(s.evals), fallback=s.fallback,
-                    second_passes=s.reorth_second_passes)
+                    second_passes=s.reorth_second_passes, warm_mode=s.warm_mode, warm_ms=s.warm_ms)
 
     def build_w(self, mpb_nmax: np.ndarray):
         m = np.ascontiguousarray(mpb_nmax, np.float32)
@@ -320,6 +351,74 @@ class Context:
 
     def set_block_size(self, block: int):
         self._chk(lib.gpb_set_block_size(self.h, block))
+
+    # ---- learned warm start ----
+    def set_warm_start(self, mode: int, basis: int = 0, keep: int = 0):
+        """0 plain Lanczos, 1 learned block, 2 block Lanczos from a random block."""
+        self._chk(lib.gpb_set_warm_start(self.h, mode, basis, keep))
+
+    def warm_init(self, seed: int):
+        self._chk(lib.gpb_warm_init(self.h, seed))
+
+    def warm_params(self) -> np.ndarray:
+        p = np.empty(warm_param_count(), np.float32)
+        self._chk(lib.gpb_warm_get_params(self.h, p))
+        return p
+
+    def set_warm_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != warm_param_count():
+            raise ValueError("parameter vector has the wrong length")
+        self._chk(lib.gpb_warm_set_params(self.h, p))
+
+    def warm_dataset_reset(self, width: int, height: int, capacity: int):
+        self._chk(lib.gpb_warm_dataset_reset(self.h, width, height, capacity))
+
+    def warm_dataset_add(self, packed: np.ndarray, codebook: np.ndarray) -> np.ndarray:
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        ev = np.empty(self.nvec)
+        self._chk(lib.gpb_warm_dataset_add(self.h, W, H, packed, np.ascontiguousarray(codebook, np.float32),
+                                           ev.ctypes.data))
+        return ev
+
+    def warm_features(self, packed: np.ndarray, codebook: np.ndarray) -> np.ndarray:
+        """Cue features [32, ceil(H/4), ceil(W/4)] (4 cues x 8 orientations, 4x4 cell means)."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        f = np.empty((32, (H + 3) // 4, (W + 3) // 4), np.float32)
+        self._chk(lib.gpb_warm_features(self.h, W, H, packed, np.ascontiguousarray(codebook, np.float32),
+                                        f.reshape(-1)))
+        return f
+
+    def warm_dataset_size(self) -> int:
+        n = C.c_int()
+        self._chk(lib.gpb_warm_dataset_size(self.h, C.byref(n)))
+        return n.value
+
+    def warm_train(self, first: int, count: int, epochs: int, batch: int, lr: float, seed: int) -> np.ndarray:
+        """Trains on training-set entries [first, first + count); returns the per-epoch mean loss."""
+        loss = np.empty(epochs, np.float32)
+        self._chk(lib.gpb_warm_train(self.h, first, count, epochs, batch, lr, seed, loss.ctypes.data))
+        return loss
+
+    def warm_eval_loss(self, first: int, count: int) -> float:
+        v = C.c_double()
+        self._chk(lib.gpb_warm_eval_loss(self.h, first, count, C.byref(v)))
+        return v.value
+
+    def warm_loss_grad(self, x: np.ndarray, targets: np.ndarray, dweights: np.ndarray):
+        """x [n,hq,wq,34], targets [n,hq,wq,8], dweights [n,hq,wq] -> (loss, grad, y [n,hq,wq,8])."""
+        x = np.ascontiguousarray(x, np.float32)
+        n, hq, wq, _ = x.shape
+        g = np.empty(warm_param_count(), np.float32)
+        y = np.empty((n, hq, wq, 8), np.float32)
+        v = C.c_double()
+        self._chk(lib.gpb_warm_loss_grad(self.h, n, hq, wq, x.reshape(-1),
+                                         np.ascontiguousarray(targets, np.float32).reshape(-1),
+                                         np.ascontiguousarray(dweights, np.float32).reshape(-1), C.byref(v),
+                                         g.ctypes.data, y.ctypes.data))
+        return v.value, g, y
 
     def set_spmv_tile(self, rows: int):
         """SpMV rows per thread (2, 4, 8), or 0 to autotune per image shape."""
