@@ -108,6 +108,39 @@ def bench_patches(gpb, ctx, tol: float, size: int = 5000, regions: int = 2500, g
     return out
 
 
+def bench_warm(gpb, ctx, step, S: int, steps: int, tol: float):
+    """The warm-start form on the same image: block Lanczos (8 columns plus the
+    locked D^1/2 1) started from the trained network's block (mode 1), beside
+    plain Lanczos (mode 0) and block Lanczos from a random block (mode 2), all
+    timed without per-kernel events, at the same residual target."""
+    import numpy as np
+    import torch
+
+    weights = ROOT / "ucm-mpi-gpu_amd" / "weights" / "warm_start.f32"
+    if not weights.exists():
+        return dict(skipped=f"no trained parameters at {weights.relative_to(ROOT)}")
+    ctx.set_warm_params(np.fromfile(weights, dtype="<f4"))
+    out = {}
+    for name, mode in (("plain", 0), ("warm_start", 1), ("random_block", 2)):
+        ctx.set_warm_start(mode, 64, 24)
+        step()  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        out[name] = dict(value=round(S * S * steps / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3 / steps, 1),
+                         matvecs=st["matvecs"], fallback=st["fallback"], max_residual=st["max_residual"],
+                         network_ms=round(st["warm_ms"], 2), stage_ms=[round(x, 1) for x in st["stage_ms"]])
+    ctx.set_warm_start(0)
+    out["speedup_warm_vs_plain"] = round(out["plain"]["ms"] / out["warm_start"]["ms"], 3)
+    out["speedup_random_block_vs_plain"] = round(out["plain"]["ms"] / out["random_block"]["ms"], 3)
+    out["workload"] = f"the plain form's image, 9 eigenvectors, residual <= {tol:g} (measured), block basis 64 keep 24"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +156,8 @@ def main():
     ap.add_argument("--spmv-tile", type=int, default=0, help="SpMV rows per thread (2, 4, 8); 0 = autotune")
     ap.add_argument("--spmv-fp32", action="store_true", help="SpMV arithmetic in FP32 (default FP64 arithmetic)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without per-kernel HIP events")
-    ap.add_argument("--forms", default="batched", help="comma list of extra forms to time: batched (C4), none")
+    ap.add_argument("--forms", default="batched,warm",
+                    help="comma list of extra forms to time: batched (C4), warm (learned warm start), none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -215,8 +249,12 @@ def main():
                                  gbs=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0)
                          for k, v in kt.items()})
     forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if forms:
+        line["forms"] = {}
+    if "warm" in forms:
+        line["forms"]["warm_start"] = bench_warm(gpb, ctx, step, S, args.steps, tol)
     if "batched" in forms:
-        line["forms"] = dict(batched_patches=bench_patches(gpb, ctx, tol))
+        line["forms"]["batched_patches"] = bench_patches(gpb, ctx, tol)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(packed, codebook, args.cpu_crop, tol)
     print(json.dumps(line), flush=True)

@@ -162,9 +162,11 @@ int gpb_warm_dataset_size(gpb_ctx* ctx, int* n);
 int gpb_warm_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
                       float* feat);
 /* Adam (lr, betas 0.9 / 0.999, gradient norm clipped at 1) over shuffled
- * mini-batches of training-set entries [first, first + count); epoch_loss
- * (optional, epochs entries): mean training loss per epoch. */
-int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed,
+ * mini-batches of training-set entries [first, first + count); augment != 0
+ * puts each image through a random symmetry of the square cell grid (flips,
+ * transpose; cue orientations permuted to match); epoch_loss (optional,
+ * epochs entries): mean training loss per epoch. */
+int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed, int augment,
                    float* epoch_loss);
 /* Mean loss (mean squared sine of the principal angles, D inner product)
  * over training-set entries [first, first + count). */

@@ -125,6 +125,13 @@ void set_device(int device);
  * (seed 1, 10 Lloyd iterations) on the image's own Schmid responses. */
 void set_texton_codebook(const float* codebook);
 
+/* Learned warm start for computeGPb (no counterpart in the reference):
+ * params = gpb_warm_param_count() floats (e.g. the trained
+ * ucm-mpi-gpu_amd/weights/warm_start.f32), copied; later computeGPb calls
+ * start block Lanczos from the network's block, with plain Lanczos as the
+ * fallback.  nullptr (default) = plain Lanczos. */
+void set_warm_start(const float* params);
+
 /* Binary PPM (P6, maxval 255) reader: rgb = H x W x 3 bytes. */
 bool read_ppm(const std::string& path, int& width, int& height, std::vector<uint8_t>& rgb);
 

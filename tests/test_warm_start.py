@@ -243,9 +243,62 @@ def test_warm_training_lowers_held_out_loss_gpu(gpb, ctx):
         ctx.warm_dataset_add(packed, gpb.make_codebook(packed, seed=100 + i))
     assert ctx.warm_dataset_size() == n_train + n_test
     before = ctx.warm_eval_loss(n_train, n_test)
-    losses = ctx.warm_train(0, n_train, 40, 8, 2e-3, 3)
+    losses = ctx.warm_train(0, n_train, 40, 8, 2e-3, 3, augment=True)
     after = ctx.warm_eval_loss(n_train, n_test)
     print(f"  held-out loss {before:.4f} -> {after:.4f}; training loss {losses[0]:.4f} -> {losses[-1]:.4f}",
           flush=True)
     assert losses[-1] < losses[0]
     assert after < before
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tf", [1, 2, 4, 7])
+def test_feature_symmetries_gpu(gpb, ctx, tf):
+    """The symmetry augmentation's orientation map, checked physically: for
+    every orientation o of a flipped / transposed image, the best-matching
+    orientation of the original's features (moved with the geometry) is
+    pi(o), with o sent to -o by a flip and to pi/2 - o by a transpose.  The
+    match is not exact: the half-discs' half-pixel origin makes the cue maps
+    only approximately mirror-symmetric (5-13 % mean difference even for the
+    orientations that map to themselves)."""
+    S = 96
+    print(f"warm start: feature symmetry {tf} on {S}x{S}", flush=True)
+    rgb, packed = synth_packed(gpb, S, S, regions=9, seed=23)
+    cb = gpb.make_codebook(packed, seed=23)
+    f0 = ctx.warm_features(packed, cb)  # [32, 24, 24]
+    img = rgb
+    if tf & 1:
+        img = img[:, ::-1]
+    if tf & 2:
+        img = img[::-1, :]
+    if tf & 4:
+        img = img.transpose(1, 0, 2)
+    f1 = ctx.warm_features(gpb.pack_rgb(np.ascontiguousarray(img)), cb)
+
+    def moved(m):  # the original's map seen from the transformed image: transpose . flip_y . flip_x
+        if tf & 4:
+            m = m.T
+        if tf & 2:
+            m = m[::-1, :]
+        if tf & 1:
+            m = m[:, ::-1]
+        return m
+
+    worst = 0.0
+    for o in range(8):
+        so = o
+        if tf & 1:
+            so = (8 - so) & 7
+        if tf & 2:
+            so = (8 - so) & 7
+        if tf & 4:
+            so = (12 - so) & 7
+        errs = []
+        for cand in range(8):
+            e = sum(np.abs(f1[8 * c + o] - moved(f0[8 * c + cand]))[2:-2, 2:-2].mean() for c in range(4))
+            errs.append(e)
+        ref = sum(np.abs(moved(f0[8 * c + so]))[2:-2, 2:-2].mean() for c in range(4))
+        worst = max(worst, errs[so] / ref)
+        print(f"  o {o}: expected source {so}, best {int(np.argmin(errs))}, rel diff {errs[so] / ref:.3f}", flush=True)
+        assert int(np.argmin(errs)) == so, o
+    assert worst < 0.25

@@ -160,6 +160,7 @@ struct Context {
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
+  std::map<uint64_t, int> spmv_band;  // tile-band width tuned with the rows (tiles)
   int spmv_rows_forced = 0;
   bool spmv_fp64 = true;   // SpMV arithmetic in FP64 (operands and results FP32 in HBM)
   bool capturing = false;

@@ -503,9 +503,9 @@ int gpb_warm_dataset_size(gpb_ctx* ctx, int* n) {
   });
 }
 
-int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed,
+int gpb_warm_train(gpb_ctx* ctx, int first, int count, int epochs, int batch, float lr, uint64_t seed, int augment,
                    float* epoch_loss) {
-  return guarded2(ctx, [&] { warm_train(ctx->c, first, count, epochs, batch, lr, seed, epoch_loss); });
+  return guarded2(ctx, [&] { warm_train(ctx->c, first, count, epochs, batch, lr, seed, augment, epoch_loss); });
 }
 
 int gpb_warm_eval_loss(gpb_ctx* ctx, int first, int count, double* loss) {

@@ -12,6 +12,7 @@
 // locked as basis vector 0.  The starting block is random or supplied by the
 // caller (warm start).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -35,14 +36,24 @@ struct BSpmvArgs {
 // u = s x is staged in LDS in FP32 (LDS bandwidth bounds this kernel); AT is
 // the accumulation type: with AT = double each product of an FP32 weight and
 // an FP32 difference is exact and the 80-term sums carry no FP32 rounding.
+// Tiles are dealt XCD-aware in band-major order (as the single-vector SpMV):
+// workgroup b runs on XCD b % 8, each XCD takes a contiguous run of tiles
+// numbered by vertical bands `band` tiles wide, row-major within a band.
 template <int B, typename AT>
 __global__ __launch_bounds__(kBTX * kBTY) void bspmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
                                                             const float* __restrict__ X, size_t ldx,
                                                             float* __restrict__ Y, size_t ldy, int W, int H, int Wp,
-                                                            int Hp, size_t Np, BSpmvArgs A) {
+                                                            int Hp, size_t Np, BSpmvArgs A, int band) {
   __shared__ float u[kBTileH * kBTileW * B];
   const int tid = threadIdx.y * kBTX + threadIdx.x;
-  const int x0 = blockIdx.x * kBTX, y0 = blockIdx.y * kBTY;
+  const int tiles_x = (W + kBTX - 1) / kBTX, tiles_y = (H + kBTY - 1) / kBTY, ntiles = tiles_x * tiles_y;
+  const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
+  const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
+  if (tile >= ntiles) return;  // whole workgroup
+  const int bw = min(band, tiles_x);
+  const int bi = tile / (bw * tiles_y), r0 = tile - bi * bw * tiles_y;
+  const int w_here = min(bw, tiles_x - bi * bw);
+  const int x0 = (bi * bw + r0 % w_here) * kBTX, y0 = (r0 / w_here) * kBTY;
   for (int i = tid; i < kBTileH * kBTileW; i += kBTX * kBTY) {
     const int ty = i / kBTileW, tx = i - ty * kBTileW;
     const int py = y0 + ty, px = x0 + tx;
@@ -420,15 +431,57 @@ struct BlockOps {
   }
   unsigned ug;
   void spmv(const float* X, float* Y) {
+    if (band == 0) band = tune_band(X, Y);
     KScope ks(c, kKSpmv, double(st.W) * st.H * (4.0 * kStencilHalf + 4 + 8.0 * B));
-    const dim3 grid(div_up(st.W, kBTX), div_up(st.H, kBTY)), block(kBTX, kBTY);
+    launch_spmv_band(X, Y, band);
+  }
+  int band = 0;
+  void launch_spmv_band(const float* X, float* Y, int bnd) {
+    const int ntiles = int(div_up(st.W, kBTX) * div_up(st.H, kBTY));
+    const dim3 grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd)), block(kBTX, kBTY);
     if (c.spmv_fp64)
       hipLaunchKernelGGL((bspmv_kernel<B, double>), grid, block, 0, c.stream, st.wup, st.s, X, Np, Y, Np, st.W, st.H,
-                         st.Wp, st.Hp, Np, A);
+                         st.Wp, st.Hp, Np, A, bnd);
     else
       hipLaunchKernelGGL((bspmv_kernel<B, float>), grid, block, 0, c.stream, st.wup, st.s, X, Np, Y, Np, st.W, st.H,
-                         st.Wp, st.Hp, Np, A);
+                         st.Wp, st.Hp, Np, A, bnd);
     GPB_LAUNCH_CHECK();
+  }
+  // Band width of the block SpMV's tile order for this shape and B: timed
+  // once per context (2 applications after a warm-up per candidate) on the
+  // solve's own operands, which the calls overwrite with the same values.
+  int tune_band(const float* X, float* Y) {
+    const int tiles_x = int(div_up(st.W, kBTX));
+    const uint64_t key = ((uint64_t(st.W) << 32) | uint32_t(st.H)) ^ (uint64_t(B) << 52) ^ (1ull << 61) ^
+                         (c.spmv_fp64 ? (1ull << 62) : 0);
+    auto it = c.spmv_band.find(key);
+    if (it != c.spmv_band.end()) return it->second;
+    hipEvent_t e0, e1;
+    GPB_HIP(hipEventCreate(&e0));
+    GPB_HIP(hipEventCreate(&e1));
+    int best = tiles_x;
+    float best_ms = 1e30f;
+    for (int bnd : {2, 4, 8, tiles_x}) {
+      if (bnd > tiles_x) continue;
+      launch_spmv_band(X, Y, bnd);
+      GPB_HIP(hipEventRecord(e0, c.stream));
+      for (int r = 0; r < 2; r++) launch_spmv_band(X, Y, bnd);
+      GPB_HIP(hipEventRecord(e1, c.stream));
+      GPB_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (c.verbose)
+        std::fprintf(stderr, "gpb: block SpMV tuning %dx%d B=%d: band %d: %.1f us\n", st.W, st.H, B, bnd,
+                     ms * 1e3f / 2);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = bnd;
+      }
+    }
+    GPB_HIP(hipEventDestroy(e0));
+    GPB_HIP(hipEventDestroy(e1));
+    c.spmv_band[key] = best;
+    return best;
   }
   // dst[i * B + r] = V_i . (D) W_r, i < n
   void dots(const float* V, int n, const float* Wm, const float* D, double* dst) {

@@ -472,6 +472,42 @@ __global__ void rms_finish_kernel(float* acc, float count) {
   if (k < kWarmOut) acc[k] = sqrtf(acc[k] / fmaxf(count, 1.f));
 }
 
+// Copies dataset image `src` into batch slot `dst` under one of the 8
+// symmetries of the square cell grid (tf bit 0: flip x, bit 1: flip y, bit 2:
+// transpose; applied in that order to the output position).  Cue channel
+// 8 c + o moves with the line direction: a flip maps the angle o pi/8 to
+// -o pi/8, a transpose to pi/2 - o pi/8; the coordinate channels stay with
+// the position; targets and weights move with the geometry.
+__global__ __launch_bounds__(256) void gather_sym_kernel(const uint16_t* __restrict__ xs, const float* __restrict__ Ts,
+                                                         const float* __restrict__ Ds, GeoD g, int src,
+                                                         uint16_t* __restrict__ xd, float* __restrict__ Td,
+                                                         float* __restrict__ Dd, int dst, int tf) {
+  const int total = g.h * g.w;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
+    const int y = e / g.w, x = e - y * g.w;
+    int sx = x, sy = y;
+    if (tf & 1) sx = g.w - 1 - sx;
+    if (tf & 2) sy = g.h - 1 - sy;
+    if (tf & 4) {
+      const int t = sx;
+      sx = sy;
+      sy = t;
+    }
+    const long long qs = gpos(g, src, sy, sx), qd = gpos(g, dst, y, x), qc = gpos(g, src, y, x);
+    for (int c = 0; c < kWarmFeat; c++) {
+      int o = c & 7;
+      if (tf & 1) o = (8 - o) & 7;
+      if (tf & 2) o = (8 - o) & 7;
+      if (tf & 4) o = (12 - o) & 7;
+      xd[qd * kWarmIn + c] = xs[qs * kWarmIn + (c & ~7) + o];
+    }
+    for (int c = kWarmFeat; c < kWarmIn; c++) xd[qd * kWarmIn + c] = xs[qc * kWarmIn + c];
+#pragma unroll
+    for (int k = 0; k < kWarmOut; k++) Td[qd * kWarmOut + k] = Ts[qs * kWarmOut + k];
+    Dd[qd] = Ds[qs];
+  }
+}
+
 // ---- loss -----------------------------------------------------------------------
 
 constexpr int kLossStats = 117;  // s, u[8], G[36], C[64], tsum[8]
@@ -1207,9 +1243,20 @@ int warm_dataset_size(const Context& c) { return c.warm ? c.warm->ds_n : 0; }
 namespace {
 
 // Copies dataset images idx[0..nb) into the batch tensors (planes are
-// contiguous, so a batch is nb consecutive planes).
-void gather_batch(Context& c, const WarmState& w, const int* idx, int nb, const Net& n, float* T, float* Db) {
+// contiguous, so a batch is nb consecutive planes); with tf non-null, image i
+// goes through grid symmetry tf[i] (gather_sym_kernel).
+void gather_batch(Context& c, const WarmState& w, const int* idx, int nb, const Net& n, float* T, float* Db,
+                  const int* tf = nullptr) {
   const size_t plane = w.ds_geo.plane();
+  if (tf) {
+    const GeoD g = dev(w.ds_geo);
+    for (int i = 0; i < nb; i++) {
+      hipLaunchKernelGGL(gather_sym_kernel, dim3(grid_for((long long)g.h * g.w)), dim3(256), 0, c.stream, w.ds_x,
+                         w.ds_T, w.ds_Db, g, idx[i], n.x0.p, T, Db, i, tf[i]);
+      GPB_LAUNCH_CHECK();
+    }
+    return;
+  }
   for (int i = 0; i < nb; i++) {
     const size_t src = size_t(idx[i]) * plane;
     GPB_HIP(hipMemcpyAsync(n.x0.p + size_t(i) * plane * kWarmIn, w.ds_x + src * kWarmIn,
@@ -1223,11 +1270,13 @@ void gather_batch(Context& c, const WarmState& w, const int* idx, int nb, const 
 
 }  // namespace
 
-void warm_train(Context& c, int first, int count, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss) {
+void warm_train(Context& c, int first, int count, int epochs, int batch, float lr, uint64_t seed, int augment,
+                float* epoch_loss) {
   WarmState& w = state(c);
   if (!w.ready) throw ArgError("warm-start network not initialised");
   if (first < 0 || count < 1 || first + count > w.ds_n) throw ArgError("dataset range out of bounds");
   if (epochs < 1 || batch < 1) throw ArgError("epochs and batch must be positive");
+  if (augment && w.ds_geo.h != w.ds_geo.w) throw ArgError("symmetry augmentation needs square training images");
   batch = std::min(batch, count);
   Net n;
   alloc_forward(c, n, batch, w.ds_geo.h, w.ds_geo.w, nullptr);
@@ -1250,8 +1299,10 @@ void warm_train(Context& c, int first, int count, int epochs, int batch, float l
     for (size_t i = order.size(); i > 1; i--) std::swap(order[i - 1], order[size_t(rnd() % i)]);
     double sum = 0;
     int steps = 0;
+    std::vector<int> tf(static_cast<size_t>(batch));
     for (int b0 = 0; b0 + batch <= count; b0 += batch) {
-      gather_batch(c, w, order.data() + b0, batch, n, T, Db);
+      for (auto& t : tf) t = augment ? int(rnd() & 7) : 0;
+      gather_batch(c, w, order.data() + b0, batch, n, T, Db, augment ? tf.data() : nullptr);
       forward(c, n);
       sum += loss_and_grad(c, n, T, Db, &dy);
       GPB_HIP(hipMemsetAsync(grad, 0, param_total() * sizeof(float), c.stream));

@@ -43,9 +43,11 @@ void warm_dataset_reset(Context& c, int W, int H, int capacity);
 void warm_dataset_add(Context& c, const float* d_feat, const Stencil& st, const float* d_evecs);
 int warm_dataset_size(const Context& c);
 // Adam over dataset entries [first, first + count) for `epochs` passes of
-// shuffled mini-batches; epoch_loss[e] (may be null) receives the mean
+// shuffled mini-batches, each image under a random symmetry of the square
+// cell grid when augment != 0; epoch_loss[e] (may be null) receives the mean
 // training loss of epoch e.
-void warm_train(Context& c, int first, int count, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
+void warm_train(Context& c, int first, int count, int epochs, int batch, float lr, uint64_t seed, int augment,
+                float* epoch_loss);
 // Mean loss over dataset entries [first, first + count).
 double warm_eval_loss(Context& c, int first, int count);
 // Loss and parameter gradient for host inputs at cell resolution (hq, wq

@@ -29,6 +29,9 @@ std::map<int, gpb_ctx*> g_ctx;          // one context per device, kept for the 
 std::map<int, std::unique_ptr<std::mutex>> g_dev_mu;  // serialises calls per device
 std::vector<float> g_codebook;          // empty: per-image codebook
 int g_device = 0;
+std::vector<float> g_warm;              // warm-start parameters (empty: plain Lanczos)
+int g_warm_version = 0;
+std::map<gpb_ctx*, int> g_warm_loaded;  // parameter version loaded on each context
 
 struct DeviceLease {
   gpb_ctx* ctx;
@@ -156,6 +159,15 @@ void detmpb::Assign(std::vector<Mat*> (&cues)[12], Mat* textons) {
 void set_device(int device) {
   std::lock_guard<std::mutex> g(g_mu);
   g_device = device;
+}
+
+void set_warm_start(const float* params) {
+  std::lock_guard<std::mutex> g(g_mu);
+  if (params)
+    g_warm.assign(params, params + gpb_warm_param_count());
+  else
+    g_warm.clear();
+  g_warm_version++;
 }
 
 void set_texton_codebook(const float* codebook) {
@@ -358,6 +370,23 @@ void computeGPb(unsigned int rank, unsigned int width, unsigned int height, unsi
     if (!g || !all) throw std::runtime_error("gpb: out of host memory");
     DeviceLease d = lease(int(rank % unsigned(ndev)));
     set_eigs(d.ctx, 9);
+    {
+      std::vector<float> warm;
+      int version = 0, loaded = -1;
+      {
+        std::lock_guard<std::mutex> lk(g_mu);
+        version = g_warm_version;
+        auto it = g_warm_loaded.find(d.ctx);
+        if (it != g_warm_loaded.end()) loaded = it->second;
+        if (loaded != version) warm = g_warm;
+      }
+      if (loaded != version) {
+        if (!warm.empty()) check(gpb_warm_set_params(d.ctx, warm.data()), d.ctx);
+        check(gpb_set_warm_start(d.ctx, warm.empty() ? 0 : 1, 0, 0), d.ctx);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_warm_loaded[d.ctx] = version;
+      }
+    }
     check(gpb_compute(d.ctx, int(width), int(height), packed.data(), cb.data(), g, all, nullptr, nullptr), d.ctx);
     *hostGPb = g;
     *hostGPbAllConcat = all;

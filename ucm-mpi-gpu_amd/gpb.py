@@ -94,7 +94,8 @@ _sig("gpb_warm_dataset_reset", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int])
 _sig("gpb_warm_dataset_add", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p])
 _sig("gpb_warm_dataset_size", C.c_int, [_ctxp, C.POINTER(C.c_int)])
 _sig("gpb_warm_features", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, _f32p])
-_sig("gpb_warm_train", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_void_p])
+_sig("gpb_warm_train", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_int,
+                                 C.c_void_p])
 _sig("gpb_warm_eval_loss", C.c_int, [_ctxp, C.c_int, C.c_int, C.POINTER(C.c_double)])
 _sig("gpb_warm_loss_grad", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, C.POINTER(C.c_double),
                                      C.c_void_p, C.c_void_p])
@@ -396,10 +397,12 @@ class Context:
         self._chk(lib.gpb_warm_dataset_size(self.h, C.byref(n)))
         return n.value
 
-    def warm_train(self, first: int, count: int, epochs: int, batch: int, lr: float, seed: int) -> np.ndarray:
-        """Trains on training-set entries [first, first + count); returns the per-epoch mean loss."""
+    def warm_train(self, first: int, count: int, epochs: int, batch: int, lr: float, seed: int,
+                   augment: bool = False) -> np.ndarray:
+        """Trains on training-set entries [first, first + count) (augment: random grid symmetries);
+        returns the per-epoch mean loss."""
         loss = np.empty(epochs, np.float32)
-        self._chk(lib.gpb_warm_train(self.h, first, count, epochs, batch, lr, seed, loss.ctypes.data))
+        self._chk(lib.gpb_warm_train(self.h, first, count, epochs, batch, lr, seed, int(augment), loss.ctypes.data))
         return loss
 
     def warm_eval_loss(self, first: int, count: int) -> float:
