@@ -262,12 +262,12 @@ __global__ __launch_bounds__(256) void britz_kernel(const float* __restrict__ V,
 #pragma unroll
       for (int c = 0; c < NC; c++) acc[c] = 0.0;
       int i = 0;
-      for (; i + 4 <= m; i += 4) {  // four loads in flight before the FMAs
-        double v[4];
+      for (; i + 16 <= m; i += 16) {  // sixteen loads in flight before the FMAs
+        double v[16];
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[q] = V[size_t(i + q) * ld + p];
+        for (int q = 0; q < 16; q++) v[q] = V[size_t(i + q) * ld + p];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < 16; q++) {
           const double* yr = ys + size_t(i + q) * nk + c0;
 #pragma unroll
           for (int c = 0; c < NC; c++) acc[c] = fma(yr[c0 + c < nk ? c : 0], v[q], acc[c]);

@@ -258,13 +258,48 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
   __syncthreads();
   const int ty = lane / kCueTile, tx = lane - ty * kCueTile;
   const uint8_t* centre = tile + (ty + kCueRmax) * kCueTileW + tx + kCueRmax;
+  const __attribute__((address_space(4))) int* offc = (const __attribute__((address_space(4))) int*)P.off;
+  const __attribute__((address_space(4))) uint8_t* slc = (const __attribute__((address_space(4))) uint8_t*)P.slice;
   CT chf[kNumOri];
 #pragma unroll
   for (int o = 0; o < kNumOri; o++) chf[o] = CT(0);
   int i = 0;
   for (int ri = 0; ri < 3; ri++) {
     const int end = P.end_by_r[radii[ri]];
-#pragma unroll 8
+    // 8 label reads are issued before their 8 histogram updates (read and
+    // update one at a time, each read waited behind the previous update: the
+    // compiler cannot separate the two LDS arrays here); the uniform offset
+    // and slice tables come through the scalar cache, the next batch's
+    // entries loaded before this batch's LDS work
+    if (i + 8 <= end) {
+      int off_n[8], sl_n[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        off_n[k] = offc[i + k];
+        sl_n[k] = slc[i + k];
+      }
+      for (; i + 8 <= end; i += 8) {
+        int off_c[8], sl_c[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          off_c[k] = off_n[k];
+          sl_c[k] = sl_n[k];
+        }
+        if (i + 16 <= end) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            off_n[k] = offc[i + 8 + k];
+            sl_n[k] = slc[i + 8 + k];
+          }
+        }
+        uint32_t lab[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) lab[k] = centre[off_c[k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          atomicAdd(&hist[((sl_c[k] >> 2) * kNbMax + lab[k]) * kCueLanes + lane], 1u << (8 * (sl_c[k] & 3)));
+      }
+    }
     for (; i < end; i++) {
       const int s = P.slice[i];
       const uint32_t lab = centre[P.off[i]];

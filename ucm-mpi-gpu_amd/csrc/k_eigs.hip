@@ -435,8 +435,8 @@ __global__ __launch_bounds__(256) void scale_kernel(const float* __restrict__ w,
 // out[i] = sum_j Y[j][i] V[j], i < nk, j < m <= kRitzMaxM (FP64
 // accumulation), kRitzCols outputs per pass over V.  Y is read with
 // wave-uniform indices, so its values come through the scalar cache and feed
-// the FMAs as scalar operands; the basis is loaded 8 rows at a time, all 8
-// loads issued before use.
+// the FMAs as scalar operands; the basis is loaded 16 rows at a time, all 16
+// loads issued before use (memory-level parallelism bounds this kernel).
 constexpr int kRitzMaxM = 344;
 constexpr int kRitzCols = 24;
 
@@ -449,12 +449,12 @@ __global__ __launch_bounds__(256) void ritz_kernel(const float* __restrict__ V, 
       double acc[kRitzCols];
 #pragma unroll
       for (int i = 0; i < kRitzCols; i++) acc[i] = 0;
-      for (int j0 = 0; j0 < m; j0 += 8) {
-        float v[8];
+      for (int j0 = 0; j0 < m; j0 += 16) {
+        float v[16];
 #pragma unroll
-        for (int t = 0; t < 8; t++) v[t] = V[size_t(j0 + t < m ? j0 + t : 0) * ld + p];
+        for (int t = 0; t < 16; t++) v[t] = V[size_t(j0 + t < m ? j0 + t : 0) * ld + p];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
+        for (int t = 0; t < 16; t++) {
           if (j0 + t >= m) break;  // uniform
           const double* y = Y + size_t(j0 + t) * nk + i0;
           const double vt = double(v[t]);
