@@ -246,46 +246,6 @@ __global__ __launch_bounds__(256) void bwnorm_kernel(const float* __restrict__ X
   if (threadIdx.x == 0) partial[size_t(i) * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// out_c = sum_i Y[i][c] V_i for c < nk (Ritz recombination, FP64
-// accumulation): Y (m x nk) sits in LDS; each pass over V produces NC output
-// columns, so V is read ceil(nk / NC) times.
-template <int NC>
-__global__ __launch_bounds__(256) void britz_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
-                                                    const double* __restrict__ Y, float* __restrict__ out, size_t ldo,
-                                                    size_t Np) {
-  extern __shared__ double ys[];  // [m][nk]
-  for (int t = threadIdx.x; t < m * nk; t += blockDim.x) ys[t] = Y[t];
-  __syncthreads();
-  for (int c0 = 0; c0 < nk; c0 += NC) {
-    for (size_t p = size_t(blockIdx.x) * 256 + threadIdx.x; p < Np; p += size_t(gridDim.x) * 256) {
-      double acc[NC];
-#pragma unroll
-      for (int c = 0; c < NC; c++) acc[c] = 0.0;
-      int i = 0;
-      for (; i + 16 <= m; i += 16) {  // sixteen loads in flight before the FMAs
-        double v[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) v[q] = V[size_t(i + q) * ld + p];
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-          const double* yr = ys + size_t(i + q) * nk + c0;
-#pragma unroll
-          for (int c = 0; c < NC; c++) acc[c] = fma(yr[c0 + c < nk ? c : 0], v[q], acc[c]);
-        }
-      }
-      for (; i < m; i++) {
-        const double v = V[size_t(i) * ld + p];
-        const double* yr = ys + size_t(i) * nk + c0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] = fma(yr[c0 + c < nk ? c : 0], v, acc[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < NC; c++)
-        if (c0 + c < nk) out[size_t(c0 + c) * ldo + p] = float(acc[c]);
-    }
-  }
-}
-
 // In place X_r <- sum_s X_s Rinv[s][r] (upper triangular Rinv), FP64.
 template <int B>
 __global__ __launch_bounds__(256) void btransform_kernel(float* __restrict__ X, size_t ld, const double* __restrict__ Rinv,
@@ -501,23 +461,7 @@ struct BlockOps {
     GPB_LAUNCH_CHECK();
   }
   // out_c = sum_i Y[i][c] V_i, c < nk
-  void ritz(const float* V, int m, int nk, const double* Ydev, float* out) {
-    const size_t lds = sizeof(double) * size_t(m) * nk;
-    if (lds > 64 * 1024 || nk > 64) {  // Y does not fit the LDS budget: the general kernel
-      launch_ritz(c, V, Np, m, nk, Ydev, out, Np);
-      return;
-    }
-    const int nc = nk <= 24 ? 24 : (nk <= 48 ? 48 : 64);
-    KScope ks(c, kKRitz, double(Np) * 4.0 * (double(m) * div_up(nk, nc) + nk));
-    const dim3 grid(std::min<unsigned>(2048, div_up(Np, 256)));
-    if (nc == 24)
-      hipLaunchKernelGGL(britz_kernel<24>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
-    else if (nc == 48)
-      hipLaunchKernelGGL(britz_kernel<48>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
-    else
-      hipLaunchKernelGGL(britz_kernel<64>, grid, dim3(256), lds, c.stream, V, Np, m, nk, Ydev, out, Np, Np);
-    GPB_LAUNCH_CHECK();
-  }
+  void ritz(const float* V, int m, int nk, const double* Ydev, float* out) { launch_ritz(c, V, Np, m, nk, Ydev, out, Np); }
   // dst[i] = ||D^1/2 X_i||^2, i < n
   void wnorms(const float* X, int n, double* dst) {
     {

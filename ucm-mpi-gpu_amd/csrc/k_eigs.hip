@@ -470,6 +470,69 @@ __global__ __launch_bounds__(256) void ritz_kernel(const float* __restrict__ V, 
   }
 }
 
+// out_c = sum_j Y[j][c] V_j for c < nk with Y staged in LDS, zero-padded to
+// ncp = nk rounded up to NC columns, so that each row's NC coefficients are
+// read unconditionally as 16-byte pairs.  LDS bandwidth bounds a broadcast
+// read per FMA, so each coefficient pair read serves PX pixels per thread
+// (pixels p0 + k * stride, coalesced per k); 4 basis rows x PX pixels of
+// loads are in flight before their FMAs.  One pass over V per NC columns.
+template <int NC, int PX>
+__global__ __launch_bounds__(256) void ritz_lds_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
+                                                       const double* __restrict__ Y, float* __restrict__ out,
+                                                       size_t ldo, size_t Np) {
+  extern __shared__ double ys[];  // [m][ncp]
+  const int ncp = (nk + NC - 1) / NC * NC;
+  for (int t = threadIdx.x; t < m * ncp; t += blockDim.x) {
+    const int j = t / ncp, c = t - j * ncp;
+    ys[t] = c < nk ? Y[size_t(j) * nk + c] : 0.0;
+  }
+  __syncthreads();
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (int c0 = 0; c0 < nk; c0 += NC) {
+    for (size_t p0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p0 < Np; p0 += stride * PX) {
+      size_t pp[PX];
+#pragma unroll
+      for (int k = 0; k < PX; k++) {
+        const size_t p = p0 + size_t(k) * stride;
+        pp[k] = p < Np ? p : Np - 1;
+      }
+      double acc[PX][NC];
+#pragma unroll
+      for (int k = 0; k < PX; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[k][c] = 0.0;
+      for (int j0 = 0; j0 < m; j0 += 4) {
+        float v[4][PX];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int k = 0; k < PX; k++) v[t][k] = V[size_t(j0 + t < m ? j0 + t : m - 1) * ld + pp[k]];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          if (j0 + t >= m) break;  // uniform
+          const double2* yr = reinterpret_cast<const double2*>(ys + size_t(j0 + t) * ncp + c0);
+#pragma unroll
+          for (int c2 = 0; c2 < NC / 2; c2++) {
+            const double2 y2 = yr[c2];
+#pragma unroll
+            for (int k = 0; k < PX; k++) {
+              acc[k][2 * c2] = fma(y2.x, double(v[t][k]), acc[k][2 * c2]);
+              acc[k][2 * c2 + 1] = fma(y2.y, double(v[t][k]), acc[k][2 * c2 + 1]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PX; k++) {
+        if (p0 + size_t(k) * stride >= Np) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+          if (c0 + c < nk) out[size_t(c0 + c) * ldo + pp[k]] = float(acc[k][c]);
+      }
+    }
+  }
+}
+
 // Interior-only helpers
 __global__ __launch_bounds__(256) void sqrt_degree_kernel(const float* __restrict__ D, float* __restrict__ v, size_t Np) {
   for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
@@ -513,9 +576,22 @@ void launch_reduce(Context& c, const double* partial, int G, int n, double* dst)
 
 void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np) {
   if (m > kRitzMaxM) throw ArgError("basis too large for the Ritz recombination");
+  const dim3 grid(std::min<unsigned>(2048, div_up(Np, 256)));
+  const int nc = nk <= 16 ? 16 : (nk <= 24 ? 24 : 32);
+  const size_t lds = sizeof(double) * size_t(m) * size_t((nk + nc - 1) / nc * nc);
+  if (lds <= 64 * 1024) {
+    KScope ks(c, kKRitz, double(Np) * 4.0 * (double(m) * div_up(nk, nc) + nk));
+    if (nc == 16)
+      hipLaunchKernelGGL((ritz_lds_kernel<16, 4>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
+    else if (nc == 24)
+      hipLaunchKernelGGL((ritz_lds_kernel<24, 4>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
+    else
+      hipLaunchKernelGGL((ritz_lds_kernel<32, 2>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
+    GPB_LAUNCH_CHECK();
+    return;
+  }
   KScope ks(c, kKRitz, double(Np) * 4.0 * (m + nk));
-  hipLaunchKernelGGL(ritz_kernel, dim3(std::min<unsigned>(2048, div_up(Np, 256))), dim3(256), 0, c.stream, V, ld, m, nk,
-                     Ydev, out, Np);
+  hipLaunchKernelGGL(ritz_kernel, grid, dim3(256), 0, c.stream, V, ld, m, nk, Ydev, out, Np);
   GPB_LAUNCH_CHECK();
 }
 
