@@ -70,9 +70,13 @@ struct EigOptions {
   int max_matvecs = 20000;
 };
 // Generalised eigenpairs of (D - W, D): evals ascending (host), evecs [K][Np]
-// padded layout, generalised normalisation v^T D v = 1.
+// padded layout, generalised normalisation v^T D v = 1.  start_evecs
+// (optional, [start_count][Np] generalised vectors, padded, may alias d_evecs)
+// replaces the random starting vector by D^1/2 times the sum of vectors
+// 1 .. start_count-1: approximate eigenvectors from another solve, whose span
+// the first restart cycle then recovers.
 void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, double* evals, float* d_evecs,
-                   LanczosStats& stats);
+                   LanczosStats& stats, const float* start_evecs = nullptr, int start_count = 0);
 // block form: nvec > 1 columns per Lanczos step (block thick-restart Lanczos);
 // init_block (optional, [block][Np] padded, generalised or normalised-Laplacian
 // vectors in u-space) replaces the random starting block.
@@ -83,6 +87,17 @@ struct BlockOptions {
 void block_lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt, const BlockOptions& bo, double* evals,
                          float* d_evecs, LanczosStats& stats);
 void debug_block_spmv(Context& c, const Stencil& st, int B, const float* X, float* Y);
+// FP64 Rayleigh-Ritz on the span of K returned pairs (generalised vectors,
+// padded): refined eigenvalues (ascending) and rotated vectors, in place.
+void refine_pairs(Context& c, const Stencil& st, double* evals, float* d_evecs, int K);
+// Residual targets below this get the FP64 refinement: there the FP32
+// storage of the Krylov vectors, not the target, limits the eigenvalues.
+constexpr double kRefineBelowTol = 1e-4;
+// Warm-started solves with a target below this run the block phase to this
+// residual and finish with the single-vector solver started from the block's
+// Ritz vectors: block Lanczos with a few block steps per restart converges
+// slowly once the wanted pairs are resolved, single-vector Lanczos does not.
+constexpr double kWarmHandoverTol = 1e-3;
 // Measured relative generalised residuals of K <= 9 returned pairs; returns
 // the largest (resid, optional, receives all K).
 double measured_max_residual(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K,

@@ -110,6 +110,7 @@ void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* eva
   const size_t N = size_t(st.W) * st.H;
   if (c.eig_block <= 1 || N < size_t(64) * eo.m) {
     lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
+    if (eo.tol < kRefineBelowTol) refine_pairs(c, st, evals, d_evecs, eo.K);
     return;
   }
   BlockOptions bo;
@@ -122,15 +123,19 @@ void solve_eigs(Context& c, const Stencil& st, const EigOptions& eo, double* eva
     c.last_lanczos.fallback = 2;
     c.last_lanczos.iterations += spent;
   }
+  if (eo.tol < kRefineBelowTol) refine_pairs(c, st, evals, d_evecs, eo.K);
 }
 
 // Warm-started eigensolve: block Lanczos with 8 columns (plus the locked
 // D^1/2 1, 9 vectors) from the network's predicted block (mode 1) or from a
-// random block (mode 2).  Plain single-vector Lanczos from its own start is
-// the fallback whenever the block solve breaks down or stops short, or the
-// measured residual ||(D - W)v - lambda D v|| / ||D v|| of a returned pair
-// exceeds the target (stats.fallback = 1; the operator applications of both
-// solves are counted).
+// random block (mode 2).  Single-vector Lanczos is the fallback whenever the
+// block solve breaks down or stops short, or the measured residual
+// ||(D - W)v - lambda D v|| / ||D v|| of a returned pair exceeds the target
+// (stats.fallback = 1); it starts from the block solve's last Ritz vectors
+// when there are any.  Targets below kWarmHandoverTol run the block phase to
+// that residual only and always finish with the single-vector solver from
+// the block's Ritz vectors (stats.fallback = 0 when the block phase reached
+// its residual).  The operator applications of both solves are counted.
 void solve_eigs_warm(Context& c, const Stencil& st, const EigOptions& eo, const float* d_feat, double* evals,
                      float* d_evecs) {
   if (eo.K != 9) throw ArgError("the warm start computes 9 eigenpairs (nvec = 9)");
@@ -158,24 +163,35 @@ void solve_eigs_warm(Context& c, const Stencil& st, const EigOptions& eo, const 
     c.warm_ms = ms;
     bo.init_block = X0;
   }
-  bool ok = false;
+  const bool polish = eo.tol < kWarmHandoverTol;
+  if (polish) be.tol = kWarmHandoverTol;
+  bool ok = false, have_vectors = false;
   int spent = 0;
   try {
     block_lanczos_solve(c, st, be, bo, evals, d_evecs, c.last_lanczos);
     spent = c.last_lanczos.iterations;
-    const double r = measured_max_residual(c, st, d_evecs, evals, eo.K);
-    c.last_lanczos.max_residual = r;
-    ok = r <= eo.tol;
+    have_vectors = true;
+    if (polish) {
+      ok = true;
+    } else {
+      const double r = measured_max_residual(c, st, d_evecs, evals, eo.K);
+      c.last_lanczos.max_residual = r;
+      ok = r <= eo.tol;
+    }
   } catch (const BlockBreakdown&) {
     spent = c.last_lanczos.iterations;
   } catch (const NoConvergence&) {
     spent = c.last_lanczos.iterations;
+    have_vectors = true;  // the solver returns its last Ritz pairs before it throws
   }
-  if (!ok) {
-    lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos);
-    c.last_lanczos.fallback = 1;
+  if (polish || !ok) {
+    // single-vector Lanczos from the block's Ritz vectors (or from its own
+    // random start when the block broke down before producing any)
+    lanczos_solve(c, st, eo, evals, d_evecs, c.last_lanczos, have_vectors ? d_evecs : nullptr, eo.K);
+    c.last_lanczos.fallback = ok ? 0 : 1;
     c.last_lanczos.iterations += spent;
   }
+  if (eo.tol < kRefineBelowTol) refine_pairs(c, st, evals, d_evecs, eo.K);
 }
 
 static EigOptions ctx_eig_options(const Context& c) {

@@ -632,6 +632,9 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
       worst = std::max(worst, resid[size_t(i)]);
     }
     stats.max_residual = worst;
+    if (c.verbose)
+      std::fprintf(stderr, "gpb: block Lanczos %dx%d (block %d): %d operator applications, residual %.3e\n", st.W,
+                   st.H, B, matvecs, worst);
     converged = worst <= opt.tol;
     if (converged || matvecs >= opt.max_matvecs) break;
     std::vector<double> T0(size_t(M) * M, 0.0);

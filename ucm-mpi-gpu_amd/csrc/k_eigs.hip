@@ -555,6 +555,18 @@ __global__ __launch_bounds__(256) void random_start_kernel(float* __restrict__ v
   }
 }
 
+// v = D^1/2 sum_{i=1}^{n-1} e_i: the u-space image of approximate generalised
+// eigenvectors e_i (the trivial e_0 is left out: the solver locks D^1/2 1).
+__global__ __launch_bounds__(256) void combine_start_kernel(const float* __restrict__ ev, int n,
+                                                            const float* __restrict__ D, float* __restrict__ v,
+                                                            size_t Np) {
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    float a = 0.f;
+    for (int i = 1; i < n; i++) a += ev[size_t(i) * Np + p];
+    v[p] = D[p] > 0.f ? sqrtf(D[p]) * a : 0.f;
+  }
+}
+
 __global__ __launch_bounds__(256) void scale_by_kernel(float* __restrict__ v, const double* __restrict__ nrm2, size_t Np) {
   const double ib = 1.0 / sqrt(nrm2[0]);
   for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
@@ -682,7 +694,7 @@ unsigned vec_grid(size_t Np) { return std::min<unsigned>(2048, div_up(Np, 256));
 }  // namespace
 
 void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, double* evals, float* d_evecs,
-                   LanczosStats& stats) {
+                   LanczosStats& stats, const float* start_evecs, int start_count) {
   EigOptions opt = opt_in;
   const size_t Np = st.Np;
   const size_t N = size_t(st.W) * st.H;
@@ -727,10 +739,13 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   red.dots(V, Np, 1, w, nullptr, false, h);
   hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(64), 0, c.stream, T, m, 0, 0, nullptr, h, h, nullptr, h, h, nullptr);
   GPB_LAUNCH_CHECK();
-  // V1: random, orthogonal to V0, normalised
+  // V1: random (or the given vectors' combination), orthogonal to V0, normalised
   float* V1 = V + Np;
-  hipLaunchKernelGGL(random_start_kernel, dim3(vg), dim3(256), 0, c.stream, V1, st.W, st.H, st.Wp, Np,
-                     uint64_t(0x5eed1234abcdull));
+  if (start_evecs && start_count > 1)
+    hipLaunchKernelGGL(combine_start_kernel, dim3(vg), dim3(256), 0, c.stream, start_evecs, start_count, st.D, V1, Np);
+  else
+    hipLaunchKernelGGL(random_start_kernel, dim3(vg), dim3(256), 0, c.stream, V1, st.W, st.H, st.Wp, Np,
+                       uint64_t(0x5eed1234abcdull));
   GPB_LAUNCH_CHECK();
   for (int pass = 0; pass < 2; pass++) {
     red.dots(V, Np, 1, V1, nullptr, false, h);
