@@ -153,8 +153,10 @@ def main():
     ap.add_argument("--block", type=int, default=1, help="Lanczos block size (1 = single-vector)")
     ap.add_argument("--basis", type=int, default=0)
     ap.add_argument("--keep", type=int, default=0)
-    ap.add_argument("--spmv-tile", type=int, default=0, help="SpMV rows per thread (2, 4, 8); 0 = autotune")
+    ap.add_argument("--spmv-tile", type=int, default=0, help="SpMV tile code 10 * wavefronts + rows per thread (42, 44, 48, 82, 84, 161, 162); 0 = autotune")
     ap.add_argument("--spmv-fp32", action="store_true", help="SpMV arithmetic in FP32 (default FP64 arithmetic)")
+    ap.add_argument("--full-reorth", action="store_true",
+                    help="whole-basis Gram-Schmidt on every Lanczos step (default: partial reorthogonalisation)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without per-kernel HIP events")
     ap.add_argument("--forms", default="batched,warm",
                     help="comma list of extra forms to time: batched (C4), warm (learned warm start), none")
@@ -198,6 +200,7 @@ def main():
     ctx.set_block_size(args.block)
     ctx.set_spmv_tile(args.spmv_tile)
     ctx.set_spmv_fp64(not args.spmv_fp32)
+    ctx.set_reorth(not args.full_reorth)
 
     def step():
         ctx.compute_device(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), d_gmax.data_ptr(), d_gall.data_ptr())
@@ -242,7 +245,7 @@ def main():
                             form="plain", lanczos_block=st["block"], lanczos_basis=st["basis"],
                             lanczos_matvecs=st["matvecs"],
                             lanczos_restarts=st["restarts"], reorth_second_passes=st["second_passes"],
-                            max_residual=st["max_residual"], spmv_tile_rows=ctx.spmv_tile(S, S), spmv_arith="f32" if args.spmv_fp32 else "f64",
+                            max_residual=st["max_residual"], spmv_tile=ctx.spmv_tile(S, S), spmv_arith="f32" if args.spmv_fp32 else "f64",
                             stage_ms=[round(x, 2) for x in st["stage_ms"]]),
                 roofline=roofline,
                 kernels={k: dict(ms=round(v["ms"] / args.steps, 3), launches=v["launches"] // args.steps,

@@ -178,14 +178,15 @@ int gpb_warm_eval_loss(gpb_ctx* ctx, int first, int count, double* loss);
 int gpb_warm_loss_grad(gpb_ctx* ctx, int nimg, int hq, int wq, const float* x, const float* targets,
                        const float* dweights, double* loss, float* grad, float* y);
 
-/* SpMV tile for later solves on ctx: rows = 2, 4 or 8 rows per thread
- * (tiles of 64 x 4*rows pixels), or 0 (default) to time the three variants on
- * the first solve of each image shape and keep the fastest.
- * gpb_get_spmv_tile reports the tile in use for a shape (0 = not tuned yet).
- * No counterpart in the reference (its SpMV has one fixed launch shape,
- * damascene/src/stencilMVM/stencilMVM.cu:129-182). */
-int gpb_set_spmv_tile(gpb_ctx* ctx, int rows);
-int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows);
+/* SpMV tile for later solves on ctx: tile = 10 * ny + rows, a workgroup of
+ * 64 x ny threads (ny wavefronts) with rows rows per thread covering 64 x
+ * ny*rows pixels -- one of 42, 44, 48, 82, 84, 161, 162 -- or 0 (default) to
+ * time every variant on the first solve of each image shape and keep the
+ * fastest.  gpb_get_spmv_tile reports the tile in use for a shape (0 = not
+ * tuned yet).  No counterpart in the reference (its SpMV has one fixed launch
+ * shape, damascene/src/stencilMVM/stencilMVM.cu:129-182). */
+int gpb_set_spmv_tile(gpb_ctx* ctx, int tile);
+int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* tile);
 
 /* Arithmetic of the stencil SpMV for later solves on ctx: 1 (default) = the
  * differences, products and row sums in FP64, 0 = in FP32.  Operands (W, x,
@@ -195,6 +196,15 @@ int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows);
  * reference SpMVs are FP32 (damascene/src/stencilMVM/stencilMVM.cu:129-182)
  * and FP64 (Spectra, src/spectralPb.cpp:221-244). */
 int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
+
+/* Reorthogonalisation of the single-vector Lanczos steps for later solves on
+ * ctx: 1 (default) = partial (Simon's omega recurrence on the device: the
+ * whole-basis Gram-Schmidt pass runs only when the estimated loss of
+ * orthogonality exceeds sqrt(2^-24)) for residual targets >= 1e-4, and the
+ * whole-basis pass every step below that; 0 = the whole-basis pass every
+ * step.  No counterpart in the reference (Spectra reorthogonalises fully,
+ * SymEigsSolver.h:200-280; damascene not at all, lanczos.cu:192-282). */
+int gpb_set_reorth(gpb_ctx* ctx, int mode);
 
 /* level > 0: the eigensolvers write one progress line per restart cycle to
  * stderr (operator applications, residual); 0 (default): silent. */
@@ -262,6 +272,18 @@ int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, con
  * any may be NULL); returns after the stream is synchronised. */
 int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
                        float* d_gpb_max, float* d_gpb_all, float* d_gpb_orient);
+
+/* The spectral result of the last gpb_compute / gpb_compute_device call on
+ * this context (width, height: that call's, checked): evecs[nvec][H][W]
+ * (nvec <= the solve's nvec; generalised
+ * eigenvectors, v^T D v = 1, eigenvalue order ascending as gpb_get_stats
+ * reports them) and degree[H][W] = D = 1 + sum of the pixel's affinities;
+ * either may be NULL.  Any later call that builds an affinity (eigs,
+ * build_w, spectral_pb, the warm-start training set) replaces it and this
+ * call then fails.  No counterpart in the reference (computeGPb keeps its
+ * eigenvectors internal): it serves evaluation code that compares the
+ * eigenvectors of two solves. */
+int gpb_last_eigenvectors(gpb_ctx* ctx, int width, int height, int nvec, float* evecs, float* degree);
 
 /* ---- patches: the batched form --------------------------------------------------- */
 

@@ -87,3 +87,30 @@ def test_c1_whole_path_gpu(gpb, ctx, c1, warm):
         ang = _angle(vec[1:], ref["evecs"][1:], ref["D"])
         print(f"  subspace angle {ang:.2e} rad", flush=True)
         assert ang <= 1e-3
+
+
+@pytest.mark.gpu
+def test_last_eigenvectors_gpu(gpb, ctx):
+    """The whole path's eigenvectors and degree read back after compute are the
+    ones the same solve returns from the path's own mPb_nmax (the same
+    deterministic solver on the same stencil), and a later call that builds an
+    affinity invalidates them."""
+    W, H = 96, 72
+    rgb, packed = synth_packed(gpb, W, H, regions=12, sigma=6.0, seed=4)
+    cb = gpb.make_codebook(packed, seed=4, iters=10)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=1e-5)
+    out = ctx.compute(packed, cb)
+    vec, deg = ctx.last_eigenvectors(W, H, 9)
+    with pytest.raises(gpb.GpbError):
+        ctx.last_eigenvectors(W + 1, H, 9)
+    with pytest.raises(gpb.GpbError):
+        ctx.last_eigenvectors(W, H, 10)
+    _, D = ctx.build_w(out["mpb_nmax"])
+    assert np.array_equal(deg, D)
+    with pytest.raises(gpb.GpbError):
+        ctx.last_eigenvectors(W, H, 9)
+    _, vec2 = ctx.eigs(out["mpb_nmax"])
+    assert np.abs(vec - vec2).max() <= 1e-6
+    # generalised normalisation v^T D v = 1 and D-orthogonality
+    G = np.einsum("ihw,jhw,hw->ij", vec.astype(np.float64), vec.astype(np.float64), D.astype(np.float64))
+    assert np.abs(G - np.eye(9)).max() < 1e-4

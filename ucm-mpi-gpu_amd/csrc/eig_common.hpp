@@ -26,8 +26,10 @@ struct SpmvOffsets {
 // against 1).  W is read once per unordered pair: W[k][p] for q = p + e_k and
 // W[k][p - e_k] for q = p - e_k.  A 64x16 tile of u (plus a 5-pixel halo) is
 // staged in LDS; W streams from HBM with coalesced row reads.
-// Tiles are 64 columns by 4 * ROWS rows (64x4 threads, ROWS rows each); ROWS
-// is picked per image shape by timing (tune_spmv).
+// Tiles are 64 columns by NY * ROWS rows (64 x NY threads, NY wavefronts of
+// one 64-pixel row each, ROWS rows per thread); the shape is picked per image
+// by timing (tune_spmv).  Taller tiles re-read fewer lower-half weights
+// W[k][p - e_k] from rows above the tile (5 of every NY * ROWS rows).
 constexpr int kSpmvTX = 64;
 constexpr int kSpmvTileW = kSpmvTX + 2 * kICRadius;
 
@@ -36,11 +38,11 @@ struct SpmvArgs {
   int lo[kStencilHalf];  // LDS tile offsets
 };
 
-// One 64 x 4*ROWS tile of y = L x at interior origin (x0, y0); returns this
-// thread's share of x . y over the tile.  AT is the arithmetic type of the
+// One 64 x NY*ROWS tile of y = L x at interior origin (x0, y0) (blockDim =
+// 64 x NY); returns this thread's share of x . y over the tile.  AT is the arithmetic type of the
 // differences, products and sum (float, or double with the FP32 operands
 // widened exactly); W, x, s and y stay FP32 in HBM either way.  u: LDS tile
-// of (4*ROWS+10) x 74 values of s .* x in AT.
+// of (NY*ROWS+10) x 74 values of s .* x in AT.
 // Pointers known to address device global memory (HBM).  Loads through
 // them are global_load instructions; through a generic pointer read from a
 // table they would be flat loads, which also count against the LDS wait
@@ -57,7 +59,7 @@ __device__ __forceinline__ gptr<T> as_global_mut(T* p) {
 }
 
 
-template <int ROWS, typename AT>
+template <int ROWS, typename AT, int NY = 4>
 __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, const float* __restrict__ s_,
                                             const float* __restrict__ x_, float* __restrict__ y_, int W, int H, int Wp,
                                             int Hp, size_t Np, const SpmvArgs A, int x0, int y0, AT* __restrict__ u) {
@@ -65,9 +67,9 @@ __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, cons
   const gptr<const float> s = as_global(s_);
   const gptr<const float> x = as_global(x_);
   const gptr<float> y = as_global_mut(y_);
-  constexpr int kSpmvTY = 4 * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+  constexpr int kSpmvTY = NY * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
   const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
-  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * 4) {
+  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * NY) {
     const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
     const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
     AT v = 0;
@@ -83,7 +85,7 @@ __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, cons
   if (xi < W) {
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
-      const int ty = threadIdx.y + 4 * r;
+      const int ty = threadIdx.y + NY * r;
       const int yi = y0 + ty;
       if (yi >= H) break;
       const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
@@ -110,6 +112,20 @@ __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, cons
     }
   }
   return xy;
+}
+
+// Block sum of one double per thread over NW wavefronts (blockDim.x * y =
+// 64 * NW); every thread gets the result.
+template <int NW>
+__device__ __forceinline__ double block_sum_waves(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double t = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) t += red[w];
+  return t;
 }
 
 // Block sum of one double per thread (256 threads); result in thread 0.

@@ -150,6 +150,9 @@ struct Context {
   LanczosStats last_lanczos;
   KernelTimes kt;
   double last_evals[kMaxNvec] = {0};
+  // dimensions of the last whole-path run whose eigenvectors ("run.evecs")
+  // and degree ("st.D") are still in the workspace; 0 once replaced
+  int last_run_W = 0, last_run_H = 0, last_run_K = 0;
   double stage_ms[8] = {0};
   // eigensolver options (gpb_set_eig_options)
   int eig_K = 9;
@@ -158,6 +161,9 @@ struct Context {
   double eig_tol = 1e-3;
   int eig_max_matvecs = 20000;
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
+  // single-vector Lanczos steps: 1 (default) partial reorthogonalisation for
+  // residual targets >= 1e-4, 0 whole-basis Gram-Schmidt every step
+  int reorth_mode = 1;
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
   std::map<uint64_t, int> spmv_band;  // tile-band width tuned with the rows (tiles)

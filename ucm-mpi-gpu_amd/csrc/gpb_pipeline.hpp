@@ -110,7 +110,10 @@ void launch_to_general(Context& c, const float* u, const float* s, float* v, siz
 void launch_scale_by(Context& c, float* v, const double* nrm2, size_t Np);
 // y = D^-1/2 (D - W) D^-1/2 x on padded vectors
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y);
+// SpMV tile code (10 * wavefronts + rows per thread) for this shape, timed
+// over every variant the first time the shape is seen.
 int tune_spmv(Context& c, const Stencil& st);
+bool spmv_tile_valid(int code);
 
 // stage (e)
 void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb /* [8][N] */);

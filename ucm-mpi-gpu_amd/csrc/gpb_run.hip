@@ -99,6 +99,9 @@ static void run_pipeline(Context& c, const DeviceImage& im, const EigOptions& eo
     GPB_HIP(hipEventElapsedTime(&ms, se.ev[s], se.ev[s + 1]));
     c.stage_ms[s] = ms;
   }
+  c.last_run_W = W;
+  c.last_run_H = H;
+  c.last_run_K = eo.K;
   if (keep_buffers) *keep_buffers = b;
 }
 
@@ -359,10 +362,18 @@ int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol,
   });
 }
 
-int gpb_set_spmv_tile(gpb_ctx* ctx, int rows) {
+int gpb_set_spmv_tile(gpb_ctx* ctx, int tile) {
   return guarded2(ctx, [&] {
-    if (!(rows == 0 || rows == 2 || rows == 4 || rows == 8)) throw ArgError("SpMV tile rows must be 0, 2, 4 or 8");
-    ctx->c.spmv_rows_forced = rows;
+    if (!(tile == 0 || spmv_tile_valid(tile)))
+      throw ArgError("SpMV tile must be 0 or one of 42, 44, 48, 82, 84, 161, 162");
+    ctx->c.spmv_rows_forced = tile;
+  });
+}
+
+int gpb_set_reorth(gpb_ctx* ctx, int mode) {
+  return guarded2(ctx, [&] {
+    if (mode != 0 && mode != 1) throw ArgError("reorthogonalisation mode must be 0 (every step) or 1 (partial)");
+    ctx->c.reorth_mode = mode;
   });
 }
 
@@ -373,16 +384,16 @@ int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64) {
   });
 }
 
-int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* rows) {
+int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* tile) {
   return guarded2(ctx, [&] {
-    if (!rows) throw ArgError("null pointer");
+    if (!tile) throw ArgError("null pointer");
     if (ctx->c.spmv_rows_forced) {
-      *rows = ctx->c.spmv_rows_forced;
+      *tile = ctx->c.spmv_rows_forced;
       return;
     }
     auto it = ctx->c.spmv_rows.find(((uint64_t(width) << 32) | uint32_t(height)) ^
                                     (ctx->c.spmv_fp64 ? (1ull << 62) : 0));
-    *rows = it == ctx->c.spmv_rows.end() ? 0 : it->second;
+    *tile = it == ctx->c.spmv_rows.end() ? 0 : it->second;
   });
 }
 
@@ -590,6 +601,20 @@ int gpb_build_w(gpb_ctx* ctx, int width, int height, const float* mpb_nmax, floa
     build_stencil(c, d_m, width, height, st);
     unpad_to_host(c, st.wup, kStencilHalf, width, height, w_upper);
     unpad_to_host(c, st.D, 1, width, height, degree);
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gpb_last_eigenvectors(gpb_ctx* ctx, int width, int height, int nvec, float* evecs, float* degree) {
+  return guarded2(ctx, [&] {
+    Context& c = ctx->c;
+    if (c.last_run_W == 0) throw ArgError("no whole-path result on this context (or a later call replaced it)");
+    if (width != c.last_run_W || height != c.last_run_H) throw ArgError("width / height differ from the last run's");
+    if (nvec < 1 || nvec > c.last_run_K) throw ArgError("nvec must be 1 .. the last run's eigenvector count");
+    const int W = c.last_run_W, H = c.last_run_H;
+    const size_t Np = size_t(W + 2 * kICRadius) * (H + 2 * kICRadius);
+    if (evecs) unpad_to_host(c, c.ws<float>("run.evecs", size_t(c.last_run_K) * Np), nvec, W, H, evecs);
+    if (degree) unpad_to_host(c, c.ws<float>("st.D", Np), 1, W, H, degree);
     GPB_HIP(hipStreamSynchronize(c.stream));
   });
 }

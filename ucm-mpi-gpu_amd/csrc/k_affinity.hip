@@ -180,6 +180,7 @@ void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, co
   st.Wp = W + 2 * kICRadius;
   st.Hp = H + 2 * kICRadius;
   st.Np = size_t(st.Wp) * st.Hp;
+  if (tag.empty()) c.last_run_W = 0;  // the untagged stencil of the last whole-path run is replaced
   st.wup = c.ws<float>(tag + "st.wup", kStencilHalf * st.Np);
   st.D = c.ws<float>(tag + "st.D", st.Np);
   st.s = c.ws<float>(tag + "st.s", st.Np);

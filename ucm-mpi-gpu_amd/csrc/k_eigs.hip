@@ -40,14 +40,14 @@ void launch_fill(Context& c, float* p, size_t n, float v) {
   GPB_LAUNCH_CHECK();
 }
 
-template <int ROWS, typename AT>
-__global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
-                                                           const float* __restrict__ x, float* __restrict__ y, int W,
-                                                           int H, int Wp, int Hp, size_t Np, SpmvArgs A,
-                                                           double* __restrict__ xy_partial) {
-  constexpr int kSpmvTY = 4 * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+template <int ROWS, typename AT, int NY>
+__global__ __launch_bounds__(kSpmvTX * NY) void spmv_kernel(const float* __restrict__ wup, const float* __restrict__ s,
+                                                            const float* __restrict__ x, float* __restrict__ y, int W,
+                                                            int H, int Wp, int Hp, size_t Np, SpmvArgs A,
+                                                            double* __restrict__ xy_partial) {
+  constexpr int kSpmvTY = NY * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
   __shared__ AT u[kSpmvTileH * kSpmvTileW];
-  __shared__ double red[4];
+  __shared__ double red[NY];
   // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
   // workgroup b runs on XCD b % 8; each XCD takes a contiguous run of tiles
   // (row-major), so horizontally and vertically adjacent tiles, which read
@@ -58,17 +58,26 @@ __global__ __launch_bounds__(kSpmvTX * 4) void spmv_kernel(const float* __restri
   const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
   if (tile >= ntiles) return;  // whole workgroup
   const int x0 = (tile % tiles_x) * kSpmvTX, y0 = (tile / tiles_x) * kSpmvTY;  // interior coordinates of the tile origin
-  const double xy = spmv_tile<ROWS, AT>(wup, s, x, y, W, H, Wp, Hp, Np, A, x0, y0, u);
+  const double xy = spmv_tile<ROWS, AT, NY>(wup, s, x, y, W, H, Wp, Hp, Np, A, x0, y0, u);
   if (!xy_partial) return;  // uniform
-  const double t = block_sum256(xy, red);
+  const double t = block_sum_waves<NY>(xy, red);
   if (threadIdx.x == 0 && threadIdx.y == 0) xy_partial[tile] = t;
 }
 
-int spmv_blocks(const Stencil& st, int rows) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, 4 * rows)); }
-int spmv_blocks(const Stencil& st) { return spmv_blocks(st, 2); }  // upper bound over the variants
+// SpMV tile codes: 10 * NY + ROWS (NY wavefronts of 64 threads, ROWS rows per
+// thread; tile 64 x NY*ROWS pixels).  Every tile is at least 8 rows tall.
+constexpr int kSpmvTiles[] = {42, 44, 48, 82, 84, 161, 162};
+bool spmv_tile_valid(int code) {
+  for (int t : kSpmvTiles)
+    if (t == code) return true;
+  return false;
+}
+static int tile_rows(int code) { return (code / 10) * (code % 10); }
+int spmv_blocks(const Stencil& st, int code) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, tile_rows(code))); }
+int spmv_blocks(const Stencil& st) { return spmv_blocks(st, 42); }  // upper bound over the variants (8-row tiles)
 
 namespace {
-void spmv_dispatch(Context& c, const Stencil& st, int rows, const float* x, float* y, double* xy_partial) {
+void spmv_dispatch(Context& c, const Stencil& st, int code, const float* x, float* y, double* xy_partial) {
   SpmvArgs A;
   int dx[kStencilHalf], dy[kStencilHalf];
   stencil_upper_offsets(dx, dy);
@@ -76,22 +85,25 @@ void spmv_dispatch(Context& c, const Stencil& st, int rows, const float* x, floa
     A.o[j] = st.off_host[j];
     A.lo[j] = dy[j] * kSpmvTileW + dx[j];
   }
-  const int ntiles = spmv_blocks(st, rows);
-  dim3 block(kSpmvTX, 4), grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
-#define GPB_SPMV(R, AT)                                                                                        \
-  hipLaunchKernelGGL((spmv_kernel<R, AT>), grid, block, 0, c.stream, st.wup, st.s, x, y, st.W, st.H, st.Wp, st.Hp, \
-                     st.Np, A, xy_partial);
-  const bool f64 = c.spmv_fp64;
-  switch (rows) {
-    case 2:
-      if (f64) GPB_SPMV(2, double) else GPB_SPMV(2, float)
-      break;
-    case 4:
-      if (f64) GPB_SPMV(4, double) else GPB_SPMV(4, float)
-      break;
-    case 8:
-      if (f64) GPB_SPMV(8, double) else GPB_SPMV(8, float)
-      break;
+  const int ntiles = spmv_blocks(st, code);
+  const dim3 grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
+#define GPB_SPMV(NY, R)                                                                                           \
+  case 10 * NY + R:                                                                                              \
+    if (c.spmv_fp64)                                                                                             \
+      hipLaunchKernelGGL((spmv_kernel<R, double, NY>), grid, dim3(kSpmvTX, NY), 0, c.stream, st.wup, st.s, x, y, \
+                         st.W, st.H, st.Wp, st.Hp, st.Np, A, xy_partial);                                        \
+    else                                                                                                         \
+      hipLaunchKernelGGL((spmv_kernel<R, float, NY>), grid, dim3(kSpmvTX, NY), 0, c.stream, st.wup, st.s, x, y,  \
+                         st.W, st.H, st.Wp, st.Hp, st.Np, A, xy_partial);                                        \
+    break;
+  switch (code) {
+    GPB_SPMV(4, 2)
+    GPB_SPMV(4, 4)
+    GPB_SPMV(4, 8)
+    GPB_SPMV(8, 2)
+    GPB_SPMV(8, 4)
+    GPB_SPMV(16, 1)
+    GPB_SPMV(16, 2)
     default:
       throw ArgError("unsupported SpMV tile");
   }
@@ -114,21 +126,22 @@ int tune_spmv(Context& c, const Stencil& st) {
   hipEvent_t e0, e1;
   GPB_HIP(hipEventCreate(&e0));
   GPB_HIP(hipEventCreate(&e1));
-  int best = 4;
+  int best = 42;
   float best_ms = 1e30f;
-  for (int rows : {2, 4, 8}) {
-    spmv_dispatch(c, st, rows, x, y, nullptr);
+  for (int code : kSpmvTiles) {
+    spmv_dispatch(c, st, code, x, y, nullptr);
     GPB_HIP(hipEventRecord(e0, c.stream));
-    for (int r = 0; r < 3; r++) spmv_dispatch(c, st, rows, x, y, nullptr);
+    for (int r = 0; r < 3; r++) spmv_dispatch(c, st, code, x, y, nullptr);
     GPB_HIP(hipEventRecord(e1, c.stream));
     GPB_HIP(hipEventSynchronize(e1));
     float ms = 0;
     GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (c.verbose)
-      std::fprintf(stderr, "gpb: SpMV tuning %dx%d: rows %d: %.1f us\n", st.W, st.H, rows, ms * 1e3f / 3);
+      std::fprintf(stderr, "gpb: SpMV tuning %dx%d: tile %d (64x%d): %.1f us\n", st.W, st.H, code, tile_rows(code),
+                   ms * 1e3f / 3);
     if (ms < best_ms) {
       best_ms = ms;
-      best = rows;
+      best = code;
     }
   }
   GPB_HIP(hipEventDestroy(e0));
@@ -138,11 +151,11 @@ int tune_spmv(Context& c, const Stencil& st) {
 }
 
 // y = L x; with xy_partial, also the per-tile partial sums of x . y
-// (spmv_blocks(st, rows) of them, rows = the tile the shape was tuned to).
+// (spmv_blocks(st, code) of them, code = the tile the shape was tuned to).
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double* xy_partial) {
-  const int rows = tune_spmv(c, st);
+  const int code = tune_spmv(c, st);
   KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + 12));
-  spmv_dispatch(c, st, rows, x, y, xy_partial);
+  spmv_dispatch(c, st, code, x, y, xy_partial);
 }
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) { launch_spmv(c, st, x, y, nullptr); }
@@ -423,6 +436,124 @@ __global__ void tcol_kernel(double* __restrict__ T, int m, int j, int couple_nex
       T[size_t(j + 1) * m + j] = b;
     }
   }
+}
+
+// ---- partial reorthogonalisation (Simon 1984) ---------------------------------
+// Semi-orthogonal Lanczos: a step applies the three-term recurrence only, and
+// the omega recurrence estimates omega_{j+1,k} = v_{j+1} . v_k from the
+// projected matrix; the whole-basis pass runs only when some estimate exceeds
+// sqrt(eps) (and once more on the step after, whose estimate depends on the
+// pre-correction vector too).  eps is the unit roundoff of the FP32-stored
+// basis.  T is the dense projected matrix (tridiagonal plus the arrowhead of a
+// thick restart), so (T omega_j)_k also covers the kept Ritz vectors.
+constexpr double kProEps = 5.9604644775390625e-08;  // 2^-24
+constexpr double kProSqrtEps = 2.44140625e-04;      // 2^-12
+
+// w <- w - alpha v_j - beta_{j-1} v_{j-1} (FP64, stored FP32); per-block
+// partials of ||w||^2.
+__global__ __launch_bounds__(256) void three_term_kernel(const float* __restrict__ V, size_t ld, int j,
+                                                         const double* __restrict__ alpha,
+                                                         const double* __restrict__ beta, float* __restrict__ w,
+                                                         size_t Np, double* __restrict__ nrm_partial) {
+  __shared__ double sh[16];
+  const double a = *alpha, b = beta[j - 1];
+  const float* vj = V + size_t(j) * ld;
+  const float* vjm = V + size_t(j - 1) * ld;
+  double nrm = 0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    const float f = float(double(w[p]) - a * double(vj[p]) - b * double(vjm[p]));
+    w[p] = f;
+    nrm += double(f) * double(f);
+  }
+  const double r = block_sum(nrm, sh);
+  if (threadIdx.x == 0) nrm_partial[blockIdx.x] = r;
+}
+
+// Step j (one block of 256 threads, j < m <= kProjectMaxN): beta_j from the
+// partial norms, the omega recurrence for k < j,
+//   b_new om1[k] = (T om0)_k - alpha_j om0[k] - b_cur om_1[k] +- eps (b_new + |T[k][k+1]|),
+// with om0 = omega_j (om0[j] = 1), om_1 = omega_{j-1} (om_1[j-1] = 1); the
+// reorthogonalisation gate; T[j][j] = alpha and, without a reorthogonalisation,
+// beta[j] and the coupling T[j][j+1].  om1 overwrites om_1 in place.
+// acc: byte and pass counters as for the DGKS gate.
+__global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int m, int j,
+                                                    const double* __restrict__ alpha, double* __restrict__ beta,
+                                                    const double* __restrict__ nrm_partial, int G,
+                                                    const double* __restrict__ om0, double* __restrict__ om_1,
+                                                    int* __restrict__ flag, int* __restrict__ pending, double np,
+                                                    double* __restrict__ acc) {
+  __shared__ double red4[4];
+  __shared__ double mx[4];
+  const int t = threadIdx.x;
+  double a = 0;
+  for (int b = t; b < G; b += 256) a += nrm_partial[b];
+  const double nrm = block_sum_waves<4>(a, red4);
+  const double bn = sqrt(nrm);
+  const double al = *alpha, bc = beta[j - 1];
+  double om = 0;
+  if (t < j) {
+    double tw = T[size_t(t) * m + j];  // om0[j] = 1
+    for (int l = 0; l < j; l++) tw += T[size_t(t) * m + l] * om0[l];
+    const double prev = (t == j - 1) ? 1.0 : om_1[t];
+    const double e = tw - al * om0[t] - bc * prev;
+    const double noise = kProEps * (bn + fabs(T[size_t(t) * m + t + 1]));
+    om = (e + (e >= 0 ? noise : -noise)) / (bn > 1e-300 ? bn : 1e-300);
+  }
+  double v = fabs(om);
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((t & 63) == 0) mx[t >> 6] = v;
+  __syncthreads();
+  const double worst = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
+  const int pend = *pending;
+  const bool viol = worst > kProSqrtEps;
+  const bool f = viol || pend;
+  // after a reorthogonalisation the new vector is orthogonal to rounding level
+  if (t < j) om_1[t] = f ? kProEps : om;
+  if (t == j) om_1[t] = kProEps;
+  if (t == 0) {
+    T[size_t(j) * m + j] = al;
+    *flag = f ? 1 : 0;
+    *pending = (viol && !pend) ? 1 : 0;
+    if (f) {
+      acc[0] += np * (4.0 * (j + 1) + 4);
+      acc[1] += np * (4.0 * (j + 1) + 8);
+      acc[2] += 1.0;
+    } else {
+      beta[j] = bn;
+      beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+      if (j + 1 < m) {
+        T[size_t(j) * m + j + 1] = bn;
+        T[size_t(j + 1) * m + j] = bn;
+      }
+    }
+  }
+}
+
+// After a gated reorthogonalisation: beta_j from ||w||^2 and the coupling.
+// The removed components stay out of T, which keeps the tridiagonal form the
+// three-term steps and the omega recurrence assume (recording them at
+// isolated steps made the restarted iteration diverge).
+__global__ void pro_finish_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ nrm2,
+                                  double* __restrict__ beta, const int* __restrict__ flag) {
+  if (*flag == 0) return;
+  const double bn = sqrt(*nrm2);
+  beta[j] = bn;
+  beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+  if (j + 1 < m) {
+    T[size_t(j) * m + j + 1] = bn;
+    T[size_t(j + 1) * m + j] = bn;
+  }
+}
+
+// omega_{j+1}[k] = omega_j[k] = eps for k <= j after a full
+// Gram-Schmidt step j (the first of a cycle); pending cleared.
+__global__ void omega_reset_kernel(double* __restrict__ om_a, double* __restrict__ om_b, int j,
+                                   int* __restrict__ pending) {
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) {
+    om_a[k] = kProEps;
+    om_b[k] = kProEps;
+  }
+  if (threadIdx.x == 0) *pending = 0;
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(const float* __restrict__ w, const double* __restrict__ beta, int j,
@@ -724,6 +855,13 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   const int spmv_g = spmv_blocks(st, tune_spmv(c, st));
   double* xy_part = c.ws<double>("eig.xy", size_t(spmv_g));
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
+  // partial reorthogonalisation for targets the FP64 refinement does not
+  // follow (see kRefineBelowTol); whole-basis Gram-Schmidt every step otherwise
+  const bool pro = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  double* om = c.ws<double>("eig.omega", size_t(2) * (kMaxBasis + 1));
+  int* pend = c.ws<int>("eig.pending", 1);
+  double* rnrm = c.ws<double>("eig.rnrm", 1);
+  auto om_slot = [&](int i) { return om + size_t(i & 1) * (kMaxBasis + 1); };
   Reducer red(c, Np);
   const unsigned vg = vec_grid(Np);
   launch_fill(c, w, Np, 0.f);
@@ -788,6 +926,29 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, int(m <= kProjectMaxN), nullptr,
                            h1, h2, one_flag,
                            nrm + 1, nrm + 1, beta);
+        GPB_LAUNCH_CHECK();
+        if (pro) {
+          hipLaunchKernelGGL(omega_reset_kernel, dim3(1), dim3(256), 0, c.stream, om_slot(j), om_slot(j + 1), j, pend);
+          GPB_LAUNCH_CHECK();
+        }
+      } else if (pro) {
+        // three-term step; the whole-basis pass only when the omega
+        // recurrence says semi-orthogonality is about to be lost
+        launch_spmv(c, st, vj, w, xy_part);
+        hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, c.stream, xy_part, spmv_g, alpha, nullptr);
+        GPB_LAUNCH_CHECK();
+        {
+          KScope ks(c, kKUpdate, double(Np) * 16.0);
+          hipLaunchKernelGGL(three_term_kernel, dim3(red.GU), dim3(256), 0, c.stream, V, Np, j, alpha, beta, w, Np,
+                             red.npart);
+          GPB_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(omega_kernel, dim3(1), dim3(256), 0, c.stream, T, m, j, alpha, beta, red.npart, red.GU,
+                           om_slot(j), om_slot(j + 1), flag, pend, double(Np), gacc);
+        GPB_LAUNCH_CHECK();
+        red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
+        red.update(V, Np, n, h2, w, nrm + 1, flag);
+        hipLaunchKernelGGL(pro_finish_kernel, dim3(1), dim3(1), 0, c.stream, T, m, j, nrm + 1, beta, flag);
         GPB_LAUNCH_CHECK();
       } else {
         // three-term step, one full projection pass, and a second pass only
@@ -865,6 +1026,18 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
+    if (pro) {
+      // The kept Ritz vectors inherit the basis' loss of orthogonality, and
+      // the residual vector's overlap with them would feed every three-term
+      // step of the next cycle (the omega recurrence assumes it is at
+      // rounding level): Gram-Schmidt it against them twice, then normalise.
+      float* vk = V + size_t(keep) * Np;
+      for (int pass = 0; pass < 2; pass++) {
+        red.project(V, Np, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+        red.update(V, Np, keep, h1, vk, rnrm);
+      }
+      launch_scale_by(c, vk, rnrm, Np);
+    }
   }
   stats.iterations = matvecs;
   stats.converged = 0;

@@ -8,9 +8,11 @@ single-vector Lanczos (mode 0), block Lanczos from the network's block
 (mode 1) and block Lanczos from a random block (mode 2, the same solver
 without the network).  All at the same residual target (1e-3); each run
 reports wall time, eigensolve-stage time, network time, operator
-applications, fallback and measured residual.  The first image's runs are
-repeated once untimed so that allocation and tuning stay out of the timings.
-Prints one JSON object.
+applications, fallback and measured residual, and for the modes other than
+plain the largest principal angle between their 8 nontrivial eigenvectors and
+the plain solve's (D-weighted, read back after the timed run).  The first
+image's runs are repeated once untimed so that allocation and tuning stay out
+of the timings.  Prints one JSON object.
 
     python ucm-mpi-gpu_amd/eval_warm_start.py --weights ucm-mpi-gpu_amd/weights/warm_start.f32
 """
@@ -37,6 +39,19 @@ def _binding():
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def subspace_angle(Va: np.ndarray, Vb: np.ndarray, D: np.ndarray) -> float:
+    """Largest principal angle between span(Va) and span(Vb) in the D inner product."""
+    s = np.sqrt(D.astype(np.float64)).reshape(-1)
+    A = Va.reshape(len(Va), -1).astype(np.float64) * s
+    B = Vb.reshape(len(Vb), -1).astype(np.float64) * s
+    La = np.linalg.cholesky(A @ A.T)
+    Lb = np.linalg.cholesky(B @ B.T)
+    M = np.linalg.solve(La, A @ B.T)
+    M = np.linalg.solve(Lb, M.T).T  # Qa Qb^T with Qa = La^-1 A, Qb = Lb^-1 B orthonormal rows
+    sv = np.linalg.svd(M, compute_uv=False)
+    return float(np.arccos(np.clip(sv.min(), -1.0, 1.0)))
 
 
 def main():
@@ -74,6 +89,7 @@ def main():
         d_cb = torch.from_numpy(cb).to(dev)
         torch.cuda.synchronize()
         row = dict(seed=seed)
+        vecs = {}
         for rep in range(2 if i == 0 else 1):
             for m in modes:
                 ctx.set_warm_start(m, args.basis, args.keep)
@@ -87,10 +103,22 @@ def main():
                                      warm_ms=round(st["warm_ms"], 2), matvecs=st["matvecs"],
                                      fallback=st["fallback"], residual=st["max_residual"],
                                      evals=[float(x) for x in st["evals"][:9]])
+                if rep == (1 if i == 0 else 0):
+                    v, deg = ctx.last_eigenvectors(S, S, 9, with_degree=(m == modes[0]))
+                    vecs[m] = v[1:]
+                    if deg is not None:
+                        vecs["D"] = deg
+        if 0 in vecs:
+            for m in modes:
+                if m != 0:
+                    row[names[m]]["angle_vs_plain"] = subspace_angle(vecs[m], vecs[0], vecs["D"])
+        del vecs
         ctx.set_warm_start(0)
         per_image.append(row)
-        brief = {names[m]: (row[names[m]]["ms"], row[names[m]]["matvecs"], row[names[m]]["fallback"]) for m in modes}
-        print(f"image {i + 1}/{args.count} seed {seed}: (ms, matvecs, fallback) {brief}", file=sys.stderr, flush=True)
+        brief = {names[m]: (row[names[m]]["ms"], row[names[m]]["matvecs"], row[names[m]]["fallback"],
+                            round(row[names[m]].get("angle_vs_plain", 0.0), 4)) for m in modes}
+        print(f"image {i + 1}/{args.count} seed {seed}: (ms, matvecs, fallback, angle vs plain) {brief}",
+              file=sys.stderr, flush=True)
     summary = {}
     for m in modes:
         k = names[m]
@@ -108,6 +136,10 @@ def main():
         se = [r["plain"]["eig_ms"] / r[k]["eig_ms"] for r in per_image]
         summary[k]["speedup_vs_plain_geomean"] = float(np.exp(np.mean(np.log(sp))))
         summary[k]["eig_speedup_vs_plain_geomean"] = float(np.exp(np.mean(np.log(se))))
+        ang = [r[k]["angle_vs_plain"] for r in per_image if "angle_vs_plain" in r[k]]
+        if ang:
+            summary[k]["angle_vs_plain_median"] = float(np.median(ang))
+            summary[k]["angle_vs_plain_max"] = float(np.max(ang))
     out = dict(workload=f"{args.count} images {S}x{S}, three-class Voronoi, {args.regions} regions, sigma 8, seeds "
                         f"{args.seed0}..{args.seed0 + args.count - 1}, 9 eigenvectors, residual {args.tol:g}",
                block=dict(basis=args.basis, keep=args.keep), summary=summary, images=per_image)

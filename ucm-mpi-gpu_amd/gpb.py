@@ -51,6 +51,7 @@ _sig("gpb_build_w", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, _f32p])
 _sig("gpb_eigs", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f64p, C.c_void_p])
 _sig("gpb_spectral_pb", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, C.c_void_p])
 _sig("gpb_spb_from_evecs", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _f32p, _f64p, _f32p])
+_sig("gpb_last_eigenvectors", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p])
 _sig("gpb_compute", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
 _sig("gpb_compute_device", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p])
@@ -70,6 +71,7 @@ _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_reorth", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_verbose", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_compute_f64", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p])
@@ -350,6 +352,14 @@ class Context:
         self._chk(lib.gpb_compute_device(self.h, width, height, d_packed, d_codebook, d_gmax or None, d_gall or None,
                                          d_orient or None))
 
+    def last_eigenvectors(self, width: int, height: int, nvec: int = 9, with_degree: bool = True):
+        """Eigenvectors [nvec, H, W] and degree [H, W] of the last compute / compute_device call."""
+        vec = np.empty((nvec, height, width), np.float32)
+        deg = np.empty((height, width), np.float32) if with_degree else None
+        self._chk(lib.gpb_last_eigenvectors(self.h, width, height, nvec, vec.ctypes.data,
+                                            deg.ctypes.data if with_degree else None))
+        return vec, deg
+
     def set_block_size(self, block: int):
         self._chk(lib.gpb_set_block_size(self.h, block))
 
@@ -423,9 +433,11 @@ class Context:
                                          g.ctypes.data, y.ctypes.data))
         return v.value, g, y
 
-    def set_spmv_tile(self, rows: int):
-        """SpMV rows per thread (2, 4, 8), or 0 to autotune per image shape."""
-        self._chk(lib.gpb_set_spmv_tile(self.h, rows))
+    SPMV_TILES = (42, 44, 48, 82, 84, 161, 162)
+
+    def set_spmv_tile(self, tile: int):
+        """SpMV tile code 10 * wavefronts + rows per thread (one of SPMV_TILES), or 0 to autotune per shape."""
+        self._chk(lib.gpb_set_spmv_tile(self.h, tile))
 
     def compute_f64(self, packed: np.ndarray, codebook: np.ndarray, with_evecs: bool = True) -> dict:
         """The all-FP64 GPU path: mpb_nmax, gpb_orient [8,H,W], gpb_max, evals, evecs [nvec,H,W] (float64)."""
@@ -455,6 +467,10 @@ class Context:
     def set_spmv_fp64(self, on: bool):
         """SpMV arithmetic in FP64 (default) or FP32; HBM operands are FP32 either way."""
         self._chk(lib.gpb_set_spmv_precision(self.h, 1 if on else 0))
+
+    def set_reorth(self, partial: bool):
+        """Partial reorthogonalisation (default, residual targets >= 1e-4) or the whole-basis pass every step."""
+        self._chk(lib.gpb_set_reorth(self.h, 1 if partial else 0))
 
     def spmv_tile(self, width: int, height: int) -> int:
         r = C.c_int(0)
