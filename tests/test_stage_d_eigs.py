@@ -149,30 +149,40 @@ def _apply_w(orc, Wd, v):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("W,H,seed,tol", [(96, 80, 6, 1e-3), (96, 80, 6, 1e-4), (256, 224, 4, 1e-4),
-                                          (256, 224, 4, 1e-3)])
-def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol):
-    """Partial reorthogonalisation (the default for residual targets >= 1e-4)
-    against the whole-basis pass on every step: with either, every returned
-    pair meets the target when its residual is recomputed in FP64 from the
-    oracle's W, and every Ritz value lies within its FP64 residual in the
-    normalised-Laplacian norm, ||L u - theta u|| / ||u|| with u = D^1/2 v, of
-    an eigenvalue of the FP64 oracle (Bauer-Fike for the symmetric L); the
-    partial form runs the whole-basis pass on a minority of its steps."""
-    print(f"partial reorth {W}x{H} seed {seed} tol {tol}", flush=True)
+@pytest.mark.parametrize("W,H,seed,tol,block", [(96, 80, 6, 1e-3, 1), (96, 80, 6, 1e-4, 1), (256, 224, 4, 1e-4, 1),
+                                                (256, 224, 4, 1e-3, 1), (256, 224, 4, 1e-3, 8),
+                                                (256, 224, 4, 1e-4, 8), (160, 120, 5, 1e-3, 4)])
+def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
+    """Semi-orthogonal steps (the default for residual targets >= 1e-4: the
+    omega recurrence for the single-vector solver, measured overlaps for the
+    block solver) against the whole-basis pass on every step: with either,
+    every returned pair meets the target when its residual is recomputed in
+    FP64 from the oracle's W, and every Ritz value lies within its FP64
+    residual in the normalised-Laplacian norm, ||L u - theta u|| / ||u|| with
+    u = D^1/2 v, of an eigenvalue of the FP64 oracle (Bauer-Fike for the
+    symmetric L); the semi-orthogonal form runs the whole-basis pass on a
+    minority of its steps."""
+    print(f"partial reorth {W}x{H} seed {seed} tol {tol} block {block}", flush=True)
     nm = _tiny_w(orc, gpb, W, H, seed)
     Wd, D = orc.build_w(nm)
     ev_ref, _, _ = orc.eigs(Wd, D, K=20, tol=1e-12)
     sD = np.sqrt(D)
-    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
+    if block == 1:
+        ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
+    else:
+        ctx.set_eig_options(nvec=9, basis=64, keep=24, tol=tol, max_matvecs=40000)
     for partial in (False, True):
+        ctx.set_block_size(block)
         ctx.set_reorth(partial)
         try:
             ev, vec = ctx.eigs(nm.astype(np.float32))
+            st = ctx.stats()
         finally:
             ctx.set_reorth(True)
-        st = ctx.stats()
-        steps = st["matvecs"] - st["restarts"] - 2  # the first step of each cycle is always a whole-basis step
+            ctx.set_block_size(1)
+        assert st["block"] == block and st["fallback"] == 0
+        # steps of the semi-orthogonal form (the first step of each cycle is always a whole-basis step)
+        steps = (st["matvecs"] - 1) // block - st["restarts"] - 1
         print(f"  partial {partial}: matvecs {st['matvecs']} restarts {st['restarts']} whole-basis passes "
               f"{st['second_passes']} residual {st['max_residual']:.2e}", flush=True)
         assert st["max_residual"] <= tol

@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V,
 template <int B, int PX>
 __global__ __launch_bounds__(256) void bupdate_kernel(const float* __restrict__ V, size_t ld, int n,
                                                       const double* __restrict__ Hm, float* __restrict__ Wm, size_t ldw,
-                                                      size_t Np) {
+                                                      size_t Np, const int* __restrict__ gate, int want) {
+  if (gate && *gate != want) return;  // uniform
   __shared__ double hs[kBMaxBasis * B];
   for (int t = threadIdx.x; t < n * B; t += blockDim.x) hs[t] = Hm[t];
   __syncthreads();
@@ -315,13 +316,13 @@ __global__ void bchol_kernel(const double* __restrict__ G, double* __restrict__ 
   }
 }
 
-// T[i][k + r] += H1[i][r] + H2[i][r] for i < n, then mirrored: rows above the
-// panel copy their column entries, and the panel's diagonal block is made
-// symmetric by averaging.  One block; the phases are separated by barriers so
-// that no entry is read while another thread updates it.
-__global__ void btcol_kernel(double* __restrict__ T, int M, int k, int n, int B, const double* __restrict__ H1,
+// T[i][k + r] += H1[i][r] + H2[i][r] for lo <= i < n, then mirrored: rows
+// above the panel copy their column entries, and the panel's diagonal block is
+// made symmetric by averaging.  One block; the phases are separated by
+// barriers so that no entry is read while another thread updates it.
+__global__ void btcol_kernel(double* __restrict__ T, int M, int k, int n, int B, int lo, const double* __restrict__ H1,
                              const double* __restrict__ H2) {
-  for (int t = threadIdx.x; t < n * B; t += blockDim.x) {
+  for (int t = lo * B + threadIdx.x; t < n * B; t += blockDim.x) {
     const int i = t / B, r = t % B;
     T[size_t(i) * M + k + r] += H1[t] + H2[t];
   }
@@ -344,6 +345,40 @@ __global__ void fill_f64_kernel(double* p, size_t n, double v) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
 }
 
+// Semi-orthogonal block steps.  A block step's first pass measures the new
+// panel against the whole basis (H = V^T Pn, rows n .. n+B-1 holding the
+// panel's own Gram); the update then removes only the panel's two preceding
+// blocks [lo, n) -- the block three-term recurrence -- unless some overlap
+// with an older vector, relative to the norm the local update leaves
+// (||Pn_r||^2 - sum_{lo <= i < n} H[i][r]^2), exceeds kBSemiOrthTol, in which
+// case the whole basis is removed (*flag = 1).  A second Gram-Schmidt pass
+// against the two blocks follows either way; T keeps the block-tridiagonal
+// entries only.  acc[0] counts the whole-basis updates.
+constexpr double kBSemiOrthTol = 2.44140625e-04;  // sqrt of the FP32 unit roundoff 2^-24
+
+__global__ __launch_bounds__(256) void bgate_kernel(const double* __restrict__ H, int n, int lo, int B,
+                                                    int* __restrict__ flag, double* __restrict__ acc) {
+  __shared__ double est[kBMax];
+  __shared__ double mx[4];
+  const int t = threadIdx.x;
+  if (t < B) {
+    double e = H[size_t(n + t) * B + t];
+    for (int i = lo; i < n; i++) e -= H[size_t(i) * B + t] * H[size_t(i) * B + t];
+    est[t] = fmax(e, 1e-300);
+  }
+  __syncthreads();
+  double m = 0;
+  for (int q = t; q < lo * B; q += 256) m = fmax(m, fabs(H[q]) / sqrt(est[q % B]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((t & 63) == 0) mx[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    const bool f = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3])) > kBSemiOrthTol;
+    *flag = f ? 1 : 0;
+    if (f) acc[0] += 1.0;
+  }
+}
+
 __global__ void fill_i32_kernel(int* p, int v) { *p = v; }
 
 __global__ void t00_kernel(double* T, const double* h) { T[0] = h[0]; }
@@ -362,8 +397,10 @@ namespace {
 // dots group, PX pixels per thread in the dots and update kernels.
 template <int B>
 struct BlockShape {
-  static constexpr int NV = B <= 4 ? 8 : (B <= 8 ? 4 : 2);
-  static constexpr int PXD = B <= 8 ? 4 : 2;
+  // (B = 8: NV 2, PX 8 measured fastest of (4, 4), (2, 8), (2, 16) -- 19 %
+  // less dots time than (4, 4) at 4096^2; (2, 16) runs at occupancy 1)
+  static constexpr int NV = B <= 4 ? 8 : (B < 8 ? 4 : 2);
+  static constexpr int PXD = B < 8 ? 4 : (B == 8 ? 8 : 2);
   static constexpr int PXU = B <= 8 ? 2 : 1;
 };
 constexpr int kBDotsGrid = 1024;
@@ -454,10 +491,11 @@ struct BlockOps {
     }
     launch_reduce(c, partial, NC, n * B, dst);
   }
-  void update(const float* V, int n, const double* Hm, float* Wm) {
-    KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
+  // W_r -= sum_i Hm[i][r] V_i, i < n; with gate, only when *gate == want
+  void update(const float* V, int n, const double* Hm, float* Wm, const int* gate = nullptr, int want = 1) {
+    KScope ks(c, kKUpdate, gate ? 0.0 : double(Np) * (4.0 * n + 8.0 * B));
     hipLaunchKernelGGL((bupdate_kernel<B, BlockShape<B>::PXU>), dim3(ug), dim3(256), 0, c.stream, V, Np, n, Hm, Wm, Np,
-                       Np);
+                       Np, gate, want);
     GPB_LAUNCH_CHECK();
   }
   // out_c = sum_i Y[i][c] V_i, c < nk
@@ -554,19 +592,41 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   std::vector<double> resid(K, 0.0);
   bool converged = false;
   int kcur = 0;
+  // semi-orthogonal block steps for targets the FP64 refinement does not follow
+  const bool semi = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol;
+  int* gflag = c.ws<int>("beig.gflag", 1);
+  double* gacc = c.ws<double>("beig.gacc", 1);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, gacc, size_t(1), 0.0);
+  GPB_LAUNCH_CHECK();
   while (true) {
+    const int cycle_start = k;
     while (k + B <= M) {
       float* Pk = V + size_t(k) * Np;
       const int n = k + B;
       float* Pn = V + size_t(n) * Np;  // the next panel is built in place
       ops.spmv(Pk, Pn);
       matvecs += B;
-      ops.dots(V, n, Pn, nullptr, H1);
-      ops.update(V, n, H1, Pn);
-      ops.dots(V, n, Pn, nullptr, H2);
-      ops.update(V, n, H2, Pn);
-      hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, H1, H2);
-      GPB_LAUNCH_CHECK();
+      if (!semi || k == cycle_start) {
+        // whole-basis classical Gram-Schmidt twice (the first step of a
+        // cycle couples to every kept Ritz vector)
+        ops.dots(V, n, Pn, nullptr, H1);
+        ops.update(V, n, H1, Pn);
+        ops.dots(V, n, Pn, nullptr, H2);
+        ops.update(V, n, H2, Pn);
+        hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, 0, H1, H2);
+        GPB_LAUNCH_CHECK();
+      } else {
+        const int lo = k - B;
+        ops.dots(V, n + B, Pn, nullptr, H1);  // rows n .. n+B-1: the panel's own Gram
+        hipLaunchKernelGGL(bgate_kernel, dim3(1), dim3(256), 0, c.stream, H1, n, lo, B, gflag, gacc);
+        GPB_LAUNCH_CHECK();
+        ops.update(V, n, H1, Pn, gflag, 1);
+        ops.update(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0);
+        ops.dots(V + size_t(lo) * Np, n - lo, Pn, nullptr, H2 + size_t(lo) * B);
+        ops.update(V + size_t(lo) * Np, n - lo, H2 + size_t(lo) * B, Pn);
+        hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, lo, H1, H2);
+        GPB_LAUNCH_CHECK();
+      }
       ops.cholqr2(Pn, G, Rinv, R1, Rtot, brk);
       k += B;
     }
@@ -642,6 +702,23 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * M * M, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
+    if (semi) {
+      // the residual panel's overlap with the kept Ritz vectors (inherited
+      // from the basis' semi-orthogonality) would feed every local step of
+      // the next cycle: Gram-Schmidt it against them twice, re-orthonormalise
+      float* P = V + size_t(keep) * Np;
+      for (int pass = 0; pass < 2; pass++) {
+        ops.dots(V, keep, P, nullptr, H1);
+        ops.update(V, keep, H1, P);
+      }
+      ops.cholqr2(P, G, Rinv, R1, Rtot, brk);
+    }
+  }
+  if (semi) {
+    double g = 0;
+    GPB_HIP(hipMemcpyAsync(&g, gacc, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    stats.second_passes = int(g);
   }
   stats.iterations = matvecs;
   stats.converged = 0;
