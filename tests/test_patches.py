@@ -38,13 +38,23 @@ def test_cut_patches_tiles_the_image(gpb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("W,H,grid,ov", [(360, 300, 3, 8), (250, 180, 2, 16)])
-def test_batched_matches_one_by_one_gpu(gpb, ctx, W, H, grid, ov):
-    print(f"patches {W}x{H} grid {grid} overlap {ov}", flush=True)
+@pytest.mark.parametrize("W,H,grid,ov,tol", [(360, 300, 3, 8, 1e-6), (250, 180, 2, 16, 1e-6), (360, 300, 3, 8, 1e-3)])
+def test_batched_matches_one_by_one_gpu(gpb, ctx, W, H, grid, ov, tol):
+    """Batched and one-by-one solves of the same patches (at 1e-3 both take
+    partially reorthogonalised steps, per patch in the batched form): every
+    patch meets the target; at 1e-6 the eigenvalues agree to 5e-5 relative
+    and gPb to 1e-4.  At 1e-3 these clustered spectra are converged only to
+    the target, and solves whose rounding differs stop at different restart
+    cycles with Ritz values ~1e-4 apart, so there the batched partially
+    reorthogonalised solve is compared with the batched solve that
+    reorthogonalises every step: the same operator applications per patch and
+    eigenvalues within the square of the target.  A replayed graph gives the
+    identical result."""
+    print(f"patches {W}x{H} grid {grid} overlap {ov} tol {tol}", flush=True)
     rgb, packed = synth_packed(gpb, W, H, regions=24, seed=21)
     cb = gpb.make_codebook(packed, seed=21)
     rects, cores = gpb.cut_patches(W, H, grid, ov, 3)
-    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=1e-6)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
     out = {}
     for mode in (0, 1, 2, 1):  # one by one, batched, batched without graph, batched again (graph replayed)
         gmax, orient = ctx.compute_patches(packed, cb, rects, cores, mode)
@@ -53,14 +63,31 @@ def test_batched_matches_one_by_one_gpu(gpb, ctx, W, H, grid, ov):
         print(f"  mode {mode}: matvecs {[p['matvecs'] for p in ps]} residual max "
               f"{max(p['max_residual'] for p in ps):.2e} stage ms {[round(x, 1) for x in ps[0]['stage_ms']]}",
               flush=True)
-        assert all(p["max_residual"] <= 1e-6 for p in ps)
+        assert all(p["max_residual"] <= tol for p in ps)
     g0, o0, s0 = out[0][0]
     for mode, runs in out.items():
         for (g, o, s) in runs:
             for p in range(len(rects)):
                 e0, e1 = np.array(s0[p]["evals"][1:9]), np.array(s[p]["evals"][1:9])
-                assert np.max(np.abs(e1 - e0) / e0) <= 5e-5, (mode, p)
-            assert np.abs(g - g0).max() <= 1e-4 and np.abs(o - o0).max() <= 1e-4, mode
+                if tol <= 1e-6:
+                    assert np.max(np.abs(e1 - e0) / e0) <= 5e-5, (mode, p)
+            if tol <= 1e-6:
+                assert np.abs(g - g0).max() <= 1e-4 and np.abs(o - o0).max() <= 1e-4, mode
+    if tol > 1e-6:
+        ctx.set_reorth(False)
+        try:
+            ctx.compute_patches(packed, cb, rects, cores, 1)
+            full = [ctx.patch_stats(p) for p in range(len(rects))]
+        finally:
+            ctx.set_reorth(True)
+        part = out[1][0][2]
+        print(f"  batched, every-step reorthogonalisation: matvecs {[p['matvecs'] for p in full]}", flush=True)
+        for p in range(len(rects)):
+            assert part[p]["matvecs"] == full[p]["matvecs"], p
+            e0, e1 = np.array(full[p]["evals"][1:9]), np.array(part[p]["evals"][1:9])
+            # the same stopping point: rounding moves pairs converged to the
+            # target by up to the square of the target
+            assert np.max(np.abs(e1 - e0)) <= tol ** 2, p
     # the graph replay gives the identical result as the first batched call
     np.testing.assert_array_equal(out[1][0][0], out[1][1][0])
 

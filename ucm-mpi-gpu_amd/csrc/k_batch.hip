@@ -242,6 +242,130 @@ __global__ void btcol_kernel(double* __restrict__ T, int m, int j, int couple_ne
   }
 }
 
+// ---- partial reorthogonalisation, per patch (the single-image solver's
+// omega recurrence, k_eigs.hip) -------------------------------------------------
+constexpr double kBProEps = 5.9604644775390625e-08;  // 2^-24, unit roundoff of the FP32 basis
+constexpr double kBProSqrtEps = 2.44140625e-04;      // 2^-12
+constexpr int kOmS = kMaxBasis + 1;                  // omega slot stride
+
+// w <- w - alpha_p v_j - beta_p,j-1 v_{j-1} per patch; per-patch partials of ||w||^2.
+__global__ __launch_bounds__(256) void bthree_term_kernel(const BPatchDev* __restrict__ P, int np,
+                                                          const float* __restrict__ V, size_t ld, int j,
+                                                          const double* __restrict__ alpha,
+                                                          const double* __restrict__ beta, float* __restrict__ w,
+                                                          double* __restrict__ nrm_part,
+                                                          const int* __restrict__ active, size_t Ntot, size_t chunk) {
+  __shared__ double red[4];
+  const Range r = my_range(Ntot, chunk);
+  const float* vj = V + size_t(j) * ld;
+  const float* vjm = V + size_t(j - 1) * ld;
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p]) continue;  // uniform
+    const double a = alpha[p], b = beta[size_t(p) * 2 * kMaxBasis + j - 1];
+    double nrm = 0;
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+      const float f = float(double(w[e]) - a * double(vj[e]) - b * double(vjm[e]));
+      w[e] = f;
+      nrm += double(f) * double(f);
+    }
+    for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) nrm_part[size_t(p) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+  }
+}
+
+// Patch p = blockIdx.x (256 threads, j < m <= kProjectMaxN): the omega
+// recurrence b_new om1[k] = (T om0)_k - alpha om0[k] - b_cur om_1[k] +- eps (b_new + |T[k][k+1]|)
+// for k < j with om0 = omega_j (om0[j] = 1), om_1 = omega_{j-1} (om_1[j-1] = 1),
+// overwritten in place by omega_{j+1}; flag[p] = whole-basis pass needed
+// (estimate above sqrt(eps), or the step after one); T[j][j] = alpha and,
+// without a pass, beta_j and the coupling.  nrm[2 p]: ||w||^2 after the
+// three-term step.
+__global__ __launch_bounds__(256) void bomega_kernel(double* __restrict__ T, int m, int j,
+                                                     const double* __restrict__ alpha, double* __restrict__ beta,
+                                                     const double* __restrict__ nrm, double* __restrict__ om,
+                                                     int* __restrict__ flag, int* __restrict__ pending,
+                                                     const int* __restrict__ active, double* __restrict__ npasses) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  if (!active[p]) {
+    if (t == 0) flag[p] = 0;
+    return;
+  }
+  __shared__ double mx[4];
+  double* Tp = T + size_t(p) * m * m;
+  double* bp = beta + size_t(p) * 2 * kMaxBasis;
+  const double* om0 = om + (size_t(p) * 2 + (j & 1)) * kOmS;
+  double* om_1 = om + (size_t(p) * 2 + ((j + 1) & 1)) * kOmS;
+  const double bn = sqrt(nrm[2 * p]);
+  const double al = alpha[p], bc = bp[j - 1];
+  double o = 0;
+  if (t < j) {
+    double tw = Tp[size_t(t) * m + j];
+    for (int l = 0; l < j; l++) tw += Tp[size_t(t) * m + l] * om0[l];
+    const double prev = (t == j - 1) ? 1.0 : om_1[t];
+    const double e = tw - al * om0[t] - bc * prev;
+    const double noise = kBProEps * (bn + fabs(Tp[size_t(t) * m + t + 1]));
+    o = (e + (e >= 0 ? noise : -noise)) / (bn > 1e-300 ? bn : 1e-300);
+  }
+  double v = fabs(o);
+  for (int s = 32; s > 0; s >>= 1) v = fmax(v, __shfl_xor(v, s, 64));
+  if ((t & 63) == 0) mx[t >> 6] = v;
+  __syncthreads();
+  const double worst = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
+  const int pend = pending[p];
+  const bool viol = worst > kBProSqrtEps;
+  const bool f = viol || pend;
+  if (t < j) om_1[t] = f ? kBProEps : o;
+  if (t == j) om_1[t] = kBProEps;
+  if (t == 0) {
+    Tp[size_t(j) * m + j] = al;
+    flag[p] = f ? 1 : 0;
+    pending[p] = (viol && !pend) ? 1 : 0;
+    if (f) {
+      atomicAdd(npasses, 1.0);
+    } else {
+      bp[j] = bn;
+      bp[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+      if (j + 1 < m) {
+        Tp[size_t(j) * m + j + 1] = bn;
+        Tp[size_t(j + 1) * m + j] = bn;
+      }
+    }
+  }
+}
+
+// After a gated whole-basis pass: beta_j from ||w||^2 (nrm[2 p + 1]) and the
+// coupling; the removed components stay out of T.
+__global__ void bpro_finish_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ nrm,
+                                   double* __restrict__ beta, const int* __restrict__ flag,
+                                   const int* __restrict__ active, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || !active[p] || !flag[p]) return;
+  const double bn = sqrt(nrm[2 * p + 1]);
+  double* bp = beta + size_t(p) * 2 * kMaxBasis;
+  bp[j] = bn;
+  bp[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+  if (j + 1 < m) {
+    double* Tp = T + size_t(p) * m * m;
+    Tp[size_t(j) * m + j + 1] = bn;
+    Tp[size_t(j + 1) * m + j] = bn;
+  }
+}
+
+// omega_{j+1} = omega_j = eps after the whole-basis step j of a cycle; pending cleared.
+__global__ void bomega_reset_kernel(double* __restrict__ om, int j, int* __restrict__ pending) {
+  const int p = blockIdx.x;
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) {
+    om[(size_t(p) * 2 + 0) * kOmS + k] = kBProEps;
+    om[(size_t(p) * 2 + 1) * kOmS + k] = kBProEps;
+  }
+  if (threadIdx.x == 0) pending[p] = 0;
+}
+
 // out = w / beta_p,j per patch (or * 1/sqrt(nrm2[p * nstride]) when nrm2).
 __global__ __launch_bounds__(256) void bscale_kernel(const BPatchDev* __restrict__ P, int np,
                                                      const float* __restrict__ w, const double* __restrict__ beta,
@@ -662,6 +786,10 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
   int* ones = c.ws<int>("bat.ones", size_t(np));
   double* nsecond = c.ws<double>("bat.nsecond", 1);
   double* Ydev = c.ws<double>("bat.Y", size_t(np) * m * keep);
+  // partial reorthogonalisation for targets the FP64 refinement does not follow
+  const bool pro = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  double* om = c.ws<double>("bat.omega", size_t(np) * 2 * kOmS);
+  int* pend = c.ws<int>("bat.pending", size_t(np));
   int* rows_dev = c.ws<int>("bat.rows", size_t(K + 1));
   GPB_HIP(hipMemcpyAsync(ones, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
   GPB_HIP(hipMemsetAsync(T, 0, sizeof(double) * np * m * m, c.stream));
@@ -714,6 +842,29 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
         bs.update(Vc, Nt, n, h2, w, nrm + 1);
         hipLaunchKernelGGL(btcol_kernel, dim3(np), dim3(kMaxBasis), 0, c.stream, T, m, j, 1, nullptr, h1, h2, ones,
                            nrm + 1, nrm + 1, beta, bs.active);
+        GPB_LAUNCH_CHECK();
+        if (pro) {
+          hipLaunchKernelGGL(bomega_reset_kernel, dim3(np), dim3(256), 0, c.stream, om, j, pend);
+          GPB_LAUNCH_CHECK();
+        }
+      } else if (pro) {
+        bs.spmv(vj, w, alpha);
+        {
+          KScope ks(c, kKUpdate, double(Nt) * 16.0);
+          hipLaunchKernelGGL(bthree_term_kernel, dim3(bs.G), dim3(256), 0, c.stream, bs.P, np, Vc, Nt, j, alpha, beta,
+                             w, bs.npart, bs.active, Nt, bs.chunk);
+          GPB_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(breduce_kernel, dim3(1, np), dim3(kRedThreads), 0, c.stream, bs.npart, bs.G, bs.brange, 1, 0,
+                           nrm, 2, nullptr, bs.active);
+        GPB_LAUNCH_CHECK();
+        hipLaunchKernelGGL(bomega_kernel, dim3(np), dim3(256), 0, c.stream, T, m, j, alpha, beta, nrm, om, flag, pend,
+                           bs.active, nsecond);
+        GPB_LAUNCH_CHECK();
+        bs.project(Vc, Nt, n, w, nullptr, nullptr, j, flag, h2);
+        bs.update(Vc, Nt, n, h2, w, nrm + 1, flag);
+        hipLaunchKernelGGL(bpro_finish_kernel, dim3(div_up(np, 64)), dim3(64), 0, c.stream, T, m, j, nrm, beta, flag,
+                           bs.active, np);
         GPB_LAUNCH_CHECK();
       } else {
         bs.spmv(vj, w, alpha);
@@ -839,6 +990,17 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * np * m * m, hipMemcpyHostToDevice, c.stream));
     if (newly_done) GPB_HIP(hipMemcpyAsync(bs.active, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
     k = keep;
+    if (pro) {
+      // the residual vector's overlap with the kept Ritz vectors would feed
+      // every three-term step of the next cycle: Gram-Schmidt it against them
+      // twice, then normalise (per patch)
+      float* vk = V + size_t(keep) * Nt;
+      for (int pass = 0; pass < 2; pass++) {
+        bs.project(V, Nt, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+        bs.update(V, Nt, keep, h1, vk, nrm + 1);
+      }
+      bs.scale(vk, nullptr, 0, nrm + 1, 2, vk);
+    }
   }
   for (int g = 0; g < 2; g++)
     if (graphs[g]) (void)hipGraphExecDestroy(graphs[g]);
