@@ -136,7 +136,7 @@ int gpb_set_block_size(gpb_ctx* ctx, int block);
 /* Eigensolve start for later gpb_compute calls: mode 0 = plain (default),
  * 1 = learned block (needs initialised parameters), 2 = block Lanczos from a
  * random block (the ablation without the network).  basis / keep: block
- * solver basis and restart sizes (<= 0: 64 / 24).  Needs nvec = 9. */
+ * solver basis and restart sizes (<= 0: 80 / 24).  Needs nvec = 9. */
 int gpb_set_warm_start(gpb_ctx* ctx, int mode, int basis, int keep);
 
 /* Parameter vector: per layer l = 0..11, W[9][cout][cin] (tap = 3 dy + dx)

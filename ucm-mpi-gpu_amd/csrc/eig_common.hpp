@@ -38,6 +38,8 @@ struct SpmvArgs {
   int lo[kStencilHalf];  // LDS tile offsets
 };
 
+
+
 // One 64 x NY*ROWS tile of y = L x at interior origin (x0, y0) (blockDim =
 // 64 x NY); returns this thread's share of x . y over the tile.  AT is the arithmetic type of the
 // differences, products and sum (float, or double with the FP32 operands
@@ -62,8 +64,8 @@ __device__ __forceinline__ gptr<T> as_global_mut(T* p) {
 template <int ROWS, typename AT, int NY = 4>
 __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, const float* __restrict__ s_,
                                             const float* __restrict__ x_, float* __restrict__ y_, int W, int H, int Wp,
-                                            int Hp, size_t Np, const SpmvArgs A, int x0, int y0, AT* __restrict__ u) {
-  const gptr<const float> wup = as_global(wup_);
+                                            int Hp, size_t Np, const SpmvArgs& /* offsets: kUpper and Wp */, int x0,
+                                            int y0, AT* __restrict__ u) {
   const gptr<const float> s = as_global(s_);
   const gptr<const float> x = as_global(x_);
   const gptr<float> y = as_global_mut(y_);
@@ -88,23 +90,28 @@ __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, cons
       const int ty = threadIdx.y + NY * r;
       const int yi = y0 + ty;
       if (yi >= H) break;
-      const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
+      // 32-bit pixel index: each plane is addressed as a wave-uniform base
+      // plus this offset, so the loads need no per-lane 64-bit address
+      // arithmetic (planes hold < 2^30 pixels)
+      const uint32_t p = uint32_t(yi + kICRadius) * uint32_t(Wp) + uint32_t(xi + kICRadius);
       const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
-      // all 80 weights of the row are loaded before the first FMA: the
-      // pointers may come from a table, where no no-alias fact lets the
-      // compiler move loads ahead of earlier stores on its own
+      const uint32_t p4 = p * 4u;  // byte offset within a plane
       float wa[kStencilHalf], wb[kStencilHalf];
 #pragma unroll
       for (int j = 0; j < kStencilHalf; j++) {
-        wa[j] = wup[size_t(j) * Np + p];
-        wb[j] = wup[size_t(j) * Np + p - A.o[j]];
+        const gptr<const char> plane = as_global(reinterpret_cast<const char*>(wup_ + size_t(j) * Np));
+        const uint32_t o4 = uint32_t(kUpper.dy[j] * Wp + kUpper.dx[j]) * 4u;
+        wa[j] = *reinterpret_cast<gptr<const float>>(plane + p4);
+        wb[j] = *reinterpret_cast<gptr<const float>>(plane + (p4 - o4));
       }
       const AT up = u[c];
       AT acc = 0;
 #pragma unroll
       for (int j = 0; j < kStencilHalf; j++) {
-        acc = fma_t(AT(wa[j]), up - u[c + A.lo[j]], acc);
-        acc = fma_t(AT(wb[j]), up - u[c - A.lo[j]], acc);
+        constexpr int kW = kSpmvTileW;
+        const int lo = kUpper.dy[j] * kW + kUpper.dx[j];
+        acc = fma_t(AT(wa[j]), up - u[c + lo], acc);
+        acc = fma_t(AT(wb[j]), up - u[c - lo], acc);
       }
       const float yp = float(AT(s[p]) * acc);
       y[p] = yp;

@@ -178,7 +178,7 @@ struct Context {
   // by the eigensolve (0 plain, 1 learned block, 2 block from a random start)
   std::shared_ptr<WarmState> warm;
   int warm_mode = 0;
-  int warm_basis = 64, warm_keep = 24;  // block-solver basis and restart size
+  int warm_basis = 80, warm_keep = 24;  // block-solver basis and restart size
   double warm_ms = 0;  // network inference time of the last warm-started solve
 
   template <typename T>

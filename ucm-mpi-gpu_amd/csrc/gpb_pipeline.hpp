@@ -37,6 +37,27 @@ struct Stencil {
 // Workspace buffers are keyed by tag (distinct tags keep several stencils alive).
 void build_stencil(Context& c, const float* d_mpb, int W, int H, Stencil& st, const std::string& tag = "");
 void stencil_upper_offsets(int* dx, int* dy);
+// The 40 upper offsets of the radius-5 disc (dy > 0, or dy == 0 and dx > 0),
+// in the order the W planes are stored (tables.cpp upper_offsets; the host
+// checks the two agree, spmv_offsets_match).  As compile-time constants they
+// fold into the LDS read offsets of the SpMV.
+struct UpperOffsets {
+  int dx[kStencilHalf], dy[kStencilHalf];
+};
+constexpr UpperOffsets make_upper_offsets() {
+  UpperOffsets t{};
+  int n = 0;
+  for (int u = 0; u <= kICRadius; u++)
+    for (int v = -kICRadius; v <= kICRadius; v++)
+      if (u * u + v * v <= kICRadius * kICRadius && (u > 0 || v > 0)) {
+        t.dx[n] = v;
+        t.dy[n] = u;
+        n++;
+      }
+  return t;
+}
+constexpr UpperOffsets kUpper = make_upper_offsets();
+bool spmv_offsets_match();
 // The all-FP64 variant (precision reference): W, D, D^-1/2 stored in FP64.
 struct Stencil64 {
   int W = 0, H = 0, Wp = 0, Hp = 0;

@@ -413,7 +413,7 @@ int gpb_set_warm_start(gpb_ctx* ctx, int mode, int basis, int keep) {
   return guarded2(ctx, [&] {
     if (mode < 0 || mode > 2) throw ArgError("warm-start mode must be 0, 1 or 2");
     if (mode == 1 && !warm_ready(ctx->c)) throw ArgError("warm-start network not initialised");
-    const int b = basis > 0 ? basis : 64, k = keep > 0 ? keep : 24;
+    const int b = basis > 0 ? basis : 80, k = keep > 0 ? keep : 24;
     if (b < 9 + 2 * kWarmOut || k < 10 || k > b - kWarmOut) throw ArgError("bad warm-start basis / keep");
     ctx->c.warm_mode = mode;
     ctx->c.warm_basis = b;

@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kBTX * kBTY) void bspmv_kernel(const float* __restr
   __syncthreads();
   const int xi = x0 + threadIdx.x, yi = y0 + threadIdx.y;
   if (xi >= W || yi >= H) return;
-  const size_t p = size_t(yi + kICRadius) * Wp + xi + kICRadius;
+  const uint32_t p = uint32_t(yi + kICRadius) * uint32_t(Wp) + uint32_t(xi + kICRadius);
+  const uint32_t p4 = p * 4u;  // byte offset within a plane (wave-uniform plane bases, as in spmv_tile)
   const int c = (threadIdx.y + kICRadius) * kBTileW + threadIdx.x + kICRadius;
   float up[B];
   AT acc[B];
@@ -79,11 +80,14 @@ __global__ __launch_bounds__(kBTX * kBTY) void bspmv_kernel(const float* __restr
     up[r] = u[c * B + r];
     acc[r] = AT(0);
   }
-#pragma unroll 4
+#pragma unroll
   for (int j = 0; j < kStencilHalf; j++) {
-    const AT w1 = wup[size_t(j) * Np + p], w2 = wup[size_t(j) * Np + p - A.o[j]];
-    const float* u1 = u + (c + A.lo[j]) * B;
-    const float* u2 = u + (c - A.lo[j]) * B;
+    const char* plane = reinterpret_cast<const char*>(wup + size_t(j) * Np);
+    const uint32_t o4 = uint32_t(kUpper.dy[j] * Wp + kUpper.dx[j]) * 4u;
+    const AT w1 = *reinterpret_cast<const float*>(plane + p4), w2 = *reinterpret_cast<const float*>(plane + (p4 - o4));
+    const int lo = kUpper.dy[j] * kBTileW + kUpper.dx[j];
+    const float* u1 = u + (c + lo) * B;
+    const float* u2 = u + (c - lo) * B;
 #pragma unroll
     for (int r = 0; r < B; r++) {
       acc[r] = fma_t(w1, AT(up[r] - u1[r]), acc[r]);

@@ -76,8 +76,18 @@ static int tile_rows(int code) { return (code / 10) * (code % 10); }
 int spmv_blocks(const Stencil& st, int code) { return int(div_up(st.W, kSpmvTX) * div_up(st.H, tile_rows(code))); }
 int spmv_blocks(const Stencil& st) { return spmv_blocks(st, 42); }  // upper bound over the variants (8-row tiles)
 
+bool spmv_offsets_match() {
+  int dx[kStencilHalf], dy[kStencilHalf];
+  stencil_upper_offsets(dx, dy);
+  for (int j = 0; j < kStencilHalf; j++)
+    if (dx[j] != kUpper.dx[j] || dy[j] != kUpper.dy[j]) return false;
+  return true;
+}
+
 namespace {
 void spmv_dispatch(Context& c, const Stencil& st, int code, const float* x, float* y, double* xy_partial) {
+  static const bool offsets_ok = spmv_offsets_match();
+  if (!offsets_ok) throw std::logic_error("SpMV offset table differs from the W plane order");
   SpmvArgs A;
   int dx[kStencilHalf], dy[kStencilHalf];
   stencil_upper_offsets(dx, dy);

@@ -63,7 +63,7 @@ def main():
     ap.add_argument("--count", type=int, default=20)
     ap.add_argument("--modes", default="0,1,2")
     ap.add_argument("--tol", type=float, default=1e-3)
-    ap.add_argument("--basis", type=int, default=64)
+    ap.add_argument("--basis", type=int, default=80)
     ap.add_argument("--keep", type=int, default=24)
     args = ap.parse_args()
     import torch  # device buffers; torch's runtime is initialised before the library's context
