@@ -206,6 +206,12 @@ int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
  * SymEigsSolver.h:200-280; damascene not at all, lanczos.cu:192-282). */
 int gpb_set_reorth(gpb_ctx* ctx, int mode);
 
+/* Single-vector Lanczos for later solves on ctx: 1 (default) replays each
+ * restart cycle after the first from a hipGraph captured once per basis
+ * buffer, 0 launches every kernel directly (the same kernels in the same
+ * order).  No counterpart in the reference. */
+int gpb_set_lanczos_graph(gpb_ctx* ctx, int on);
+
 /* level > 0: the eigensolvers write one progress line per restart cycle to
  * stderr (operator applications, residual); 0 (default): silent. */
 int gpb_set_verbose(gpb_ctx* ctx, int level);

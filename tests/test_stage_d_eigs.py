@@ -194,3 +194,32 @@ def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
             assert np.min(np.abs(ev_ref - ev[i])) <= bound * (1 + 1e-6) + 1e-9
         if partial:
             assert st["second_passes"] < 0.5 * steps
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tol", [1e-3, 1e-6])
+def test_lanczos_graph_replay_gpu(gpb, orc, ctx, tol):
+    """Restart cycles replayed from captured hipGraphs (the default) against
+    the same kernels launched directly: identical operator applications,
+    eigenvalues and eigenvectors, over enough restarts that both graphs (one
+    per basis buffer) are replayed several times."""
+    print(f"lanczos graph replay tol {tol}", flush=True)
+    nm = _tiny_w(orc, gpb, 256, 224, 4)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
+    out = {}
+    for graph in (False, True, True):
+        ctx.set_lanczos_graph(graph)
+        try:
+            ev, vec = ctx.eigs(nm.astype(np.float32))
+            st = ctx.stats()
+        finally:
+            ctx.set_lanczos_graph(True)
+        print(f"  graph {graph}: matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e}",
+              flush=True)
+        assert st["restarts"] >= 6
+        out.setdefault(graph, []).append((ev, vec, st["matvecs"]))
+    e0, v0, m0 = out[False][0]
+    for e1, v1, m1 in out[True]:
+        assert m1 == m0
+        np.testing.assert_array_equal(e1, e0)
+        np.testing.assert_array_equal(v1, v0)

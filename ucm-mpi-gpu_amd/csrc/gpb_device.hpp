@@ -164,6 +164,8 @@ struct Context {
   // single-vector Lanczos steps: 1 (default) partial reorthogonalisation for
   // residual targets >= 1e-4, 0 whole-basis Gram-Schmidt every step
   int reorth_mode = 1;
+  // single-vector Lanczos: replay restart cycles from captured hipGraphs
+  bool lanczos_graph = true;
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
   std::map<uint64_t, int> spmv_band;  // tile-band width tuned with the rows (tiles)

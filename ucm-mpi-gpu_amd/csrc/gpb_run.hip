@@ -377,6 +377,13 @@ int gpb_set_reorth(gpb_ctx* ctx, int mode) {
   });
 }
 
+int gpb_set_lanczos_graph(gpb_ctx* ctx, int on) {
+  return guarded2(ctx, [&] {
+    if (on != 0 && on != 1) throw ArgError("graph mode must be 0 or 1");
+    ctx->c.lanczos_graph = on != 0;
+  });
+}
+
 int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64) {
   return guarded2(ctx, [&] {
     if (fp64 != 0 && fp64 != 1) throw ArgError("SpMV precision must be 0 (FP32) or 1 (FP64 arithmetic)");

@@ -912,11 +912,17 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   std::vector<double> betah(kMaxBasis);
   bool converged = false;
   std::vector<double> resid(K, 0.0);
-  while (true) {
+  // One restart cycle's device work, steps k0 .. m-1 on the basis Vc.  From
+  // the first restart on (k0 = keep) the launch sequence depends only on
+  // which of V / V2 is current, so it is captured once per buffer into a
+  // hipGraph and replayed; the first cycle, and every cycle while per-kernel
+  // timing is on, launches directly.
+  auto cycle = [&](int k0, float* Vc) {
+    float* const V = Vc;
+    const int k = k0;
     for (int j = k; j < m; j++) {
       float* vj = V + size_t(j) * Np;
       const int n = j + 1;
-      matvecs++;
       if (j == k || m > kProjectMaxN) {
         // first step of a cycle (its column couples to every kept Ritz
         // vector): classical Gram-Schmidt twice, FP64 accumulation
@@ -979,6 +985,42 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       hipLaunchKernelGGL(scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np);
       GPB_LAUNCH_CHECK();
     }
+  };
+  const bool use_graph = c.lanczos_graph && !c.kt.on;
+  hipGraphExec_t graphs[2] = {nullptr, nullptr};
+  int parity = 0;  // which of the two basis buffers is current
+  struct GraphGuard {
+    hipGraphExec_t* g;
+    ~GraphGuard() {
+      for (int i = 0; i < 2; i++)
+        if (g[i]) (void)hipGraphExecDestroy(g[i]);
+    }
+  } graph_guard{graphs};
+  while (true) {
+    if (use_graph && k == keep) {
+      hipGraphExec_t& ge = graphs[parity];
+      if (!ge) {
+        hipGraph_t g;
+        GPB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        c.capturing = true;
+        try {
+          cycle(k, V);
+        } catch (...) {
+          c.capturing = false;
+          hipGraph_t dummy;
+          (void)hipStreamEndCapture(c.stream, &dummy);
+          throw;
+        }
+        c.capturing = false;
+        GPB_HIP(hipStreamEndCapture(c.stream, &g));
+        GPB_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        GPB_HIP(hipGraphDestroy(g));
+      }
+      GPB_HIP(hipGraphLaunch(ge, c.stream));
+    } else {
+      cycle(k, V);
+    }
+    matvecs += m - k;
     // Rayleigh-Ritz on the host
     GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
@@ -1019,6 +1061,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     stats.pass2_bytes[1] = gacc_h[1];
     stats.second_passes = int(gacc_h[2]);
     std::swap(V, V2);
+    parity ^= 1;
     const double bm = betah[m - 1];
     double worst = 0;
     for (int i = 0; i < K; i++) {
