@@ -109,8 +109,8 @@ int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float*
 /* ---- stages (c)-(e): affinity, eigenvectors, sPb, gPb ---------------------- */
 
 /* Eigensolver settings for later calls on ctx: nvec eigenpairs (2..32,
- * default 9 as in damascene/src/damascene.cu:307), Lanczos basis size,
- * vectors kept at a restart, relative generalised residual target
+ * default 9 as in damascene/src/damascene.cu:307), Lanczos basis size and
+ * vectors kept at a restart (defaults 48 and 16), relative generalised residual target
  * ||(D-W)v - lambda D v|| / ||D v|| (default 1e-3) and a cap on operator
  * applications.  Replaces the hard-coded nEigNum/fEigTolerance of computeGPb
  * (damascene.cu:307-309) and nvec of spectralPb (src/globalPb.cpp:65). */
@@ -232,6 +232,11 @@ int gpb_set_verbose(gpb_ctx* ctx, int level);
  * and clear the totals.  Replaces the StopWatch stage timers of computeGPb
  * (damascene/src/damascene.cu:324-575). */
 int gpb_set_profiling(gpb_ctx* ctx, int on);
+
+/* As gpb_set_profiling, for the kernel classes whose bits are set in mask
+ * (bit k = class k of gpb_kernel_time; 0 disables): only those launches are
+ * bracketed by events, so the others run exactly as without profiling. */
+int gpb_set_profiling_classes(gpb_ctx* ctx, unsigned mask);
 
 /* Total device time (ms, HIP events on the context stream), launch count and
  * algorithmic HBM bytes (may be NULL) of one kernel class since profiling was

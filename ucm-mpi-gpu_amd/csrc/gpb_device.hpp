@@ -98,6 +98,8 @@ enum KernelClass {
 };
 struct KernelTimes {
   bool on = false;
+  unsigned mask = ~0u;  // kernel classes that get events while on
+  bool timing(int k) const { return on && ((mask >> k) & 1u); }
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<std::pair<int, size_t>> pending;  // (class, index of the start event)
@@ -157,7 +159,7 @@ struct Context {
   // eigensolver options (gpb_set_eig_options)
   int eig_K = 9;
   int eig_m = 48;
-  int eig_keep = 24;
+  int eig_keep = 16;
   double eig_tol = 1e-3;
   int eig_max_matvecs = 20000;
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
@@ -210,8 +212,9 @@ struct KScope {
   Context& c;
   int cls;
   size_t idx = 0;
-  KScope(Context& cc, int k, double algo_bytes) : c(cc), cls(k) {
-    if (c.kt.on) {
+  bool active;
+  KScope(Context& cc, int k, double algo_bytes) : c(cc), cls(k), active(cc.kt.timing(k)) {
+    if (active) {
       c.kt.bytes[k] += algo_bytes;
       idx = c.kt.used;
       hipEvent_t a = c.kt.next();
@@ -220,7 +223,7 @@ struct KScope {
     }
   }
   ~KScope() {
-    if (c.kt.on) {
+    if (active) {
       (void)hipEventRecord(c.kt.pool[idx + 1], c.stream);
       c.kt.pending.push_back({cls, idx});
     }

@@ -557,14 +557,17 @@ int gpb_warm_loss_grad(gpb_ctx* ctx, int nimg, int hq, int wq, const float* x, c
   });
 }
 
-int gpb_set_profiling(gpb_ctx* ctx, int on) {
+int gpb_set_profiling_classes(gpb_ctx* ctx, unsigned mask) {
   return guarded2(ctx, [&] {
     GPB_HIP(hipStreamSynchronize(ctx->c.stream));
     ctx->c.kt.fold();
-    ctx->c.kt.on = on != 0;
+    ctx->c.kt.mask = mask & ((1u << kNumKClasses) - 1u);
+    ctx->c.kt.on = ctx->c.kt.mask != 0;
     ctx->c.kt.reset();
   });
 }
+
+int gpb_set_profiling(gpb_ctx* ctx, int on) { return gpb_set_profiling_classes(ctx, on ? ~0u : 0u); }
 
 int gpb_kernel_time(gpb_ctx* ctx, int kernel_class, double* total_ms, long* launches, double* algo_bytes) {
   return guarded2(ctx, [&] {

@@ -129,6 +129,18 @@ __device__ __forceinline__ CT wgpb_of(const CueParams& P, int c) {
 // One snapshot: chi-squared of all orientations from the lane's histogram.
 // Histogram word [grp][bin][lane] holds the 8-bit counts of slices 4*grp..+3.
 // csum[b] (LDS, shared by the wave) = column sums of the truncated smoothing.
+//
+// The histogram has nb <= NB live bins (nb = largest label + 1 of the
+// channel).  Counts of bins nb..NB-1 are zero, so every per-bin quantity built
+// from counts (slice masses, U, V and their smoothings) is unchanged by running
+// the loops over the compile-time NB; only the chi-squared sum is cut at nb,
+// through rT[b] = 0 for b >= nb (smoothing spills mass past nb, and the
+// truncated histogram has no such bins).
+template <typename CT>
+__device__ __forceinline__ CT recip(CT x) {
+  if constexpr (sizeof(CT) == 4) return __builtin_amdgcn_rcpf(x);  // 1 ulp
+  else return CT(1) / x;
+}
 template <int NB, int KH, typename CT>
 __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
                                            const CT* __restrict__ g, const CT* __restrict__ csum,
@@ -139,34 +151,33 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
   for (int s = 0; s < kNumSlices; s++) alpha[s] = CT(0);
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    if (b < nb) {
-      const CT cb = csum[b];
+    const CT cb = csum[b];
 #pragma unroll
-      for (int grp = 0; grp < 4; grp++) {
-        const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+    for (int grp = 0; grp < 4; grp++) {
+      const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(cb, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
-      }
+      for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(cb, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
     }
   }
 #pragma unroll
-  for (int s = 0; s < kNumSlices; s++) alpha[s] = alpha[s] > CT(0) ? CT(1) / alpha[s] : CT(0);
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = alpha[s] > CT(0) ? recip(alpha[s]) : CT(0);
+  // The passes below read the histogram words again rather than keep pass
+  // A's 4 * NB words (or their 16 * NB converted counts) live in registers.
+  asm volatile("" ::: "memory");
   // pass B: U_T = sum_s a_s h_s, V_0 = right - left for orientation 0
   CT T[NB], V[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     CT u = CT(0), v = CT(0);
-    if (b < nb) {
 #pragma unroll
-      for (int grp = 0; grp < 4; grp++) {
-        const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+    for (int grp = 0; grp < 4; grp++) {
+      const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int s = 4 * grp + q;
-          const CT c = CT((w >> (8 * q)) & 0xffu);
-          u = fma_t(alpha[s], c, u);
-          v = fma_t(s < kNumOri ? -alpha[s] : alpha[s], c, v);
-        }
+      for (int q = 0; q < 4; q++) {
+        const int s = 4 * grp + q;
+        const CT c = CT((w >> (8 * q)) & 0xffu);
+        u = fma_t(alpha[s], c, u);
+        v = fma_t(s < kNumOri ? -alpha[s] : alpha[s], c, v);
       }
     }
     T[b] = u;  // U_T for now, smoothed below
@@ -182,15 +193,22 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
 #pragma unroll
       for (int k = -KH; k <= KH; k++) {
         const int n = b + k;
-        if (n >= 0 && n < NB) acc = (n < nb) ? fma_t(g[k < 0 ? -k : k], U[n], acc) : acc;
+        if (n >= 0 && n < NB) acc = fma_t(g[k < 0 ? -k : k], U[n], acc);
       }
       T[b] = acc;
     }
   }
   CT rT[NB];
 #pragma unroll
-  for (int b = 0; b < NB; b++) rT[b] = (b < nb && T[b] > CT(0)) ? CT(1) / T[b] : CT(0);
-  for (int o = 0; o < kNumOri; o++) {
+  for (int b = 0; b < NB; b++) rT[b] = (b < nb && T[b] > CT(0)) ? recip(T[b]) : CT(0);
+  asm volatile("" ::: "memory");
+  // orientations in two rolled rounds of four unrolled ones: the byte of a
+  // slice within its histogram word is then a constant
+#pragma unroll 1
+  for (int oh = 0; oh < kNumOri; oh += 4)
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int o = oh + q;
     CT acc = CT(0);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
@@ -200,31 +218,24 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
 #pragma unroll
         for (int k = -KH; k <= KH; k++) {
           const int n = b + k;
-          if (n >= 0 && n < NB) d = (n < nb) ? fma_t(g[k < 0 ? -k : k], V[n], d) : d;
+          if (n >= 0 && n < NB) d = fma_t(g[k < 0 ? -k : k], V[n], d);
         }
       }
-      if (b < nb && d != CT(0)) acc = fma_t(d * d, rT[b], acc);
+      acc = fma_t(d * d, rT[b], acc);
     }
     out[o] = CT(0.5) * acc;
     if (o + 1 < kNumOri) {
-      CT a0 = CT(0), a8 = CT(0);
-#pragma unroll
-      for (int s = 0; s < kNumOri; s++) {
-        a0 = (s == o) ? alpha[s] : a0;
-        a8 = (s == o) ? alpha[s + kNumOri] : a8;
-      }
-      a0 *= CT(2);
-      a8 *= -CT(2);
-      const int sh = 8 * (o & 3);
-      const uint32_t* h0 = hist + ((o >> 2) * kNbMax) * kCueLanes + lane;
-      const uint32_t* h8 = hist + (((o >> 2) + 2) * kNbMax) * kCueLanes + lane;
+      // orientation o + 1: slice o moves from the right half to the left,
+      // slice o + 8 from the left to the right
+      const CT a0 = CT(2) * (oh ? alpha[4 + q] : alpha[q]);
+      const CT a8 = -CT(2) * (oh ? alpha[kNumOri + 4 + q] : alpha[kNumOri + q]);
+      const uint32_t* h0 = hist + ((oh >> 2) * kNbMax) * kCueLanes + lane;
+      const uint32_t* h8 = hist + (((oh >> 2) + 2) * kNbMax) * kCueLanes + lane;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
-        if (b < nb) {
-          const CT c0 = CT((h0[b * kCueLanes] >> sh) & 0xffu);
-          const CT c8 = CT((h8[b * kCueLanes] >> sh) & 0xffu);
-          V[b] = fma_t(a8, c8, fma_t(a0, c0, V[b]));
-        }
+        const CT c0 = CT((h0[b * kCueLanes] >> (8 * q)) & 0xffu);
+        const CT c8 = CT((h8[b * kCueLanes] >> (8 * q)) & 0xffu);
+        V[b] = fma_t(a8, c8, fma_t(a0, c0, V[b]));
       }
     }
   }

@@ -1053,10 +1053,8 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     GPB_HIP(hipMemcpyAsync(gacc_h, gacc, sizeof(gacc_h), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     c.kt.fold();
-    if (c.kt.on) {
-      c.kt.bytes[kKDots] += gacc_h[0] - stats.pass2_bytes[0];
-      c.kt.bytes[kKUpdate] += gacc_h[1] - stats.pass2_bytes[1];
-    }
+    if (c.kt.timing(kKDots)) c.kt.bytes[kKDots] += gacc_h[0] - stats.pass2_bytes[0];
+    if (c.kt.timing(kKUpdate)) c.kt.bytes[kKUpdate] += gacc_h[1] - stats.pass2_bytes[1];
     stats.pass2_bytes[0] = gacc_h[0];
     stats.pass2_bytes[1] = gacc_h[1];
     stats.second_passes = int(gacc_h[2]);

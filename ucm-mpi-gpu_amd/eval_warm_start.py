@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--basis", type=int, default=80)
     ap.add_argument("--keep", type=int, default=24)
+    ap.add_argument("--plain-basis", type=int, default=48)
+    ap.add_argument("--plain-keep", type=int, default=16)
     args = ap.parse_args()
     import torch  # device buffers; torch's runtime is initialised before the library's context
 
@@ -75,7 +77,7 @@ def main():
     gpb = _binding()
     modes = [int(m) for m in args.modes.split(",")]
     ctx = gpb.Context(0)
-    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=args.tol, max_matvecs=40000)
+    ctx.set_eig_options(nvec=9, basis=args.plain_basis, keep=args.plain_keep, tol=args.tol, max_matvecs=40000)
     ctx.set_warm_params(np.fromfile(args.weights, dtype="<f4"))
     names = {0: "plain", 1: "learned", 2: "random_block"}
     per_image = []
@@ -142,7 +144,7 @@ def main():
             summary[k]["angle_vs_plain_max"] = float(np.max(ang))
     out = dict(workload=f"{args.count} images {S}x{S}, three-class Voronoi, {args.regions} regions, sigma 8, seeds "
                         f"{args.seed0}..{args.seed0 + args.count - 1}, 9 eigenvectors, residual {args.tol:g}",
-               block=dict(basis=args.basis, keep=args.keep), summary=summary, images=per_image)
+               block=dict(basis=args.basis, keep=args.keep), plain=dict(basis=args.plain_basis, keep=args.plain_keep), summary=summary, images=per_image)
     ctx.close()
     print(json.dumps(out), flush=True)
 

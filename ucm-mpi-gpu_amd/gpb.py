@@ -68,6 +68,7 @@ class Stats(C.Structure):
 
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
 _sig("gpb_set_profiling", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_profiling_classes", C.c_int, [_ctxp, C.c_uint])
 _sig("gpb_set_block_size", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
@@ -245,7 +246,7 @@ class Context:
         self._chk(lib.gpb_nonmax(self.h, W, H, m.reshape(-1), out.reshape(-1)))
         return out
 
-    def set_eig_options(self, nvec=9, basis=48, keep=24, tol=1e-3, max_matvecs=20000):
+    def set_eig_options(self, nvec=9, basis=48, keep=16, tol=1e-3, max_matvecs=20000):
         self._chk(lib.gpb_set_eig_options(self.h, nvec, basis, keep, tol, max_matvecs))
         self.nvec = nvec
 
@@ -339,6 +340,13 @@ class Context:
 
     def set_profiling(self, on: bool):
         self._chk(lib.gpb_set_profiling(self.h, int(on)))
+
+    def set_profiling_classes(self, names):
+        """Per-kernel events only around the named kernel classes (empty: off)."""
+        mask = 0
+        for n in names:
+            mask |= 1 << KERNEL_CLASSES.index(n)
+        self._chk(lib.gpb_set_profiling_classes(self.h, C.c_uint(mask)))
 
     def kernel_times(self) -> dict:
         out = {}
