@@ -11,7 +11,9 @@ Prints one JSON line (rank 0).  One untimed step with HIP events around every
 launch gives `kernels` (device ms, launches and GB/s per kernel class for that
 step) and names the class with the largest device time; in the timed steps
 events bracket only that class's launches on the library's stream, and
-`roofline` describes it from those.  `cpu_baseline` times the CPU oracle (one
+`roofline` describes it from those; its `copy_gbs` is the rate (bytes read +
+written) of a plain 1 GiB device copy on the same GPU, the practical streaming
+ceiling next to the published peak.  `cpu_baseline` times the CPU oracle (one
 thread) on a bounded crop of the same image.
 """
 from __future__ import annotations
@@ -36,6 +38,26 @@ def _load(name, path):
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def copy_rate_gbs(torch, mib: int = 1024, reps: int = 10) -> float:
+    """Bytes read + written per second of a plain device-to-device copy of mib MiB."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2.0 * n * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return rate
 
 
 def cpu_baseline(packed, codebook, crop: int, tol: float):
@@ -244,8 +266,10 @@ def main():
     avg_ms = d["ms"] / max(1, d["launches"])
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
     traffic = measured_traffic(dom, S)
+    copy_gbs = copy_rate_gbs(torch)
     roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    copy_gbs=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 4),
                     avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
                     bytes_per_launch=round(d["bytes"] / max(1, d["launches"])),
                     share_of_step=round(d["ms"] / (ms_step * args.steps), 4))
