@@ -212,6 +212,13 @@ int gpb_set_reorth(gpb_ctx* ctx, int mode);
  * order).  No counterpart in the reference. */
 int gpb_set_lanczos_graph(gpb_ctx* ctx, int on);
 
+/* Block Lanczos for later solves on ctx: 1 (default) fuses each local
+ * orthogonalisation pass with the dot products that follow it (update and
+ * re-measure, update and the panel's Gram, CholQR transform and the next
+ * Gram), 0 runs them as separate passes over the vectors.  The sums are the
+ * same either way.  No counterpart in the reference. */
+int gpb_set_block_fusion(gpb_ctx* ctx, int on);
+
 /* level > 0: the eigensolvers write one progress line per restart cycle to
  * stderr (operator applications, residual); 0 (default): silent. */
 int gpb_set_verbose(gpb_ctx* ctx, int level);

@@ -223,3 +223,37 @@ def test_lanczos_graph_replay_gpu(gpb, orc, ctx, tol):
         assert m1 == m0
         np.testing.assert_array_equal(e1, e0)
         np.testing.assert_array_equal(v1, v0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block,partial", [(8, True), (8, False), (4, True)])
+def test_block_fusion_identical_gpu(gpb, orc, ctx, block, partial):
+    """Block Lanczos with its local orthogonalisation passes fused with the
+    dot products that follow them (update and re-measure, update and the
+    panel's Gram, CholQR transform and the next Gram; the default) against the
+    separate passes: the fused kernels keep the separate kernels' chunking
+    and summation order, so operator applications, eigenvalues and
+    eigenvectors are identical, for the semi-orthogonal steps (partial) and
+    the whole-basis Gram-Schmidt steps."""
+    print(f"block fusion block {block} partial {partial}", flush=True)
+    nm = _tiny_w(orc, gpb, 256, 224, 4)
+    ctx.set_eig_options(nvec=9, basis=64, keep=24, tol=1e-3, max_matvecs=40000)
+    out = {}
+    for fused in (False, True):
+        ctx.set_block_size(block)
+        ctx.set_reorth(partial)
+        ctx.set_block_fusion(fused)
+        try:
+            ev, vec = ctx.eigs(nm.astype(np.float32))
+            st = ctx.stats()
+        finally:
+            ctx.set_block_fusion(True)
+            ctx.set_reorth(True)
+            ctx.set_block_size(1)
+        print(f"  fused {fused}: matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e}",
+              flush=True)
+        assert st["block"] == block and st["fallback"] == 0 and st["restarts"] >= 2
+        out[fused] = (ev, vec, st["matvecs"])
+    assert out[True][2] == out[False][2]
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    np.testing.assert_array_equal(out[True][1], out[False][1])

@@ -377,6 +377,13 @@ int gpb_set_reorth(gpb_ctx* ctx, int mode) {
   });
 }
 
+int gpb_set_block_fusion(gpb_ctx* ctx, int on) {
+  return guarded2(ctx, [&] {
+    if (on != 0 && on != 1) throw ArgError("fusion mode must be 0 or 1");
+    ctx->c.block_fused = on != 0;
+  });
+}
+
 int gpb_set_lanczos_graph(gpb_ctx* ctx, int on) {
   return guarded2(ctx, [&] {
     if (on != 0 && on != 1) throw ArgError("graph mode must be 0 or 1");

@@ -134,17 +134,67 @@ constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 // launch, the panel once); group totals are transpose-reduced per wave and
 // accumulated per workgroup in LDS.  Workgroup g covers chunks g, g + G, ...
 // of 256 * PX pixels and writes partial[(i * B + r) * G + g].
+//
+// dots_chunk is one chunk's share given the panel values w (and weights dd,
+// 0 past Np); the fused kernels below call it on values they have just
+// computed, with the same chunk mapping, so their sums are the ones a
+// separate bdots_kernel launch would produce.
+template <int B, int NV, int PX>
+struct DotsAcc {
+  static constexpr int K = pow2_ceil(NV * B);
+  static_assert(K <= 64, "group too wide");
+  static constexpr int SH = 6 - ilog2(K);
+};
+// SELF: the rows are the panel itself (n = B), taken from w instead of
+// memory; the loop over row groups is then unrolled.
+template <int B, int NV, int PX, bool SELF>
+constexpr int kDotsUnroll = SELF ? (B + NV - 1) / NV : 1;
+template <int B, int NV, int PX, bool SELF = false>
+__device__ __forceinline__ void dots_chunk(const float* V, size_t ld, int n, const size_t (&pp)[PX],
+                                           const float (&w)[PX][B], const double (&dd)[PX],
+                                           double (*red)[DotsAcc<B, NV, PX>::K], double* tot, int tid) {
+  constexpr int K = DotsAcc<B, NV, PX>::K, SH = DotsAcc<B, NV, PX>::SH;
+  const int lane = tid & 63, wv = tid >> 6;
+  if (SELF) n = B;
+#pragma unroll kDotsUnroll<B, NV, PX, SELF>
+  for (int i0 = 0; i0 < n; i0 += NV) {
+    double a[K];
+#pragma unroll
+    for (int t = 0; t < K; t++) a[t] = 0.0;
+    float x[PX][NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t iv = size_t(min(i0 + v, n - 1));
+#pragma unroll
+      for (int k = 0; k < PX; k++) {
+        if constexpr (SELF) x[k][v] = w[k][min(i0 + v, B - 1)];
+        else x[k][v] = V[iv * ld + pp[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PX; k++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const double xv = double(x[k][v]) * dd[k];
+#pragma unroll
+        for (int r = 0; r < B; r++) a[v * B + r] = fma(xv, double(w[k][r]), a[v * B + r]);
+      }
+    TransposeReduce<K, 32>::run(a, lane);
+    if ((lane & ((1 << SH) - 1)) == 0) red[wv][lane >> SH] = a[0];
+    __syncthreads();
+    if (tid < NV * B && i0 + tid / B < n) tot[i0 * B + tid] += red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    __syncthreads();
+  }
+}
+
 template <int B, int NV, int PX>
 __global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V, size_t ld, int n,
                                                     const float* __restrict__ Wm, size_t ldw,
                                                     const float* __restrict__ D, size_t Np,
                                                     double* __restrict__ partial) {
-  constexpr int K = pow2_ceil(NV * B);
-  static_assert(K <= 64, "group too wide");
-  constexpr int SH = 6 - ilog2(K);
-  __shared__ double red[4][K];
+  __shared__ double red[4][DotsAcc<B, NV, PX>::K];
   __shared__ double tot[kBMaxBasis * B];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x;
   const int G = gridDim.x;
   for (int t = tid; t < n * B; t += 256) tot[t] = 0.0;
   __syncthreads();
@@ -162,33 +212,131 @@ __global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V,
 #pragma unroll
       for (int r = 0; r < B; r++) w[k][r] = Wm[size_t(r) * ldw + pp[k]];
     }
-    for (int i0 = 0; i0 < n; i0 += NV) {
-      double a[K];
-#pragma unroll
-      for (int t = 0; t < K; t++) a[t] = 0.0;
-      float x[PX][NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const size_t iv = size_t(min(i0 + v, n - 1));
-#pragma unroll
-        for (int k = 0; k < PX; k++) x[k][v] = V[iv * ld + pp[k]];
-      }
-#pragma unroll
-      for (int k = 0; k < PX; k++)
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          const double xv = double(x[k][v]) * dd[k];
-#pragma unroll
-          for (int r = 0; r < B; r++) a[v * B + r] = fma(xv, double(w[k][r]), a[v * B + r]);
-        }
-      TransposeReduce<K, 32>::run(a, lane);
-      if ((lane & ((1 << SH) - 1)) == 0) red[wv][lane >> SH] = a[0];
-      __syncthreads();
-      if (tid < NV * B && i0 + tid / B < n) tot[i0 * B + tid] += red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-      __syncthreads();
-    }
+    dots_chunk<B, NV, PX>(V, ld, n, pp, w, dd, red, tot, tid);
   }
   for (int t = tid; t < n * B; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
+}
+
+// Fused local update and re-measure: W_r <- W_r - sum_{i<n} H[i][r] V_i
+// (FP64 accumulation, skipped unless *gate == want when gate is given), then
+// the dots of rows 0 .. nd-1 of Vd against the updated W_r (no weights), as
+// bdots_kernel(Vd, nd, W) would compute them.  Vd may overlap W (the panel's
+// own Gram): each thread reads back only the pixels it has just written, so
+// V, Vd and W carry no __restrict__.
+// The coefficients come through uniform (scalar) loads; rows of the dots:
+// nd <= 2 B (the two local panels), or the panel itself with SELF.
+template <int B, int NV, int PX, bool SELF>
+__global__ __launch_bounds__(256) void bupdate_dots_kernel(const float* V, size_t ld, int n,
+                                                           const double* __restrict__ Hm, float* Wm, size_t ldw,
+                                                           size_t Np, const int* __restrict__ gate, int want,
+                                                           const float* Vd, int nd, double* __restrict__ partial) {
+  constexpr int kRows = SELF ? B : 2 * B;
+  __shared__ double red[4][DotsAcc<B, NV, PX>::K];
+  __shared__ double tot[kRows * B];
+  const bool upd = n > 0 && !(gate && *gate != want);  // uniform
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  if (SELF) nd = B;
+  for (int t = tid; t < nd * B; t += 256) tot[t] = 0.0;
+  __syncthreads();
+  const size_t nchunk = (Np + 256 * PX - 1) / (256 * PX);
+  for (size_t ch = blockIdx.x; ch < nchunk; ch += G) {
+    size_t pp[PX];
+    float w[PX][B];
+    double dd[PX];
+#pragma unroll
+    for (int k = 0; k < PX; k++) {
+      const size_t p = ch * (256 * PX) + size_t(k) * 256 + tid;
+      const bool in = p < Np;
+      pp[k] = in ? p : Np - 1;
+      dd[k] = in ? 1.0 : 0.0;
+    }
+    // the update in groups of PH pixels (FP64 accumulators of one group
+    // live at a time); basis vectors outermost within a group, so the
+    // group's loads of one vector are in flight together (per element the
+    // same FMA order as bupdate_kernel)
+    constexpr int PH = PX < 4 ? PX : 4;
+#pragma unroll
+    for (int k0 = 0; k0 < PX; k0 += PH) {
+      double acc[PH][B];
+#pragma unroll
+      for (int k = 0; k < PH; k++)
+#pragma unroll
+        for (int r = 0; r < B; r++) acc[k][r] = Wm[size_t(r) * ldw + pp[k0 + k]];
+      if (upd) {
+        for (int i = 0; i < n; i++) {
+          double h[B];
+#pragma unroll
+          for (int r = 0; r < B; r++) h[r] = Hm[i * B + r];
+#pragma unroll
+          for (int k = 0; k < PH; k++) {
+            const double v = V[size_t(i) * ld + pp[k0 + k]];
+#pragma unroll
+            for (int r = 0; r < B; r++) acc[k][r] = fma(-h[r], v, acc[k][r]);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PH; k++) {
+#pragma unroll
+        for (int r = 0; r < B; r++) w[k0 + k][r] = float(acc[k][r]);
+        if (upd && dd[k0 + k] != 0.0) {
+#pragma unroll
+          for (int r = 0; r < B; r++) Wm[size_t(r) * ldw + pp[k0 + k]] = w[k0 + k][r];
+        }
+      }
+    }
+    dots_chunk<B, NV, PX, SELF>(Vd, ld, nd, pp, w, dd, red, tot, tid);
+  }
+  for (int t = tid; t < nd * B; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
+}
+
+// Fused CholQR step: X_r <- sum_{s<=r} X_s Rinv[s][r] in place (as
+// btransform_kernel), then the Gram X^T X of the result (as
+// bdots_kernel(X, B, X)), partial[(i * B + r) * G + g].
+template <int B, int NV, int PX>
+__global__ __launch_bounds__(256) void btransform_gram_kernel(float* __restrict__ X, size_t ld,
+                                                              const double* __restrict__ Rinv, size_t Np,
+                                                              double* __restrict__ partial) {
+  __shared__ double red[4][DotsAcc<B, NV, PX>::K];
+  __shared__ double tot[B * B];
+  __shared__ double rs[B * B];
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  for (int t = tid; t < B * B; t += 256) {
+    tot[t] = 0.0;
+    rs[t] = Rinv[t];
+  }
+  __syncthreads();
+  const size_t nchunk = (Np + 256 * PX - 1) / (256 * PX);
+  for (size_t ch = blockIdx.x; ch < nchunk; ch += G) {
+    size_t pp[PX];
+    float w[PX][B];
+    double dd[PX];
+#pragma unroll
+    for (int k = 0; k < PX; k++) {
+      const size_t p = ch * (256 * PX) + size_t(k) * 256 + tid;
+      const bool in = p < Np;
+      pp[k] = in ? p : Np - 1;
+      dd[k] = in ? 1.0 : 0.0;
+      double x[B];
+#pragma unroll
+      for (int r = 0; r < B; r++) x[r] = X[size_t(r) * ld + pp[k]];
+#pragma unroll
+      for (int r = 0; r < B; r++) {
+        double a = 0;
+#pragma unroll
+        for (int q = 0; q <= r; q++) a += x[q] * rs[q * B + r];
+        w[k][r] = float(a);
+      }
+      if (in) {
+#pragma unroll
+        for (int r = 0; r < B; r++) X[size_t(r) * ld + pp[k]] = w[k][r];
+      }
+    }
+    dots_chunk<B, NV, PX, true>(X, ld, B, pp, w, dd, red, tot, tid);
+  }
+  for (int t = tid; t < B * B; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
 }
 
 // W_r <- W_r - sum_i H[i][r] V_i (FP64 accumulation); PX pixels per thread
@@ -513,10 +661,55 @@ struct BlockOps {
     }
     launch_reduce(c, partial, NC, n, dst);
   }
-  // CholQR2 of the panel X (in place); Rtot = R of X_in = X_out Rtot.
-  void cholqr2(float* X, double* G, double* Rinv, double* R1, double* Rtot, int* brk) {
+  // W_r -= sum_{i<n} Hm[i][r] V_i (only when *gate == want, with gate), then
+  // dst[i * B + r] = Vd_i . W_r for i < nd: update and dots(Vd, nd, W) in one
+  // pass over the pixels, with the same sums as the two launches.
+  void update_dots(const float* V, int n, const double* Hm, float* Wm, const int* gate, int want, const float* Vd,
+                   int nd, double* dst) {
+    if (nd > 2 * B || n > kBMaxBasis) throw ArgError("block Lanczos: fused pass takes at most two local panels");
+    {
+      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
+      hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, false>), dim3(NC), dim3(256), 0,
+                         c.stream, V, Np, n, Hm, Wm, Np, Np, gate, want, Vd, nd, partial);
+      GPB_LAUNCH_CHECK();
+    }
+    launch_reduce(c, partial, NC, nd * B, dst);
+  }
+  // W_r -= sum_{i<n} Hm[i][r] V_i, then G = W^T W (B x B) of the result.
+  void update_gram(const float* V, int n, const double* Hm, float* Wm, double* G) {
+    if (n > kBMaxBasis) throw ArgError("block Lanczos: basis above the dots kernel's capacity");
+    {
+      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
+      hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, true>), dim3(NC), dim3(256), 0,
+                         c.stream, V, Np, n, Hm, Wm, Np, Np, nullptr, 1, Wm, B, partial);
+      GPB_LAUNCH_CHECK();
+    }
+    launch_reduce(c, partial, NC, B * B, G);
+  }
+  // CholQR2 of the panel X (in place); Rtot = R of X_in = X_out Rtot.  With
+  // gram_ready, G already holds X^T X (from update_gram).  The fused form
+  // computes the second pass' Gram in the first pass' transform.
+  void cholqr2(float* X, double* G, double* Rinv, double* R1, double* Rtot, int* brk, bool gram_ready = false) {
+    if (fused) {
+      if (!gram_ready) dots(X, B, X, nullptr, G);
+      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 1, brk);
+      GPB_LAUNCH_CHECK();
+      {
+        KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
+        hipLaunchKernelGGL((btransform_gram_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD>), dim3(NC), dim3(256), 0,
+                           c.stream, X, Np, Rinv, Np, partial);
+        GPB_LAUNCH_CHECK();
+      }
+      launch_reduce(c, partial, NC, B * B, G);
+      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 2, brk);
+      GPB_LAUNCH_CHECK();
+      KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
+      hipLaunchKernelGGL(btransform_kernel<B>, dim3(vg), dim3(256), 0, c.stream, X, Np, Rinv, Np);
+      GPB_LAUNCH_CHECK();
+      return;
+    }
     for (int pass = 1; pass <= 2; pass++) {
-      dots(X, B, X, nullptr, G);
+      if (pass > 1 || !gram_ready) dots(X, B, X, nullptr, G);
       hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, pass, brk);
       GPB_LAUNCH_CHECK();
       KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
@@ -524,6 +717,7 @@ struct BlockOps {
       GPB_LAUNCH_CHECK();
     }
   }
+  bool fused = true;
 };
 
 template <int B>
@@ -540,6 +734,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   int keep = std::min(std::max(opt.keep, K + 1), M - B);
   if (M < K + 2 * B || keep < K) throw ArgError("basis too small for the block size");
   BlockOps<B> ops(c, st);
+  ops.fused = c.block_fused;
   float* V = c.ws<float>("beig.V", size_t(M + B) * Np);
   float* V2 = c.ws<float>("beig.V2", size_t(M + B) * Np);
   float* Wb = c.ws<float>("beig.W", size_t(B) * Np);
@@ -616,7 +811,11 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
         ops.dots(V, n, Pn, nullptr, H1);
         ops.update(V, n, H1, Pn);
         ops.dots(V, n, Pn, nullptr, H2);
-        ops.update(V, n, H2, Pn);
+        if (ops.fused) {
+          ops.update_gram(V, n, H2, Pn, G);  // the second pass and CholQR's first Gram
+        } else {
+          ops.update(V, n, H2, Pn);
+        }
         hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, 0, H1, H2);
         GPB_LAUNCH_CHECK();
       } else {
@@ -625,13 +824,21 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
         hipLaunchKernelGGL(bgate_kernel, dim3(1), dim3(256), 0, c.stream, H1, n, lo, B, gflag, gacc);
         GPB_LAUNCH_CHECK();
         ops.update(V, n, H1, Pn, gflag, 1);
-        ops.update(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0);
-        ops.dots(V + size_t(lo) * Np, n - lo, Pn, nullptr, H2 + size_t(lo) * B);
-        ops.update(V + size_t(lo) * Np, n - lo, H2 + size_t(lo) * B, Pn);
+        if (ops.fused) {
+          // local update with the re-measure of the two local panels, then
+          // the second local pass with CholQR's first Gram
+          ops.update_dots(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0, V + size_t(lo) * Np, n - lo,
+                          H2 + size_t(lo) * B);
+          ops.update_gram(V + size_t(lo) * Np, n - lo, H2 + size_t(lo) * B, Pn, G);
+        } else {
+          ops.update(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0);
+          ops.dots(V + size_t(lo) * Np, n - lo, Pn, nullptr, H2 + size_t(lo) * B);
+          ops.update(V + size_t(lo) * Np, n - lo, H2 + size_t(lo) * B, Pn);
+        }
         hipLaunchKernelGGL(btcol_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, B, lo, H1, H2);
         GPB_LAUNCH_CHECK();
       }
-      ops.cholqr2(Pn, G, Rinv, R1, Rtot, brk);
+      ops.cholqr2(Pn, G, Rinv, R1, Rtot, brk, ops.fused);
       k += B;
     }
     kcur = k;  // projected size

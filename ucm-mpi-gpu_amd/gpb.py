@@ -74,6 +74,7 @@ _sig("gpb_set_spmv_tile", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_reorth", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_lanczos_graph", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_block_fusion", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_verbose", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_compute_f64", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p])
@@ -476,6 +477,10 @@ class Context:
     def set_spmv_fp64(self, on: bool):
         """SpMV arithmetic in FP64 (default) or FP32; HBM operands are FP32 either way."""
         self._chk(lib.gpb_set_spmv_precision(self.h, 1 if on else 0))
+
+    def set_block_fusion(self, on: bool):
+        """Block Lanczos: fused update / dot-product passes (default) or separate ones."""
+        self._chk(lib.gpb_set_block_fusion(self.h, 1 if on else 0))
 
     def set_lanczos_graph(self, on: bool):
         """Replay single-vector Lanczos restart cycles from hipGraphs (default) or launch directly."""
