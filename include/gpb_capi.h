@@ -212,12 +212,17 @@ int gpb_set_reorth(gpb_ctx* ctx, int mode);
  * order).  No counterpart in the reference. */
 int gpb_set_lanczos_graph(gpb_ctx* ctx, int on);
 
-/* Block Lanczos for later solves on ctx: 1 (default) fuses each local
- * orthogonalisation pass with the dot products that follow it (update and
- * re-measure, update and the panel's Gram, CholQR transform and the next
- * Gram), 0 runs them as separate passes over the vectors.  The sums are the
- * same either way.  No counterpart in the reference. */
-int gpb_set_block_fusion(gpb_ctx* ctx, int on);
+/* Block Lanczos for later solves on ctx.  mode 0 runs each local
+ * orthogonalisation pass and the dot products that follow it as separate
+ * passes over the vectors; 1 fuses them (update and re-measure, update and
+ * the panel's Gram, CholQR transform and the next Gram) with the same sums
+ * as 0; 2 (default) is 1 with the second local Gram-Schmidt pass of a
+ * semi-orthogonal step run only when a column of the panel kept less than
+ * 0.01 of its norm through the first (a DGKS-style test; one pass otherwise
+ * leaves local overlaps below 100 eps_32, well inside the semi-orthogonality
+ * tolerance sqrt(eps_32) the next step's measuring pass enforces).  No
+ * counterpart in the reference. */
+int gpb_set_block_fusion(gpb_ctx* ctx, int mode);
 
 /* level > 0: the eigensolvers write one progress line per restart cycle to
  * stderr (operator applications, residual); 0 (default): silent. */

@@ -242,12 +242,12 @@ def test_block_fusion_identical_gpu(gpb, orc, ctx, block, partial):
     for fused in (False, True):
         ctx.set_block_size(block)
         ctx.set_reorth(partial)
-        ctx.set_block_fusion(fused)
+        ctx.set_block_fusion(1 if fused else 0)
         try:
             ev, vec = ctx.eigs(nm.astype(np.float32))
             st = ctx.stats()
         finally:
-            ctx.set_block_fusion(True)
+            ctx.set_block_fusion(2)
             ctx.set_reorth(True)
             ctx.set_block_size(1)
         print(f"  fused {fused}: matvecs {st['matvecs']} restarts {st['restarts']} residual {st['max_residual']:.2e}",

@@ -168,7 +168,7 @@ struct Context {
   int reorth_mode = 1;
   // single-vector Lanczos: replay restart cycles from captured hipGraphs
   bool lanczos_graph = true;
-  bool block_fused = true;  // block Lanczos: fused update / dots passes
+  int block_fusion = 2;  // block Lanczos: 0 separate passes, 1 fused, 2 fused + DGKS-gated second local pass
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
   std::map<uint64_t, int> spmv_band;  // tile-band width tuned with the rows (tiles)

@@ -377,10 +377,10 @@ int gpb_set_reorth(gpb_ctx* ctx, int mode) {
   });
 }
 
-int gpb_set_block_fusion(gpb_ctx* ctx, int on) {
+int gpb_set_block_fusion(gpb_ctx* ctx, int mode) {
   return guarded2(ctx, [&] {
-    if (on != 0 && on != 1) throw ArgError("fusion mode must be 0 or 1");
-    ctx->c.block_fused = on != 0;
+    if (mode < 0 || mode > 2) throw ArgError("fusion mode must be 0, 1 or 2");
+    ctx->c.block_fusion = mode;
   });
 }
 

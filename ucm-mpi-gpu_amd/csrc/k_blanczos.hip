@@ -26,6 +26,12 @@ constexpr int kBMax = 16;
 constexpr int kBTX = 32, kBTY = 8;
 constexpr int kBTileW = kBTX + 2 * kICRadius, kBTileH = kBTY + 2 * kICRadius;
 constexpr int kBMaxBasis = 336;
+// Second local Gram-Schmidt pass of a semi-orthogonal block step when a
+// column keeps less than eta = 0.01 of its norm through the first: one pass
+// leaves local overlaps of about eps_32 ||P|| / ||P'|| <= 100 eps_32 = 6e-6,
+// far below the semi-orthogonality tolerance sqrt(eps_32) = 2.4e-4 that the
+// next step's measuring pass enforces.
+constexpr double kBDgksEta2 = 1e-4;
 
 struct BSpmvArgs {
   int o[kStencilHalf];
@@ -223,21 +229,34 @@ __global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V,
 // bdots_kernel(Vd, nd, W) would compute them.  Vd may overlap W (the panel's
 // own Gram): each thread reads back only the pixels it has just written, so
 // V, Vd and W carry no __restrict__.
-// The coefficients come through uniform (scalar) loads; rows of the dots:
-// nd <= 2 B (the two local panels), or the panel itself with SELF.
-template <int B, int NV, int PX, bool SELF>
+// The coefficients come through uniform (scalar) loads.  MODE 0: the dots
+// of nd <= 2 B rows of Vd (the two local panels); MODE 1 (SELF): the
+// panel's own Gram; MODE 2: both, the Gram appended as rows nd .. nd+B-1.
+// In MODE 1 a skipped update (gate given and *gate != want) leaves the panel
+// as it is and returns the Gram the caller already has (skip_gram, B x B)
+// through the partials, and clears skip_zero[0 .. nzero).
+template <int B, int NV, int PX, int MODE>
 __global__ __launch_bounds__(256) void bupdate_dots_kernel(const float* V, size_t ld, int n,
                                                            const double* __restrict__ Hm, float* Wm, size_t ldw,
                                                            size_t Np, const int* __restrict__ gate, int want,
-                                                           const float* Vd, int nd, double* __restrict__ partial) {
-  constexpr int kRows = SELF ? B : 2 * B;
+                                                           const float* Vd, int nd, double* __restrict__ partial,
+                                                           const double* skip_gram, double* skip_zero, int nzero) {
+  constexpr bool SELF = MODE == 1;
+  constexpr int kRows = SELF ? B : (MODE == 2 ? 3 * B : 2 * B);
   __shared__ double red[4][DotsAcc<B, NV, PX>::K];
   __shared__ double tot[kRows * B];
   const bool upd = n > 0 && !(gate && *gate != want);  // uniform
   const int tid = threadIdx.x;
   const int G = gridDim.x;
-  if (SELF) nd = B;
-  for (int t = tid; t < nd * B; t += 256) tot[t] = 0.0;
+  if (SELF) nd = 0;
+  const int nt = (nd + (MODE == 0 ? 0 : B)) * B;  // entries written
+  if (SELF && gate && !upd) {
+    for (int t = tid; t < nt; t += 256) partial[size_t(t) * G + blockIdx.x] = blockIdx.x == 0 ? skip_gram[t] : 0.0;
+    if (blockIdx.x == 0)
+      for (int t = tid; t < nzero; t += 256) skip_zero[t] = 0.0;
+    return;
+  }
+  for (int t = tid; t < nt; t += 256) tot[t] = 0.0;
   __syncthreads();
   const size_t nchunk = (Np + 256 * PX - 1) / (256 * PX);
   for (size_t ch = blockIdx.x; ch < nchunk; ch += G) {
@@ -286,9 +305,26 @@ __global__ __launch_bounds__(256) void bupdate_dots_kernel(const float* V, size_
         }
       }
     }
-    dots_chunk<B, NV, PX, SELF>(Vd, ld, nd, pp, w, dd, red, tot, tid);
+    if (MODE != 1) dots_chunk<B, NV, PX, false>(Vd, ld, nd, pp, w, dd, red, tot, tid);
+    if (MODE != 0) dots_chunk<B, NV, PX, true>(Vd, ld, B, pp, w, dd, red, tot + nd * B, tid);
   }
-  for (int t = tid; t < nd * B; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
+  for (int t = tid; t < nt; t += 256) partial[size_t(t) * G + blockIdx.x] = tot[t];
+}
+
+// Second local pass needed?  DGKS-style test per column on the panel's
+// squared norms before the first pass (measured Gram, rows n .. n+B-1 of H1)
+// and after it (appended Gram, rows n .. n+B-1 of H2): the pass runs when a
+// column's squared norm fell below kBDgksEta2 of what it was.
+__global__ void bdgks_kernel(const double* __restrict__ H1, const double* __restrict__ H2, int n, int B,
+                             int* __restrict__ flag, double* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  bool need = false;
+  for (int r = 0; r < B; r++) {
+    const double before = H1[size_t(n + r) * B + r], after = H2[size_t(n + r) * B + r];
+    need = need || !(after >= kBDgksEta2 * before);
+  }
+  *flag = need ? 1 : 0;
+  if (need) acc[0] += 1.0;
 }
 
 // Fused CholQR step: X_r <- sum_{s<=r} X_s Rinv[s][r] in place (as
@@ -664,24 +700,33 @@ struct BlockOps {
   // W_r -= sum_{i<n} Hm[i][r] V_i (only when *gate == want, with gate), then
   // dst[i * B + r] = Vd_i . W_r for i < nd: update and dots(Vd, nd, W) in one
   // pass over the pixels, with the same sums as the two launches.
+  // With append, dst rows nd .. nd+B-1 receive W^T W of the updated panel.
   void update_dots(const float* V, int n, const double* Hm, float* Wm, const int* gate, int want, const float* Vd,
-                   int nd, double* dst) {
+                   int nd, double* dst, bool append = false) {
     if (nd > 2 * B || n > kBMaxBasis) throw ArgError("block Lanczos: fused pass takes at most two local panels");
     {
       KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
-      hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, false>), dim3(NC), dim3(256), 0,
-                         c.stream, V, Np, n, Hm, Wm, Np, Np, gate, want, Vd, nd, partial);
+      if (append)
+        hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, 2>), dim3(NC), dim3(256), 0,
+                           c.stream, V, Np, n, Hm, Wm, Np, Np, gate, want, Vd, nd, partial, nullptr, nullptr, 0);
+      else
+        hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, 0>), dim3(NC), dim3(256), 0,
+                           c.stream, V, Np, n, Hm, Wm, Np, Np, gate, want, Vd, nd, partial, nullptr, nullptr, 0);
       GPB_LAUNCH_CHECK();
     }
-    launch_reduce(c, partial, NC, nd * B, dst);
+    launch_reduce(c, partial, NC, (nd + (append ? B : 0)) * B, dst);
   }
   // W_r -= sum_{i<n} Hm[i][r] V_i, then G = W^T W (B x B) of the result.
-  void update_gram(const float* V, int n, const double* Hm, float* Wm, double* G) {
+  // With gate: only when *gate == 1; otherwise G = skip_gram (the Gram of the
+  // unchanged panel) and Hm[0 .. n*B) is cleared (no coefficients applied).
+  void update_gram(const float* V, int n, const double* Hm, float* Wm, double* G, const int* gate = nullptr,
+                   const double* skip_gram = nullptr) {
     if (n > kBMaxBasis) throw ArgError("block Lanczos: basis above the dots kernel's capacity");
     {
-      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8.0 * B));
-      hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, true>), dim3(NC), dim3(256), 0,
-                         c.stream, V, Np, n, Hm, Wm, Np, Np, nullptr, 1, Wm, B, partial);
+      KScope ks(c, kKUpdate, gate ? 0.0 : double(Np) * (4.0 * n + 8.0 * B));
+      hipLaunchKernelGGL((bupdate_dots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD, 1>), dim3(NC), dim3(256), 0,
+                         c.stream, V, Np, n, Hm, Wm, Np, Np, gate, 1, Wm, 0, partial, skip_gram,
+                         const_cast<double*>(Hm), n * B);
       GPB_LAUNCH_CHECK();
     }
     launch_reduce(c, partial, NC, B * B, G);
@@ -718,6 +763,7 @@ struct BlockOps {
     }
   }
   bool fused = true;
+  bool dgks = true;  // DGKS-gated second local pass (fused form only)
 };
 
 template <int B>
@@ -734,7 +780,8 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   int keep = std::min(std::max(opt.keep, K + 1), M - B);
   if (M < K + 2 * B || keep < K) throw ArgError("basis too small for the block size");
   BlockOps<B> ops(c, st);
-  ops.fused = c.block_fused;
+  ops.fused = c.block_fusion >= 1;
+  ops.dgks = c.block_fusion >= 2;
   float* V = c.ws<float>("beig.V", size_t(M + B) * Np);
   float* V2 = c.ws<float>("beig.V2", size_t(M + B) * Np);
   float* Wb = c.ws<float>("beig.W", size_t(B) * Np);
@@ -795,6 +842,10 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   const bool semi = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol;
   int* gflag = c.ws<int>("beig.gflag", 1);
   double* gacc = c.ws<double>("beig.gacc", 1);
+  int* dflag = c.ws<int>("beig.dflag", 1);
+  double* dacc = c.ws<double>("beig.dacc", 1);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, dacc, size_t(1), 0.0);
+  GPB_LAUNCH_CHECK();
   hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, gacc, size_t(1), 0.0);
   GPB_LAUNCH_CHECK();
   while (true) {
@@ -824,7 +875,17 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
         hipLaunchKernelGGL(bgate_kernel, dim3(1), dim3(256), 0, c.stream, H1, n, lo, B, gflag, gacc);
         GPB_LAUNCH_CHECK();
         ops.update(V, n, H1, Pn, gflag, 1);
-        if (ops.fused) {
+        if (ops.fused && ops.dgks) {
+          // local update with the re-measure of the two local panels and
+          // the panel's new Gram; the second local pass only when the DGKS
+          // test asks for it (else its coefficients are cleared, and the
+          // appended Gram is CholQR's first)
+          ops.update_dots(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0, V + size_t(lo) * Np, n - lo,
+                          H2 + size_t(lo) * B, true);
+          hipLaunchKernelGGL(bdgks_kernel, dim3(1), dim3(64), 0, c.stream, H1, H2, n, B, dflag, dacc);
+          GPB_LAUNCH_CHECK();
+          ops.update_gram(V + size_t(lo) * Np, n - lo, H2 + size_t(lo) * B, Pn, G, dflag, H2 + size_t(n) * B);
+        } else if (ops.fused) {
           // local update with the re-measure of the two local panels, then
           // the second local pass with CholQR's first Gram
           ops.update_dots(V + size_t(lo) * Np, n - lo, H1 + size_t(lo) * B, Pn, gflag, 0, V + size_t(lo) * Np, n - lo,
@@ -926,10 +987,13 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     }
   }
   if (semi) {
-    double g = 0;
+    double g = 0, d = 0;
     GPB_HIP(hipMemcpyAsync(&g, gacc, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(&d, dacc, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     stats.second_passes = int(g);
+    if (c.verbose && ops.dgks)
+      std::fprintf(stderr, "gpb: block Lanczos: %d whole-basis passes, %d second local passes\n", int(g), int(d));
   }
   stats.iterations = matvecs;
   stats.converged = 0;

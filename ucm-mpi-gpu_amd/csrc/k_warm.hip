@@ -1283,6 +1283,10 @@ void warm_train(Context& c, int first, int count, int epochs, int batch, float l
   const size_t plane = n.g0.plane();
   float* T = c.ws<float>("warm.bT", size_t(batch) * plane * kWarmOut);
   float* Db = c.ws<float>("warm.bDb", size_t(batch) * plane);
+  // the symmetry gather writes the image positions only: the padding ring
+  // the loss also reads must hold zero targets and weights
+  GPB_HIP(hipMemsetAsync(T, 0, size_t(batch) * plane * kWarmOut * sizeof(float), c.stream));
+  GPB_HIP(hipMemsetAsync(Db, 0, size_t(batch) * plane * sizeof(float), c.stream));
   Ten dy = tensor(c, "warm.dy", n.g0, kHead);
   float* grad = c.ws<float>("warm.grad", param_total());
   std::vector<int> order(static_cast<size_t>(count));

@@ -478,9 +478,9 @@ class Context:
         """SpMV arithmetic in FP64 (default) or FP32; HBM operands are FP32 either way."""
         self._chk(lib.gpb_set_spmv_precision(self.h, 1 if on else 0))
 
-    def set_block_fusion(self, on: bool):
-        """Block Lanczos: fused update / dot-product passes (default) or separate ones."""
-        self._chk(lib.gpb_set_block_fusion(self.h, 1 if on else 0))
+    def set_block_fusion(self, mode: int):
+        """Block Lanczos passes: 0 separate, 1 fused (same sums), 2 fused with the DGKS-gated second pass (default)."""
+        self._chk(lib.gpb_set_block_fusion(self.h, int(mode)))
 
     def set_lanczos_graph(self, on: bool):
         """Replay single-vector Lanczos restart cycles from hipGraphs (default) or launch directly."""
