@@ -220,7 +220,9 @@ int gpb_set_lanczos_graph(gpb_ctx* ctx, int on);
  * semi-orthogonal step run only when a column of the panel kept less than
  * 0.01 of its norm through the first (a DGKS-style test; one pass otherwise
  * leaves local overlaps below 100 eps_32, well inside the semi-orthogonality
- * tolerance sqrt(eps_32) the next step's measuring pass enforces).  No
+ * tolerance sqrt(eps_32) the next step's measuring pass enforces), and with
+ * the second CholQR pass run only when the first pass' column-scaled
+ * triangular factor has a Frobenius condition number of 1e3 or more.  No
  * counterpart in the reference. */
 int gpb_set_block_fusion(gpb_ctx* ctx, int mode);
 

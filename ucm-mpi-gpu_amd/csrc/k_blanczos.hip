@@ -32,6 +32,10 @@ constexpr int kBMaxBasis = 336;
 // far below the semi-orthogonality tolerance sqrt(eps_32) = 2.4e-4 that the
 // next step's measuring pass enforces.
 constexpr double kBDgksEta2 = 1e-4;
+// CholQR's second pass runs when the column-scaled R of the first has a
+// Frobenius condition number of at least this (eps_64 * 1e6 stays far below
+// sqrt(eps_32), with room for the FP64 summation error of the Gram).
+constexpr double kBChol2Cond = 1e3;
 
 struct BSpmvArgs {
   int o[kStencilHalf];
@@ -438,7 +442,8 @@ __global__ __launch_bounds__(256) void bwnorm_kernel(const float* __restrict__ X
 // In place X_r <- sum_s X_s Rinv[s][r] (upper triangular Rinv), FP64.
 template <int B>
 __global__ __launch_bounds__(256) void btransform_kernel(float* __restrict__ X, size_t ld, const double* __restrict__ Rinv,
-                                                         size_t Np) {
+                                                         size_t Np, const int* __restrict__ skip = nullptr) {
+  if (skip && *skip) return;  // uniform
   __shared__ double rs[B * B];
   for (int t = threadIdx.x; t < B * B; t += blockDim.x) rs[t] = Rinv[t];
   __syncthreads();
@@ -459,11 +464,27 @@ __global__ __launch_bounds__(256) void btransform_kernel(float* __restrict__ X, 
 // Cholesky G = R^T R (upper R) of the B x B Gram matrix, Rinv = R^-1.
 // pass 1 stores R into R1; pass 2 stores R2 * R1 into Rtot.  Pivots below
 // 1e-12 of the largest diagonal entry are floored and counted in *brk.
+//
+// skip2 (optional): pass 1 sets *skip2 = 1 when the column-scaled factor
+// R D^-1 (D = diag R) has Frobenius condition number below kBChol2Cond (an
+// upper bound of its 2-norm condition number) and no pivot was floored;
+// pass 2 then only sets Rinv = I and Rtot = R1, and the transform that
+// follows is skipped.  With the Gram summed in FP64 one CholQR pass loses
+// orthogonality only at eps_64 * cond^2, and the FP32 rounding of the result
+// is not amplified by the condition number.
 template <int B>
 __global__ void bchol_kernel(const double* __restrict__ G, double* __restrict__ Rinv, double* __restrict__ R1,
-                             double* __restrict__ Rtot, int pass, int* __restrict__ brk) {
+                             double* __restrict__ Rtot, int pass, int* __restrict__ brk, int* __restrict__ skip2) {
   if (threadIdx.x != 0) return;
+  if (pass == 2 && skip2 && *skip2) {
+    for (int t = 0; t < B * B; t++) {
+      Rinv[t] = (t / B == t % B) ? 1.0 : 0.0;
+      Rtot[t] = R1[t];
+    }
+    return;
+  }
   double R[B * B], Ri[B * B];
+  bool floored = false;
   double gmax = 0;
   for (int r = 0; r < B; r++) gmax = fmax(gmax, G[r * B + r]);
   for (int t = 0; t < B * B; t++) R[t] = Ri[t] = 0;
@@ -473,6 +494,7 @@ __global__ void bchol_kernel(const double* __restrict__ G, double* __restrict__ 
     if (!(d > 1e-12 * gmax)) {
       d = fmax(1e-12 * gmax, 1e-300);
       atomicAdd(brk, 1);
+      floored = true;
     }
     const double rjj = sqrt(d);
     R[j * B + j] = rjj;
@@ -494,6 +516,16 @@ __global__ void bchol_kernel(const double* __restrict__ G, double* __restrict__ 
   for (int t = 0; t < B * B; t++) Rinv[t] = Ri[t];
   if (pass == 1) {
     for (int t = 0; t < B * B; t++) R1[t] = R[t];
+    if (skip2) {
+      double a = 0, b = 0;
+      for (int i = 0; i < B; i++)
+        for (int j = i; j < B; j++) {
+          const double x = R[i * B + j] / R[j * B + j], y = Ri[i * B + j] * R[i * B + i];
+          a += x * x;
+          b += y * y;
+        }
+      *skip2 = (!floored && sqrt(a * b) < kBChol2Cond) ? 1 : 0;
+    }
   } else {
     for (int i = 0; i < B; i++)
       for (int j = 0; j < B; j++) {
@@ -733,11 +765,13 @@ struct BlockOps {
   }
   // CholQR2 of the panel X (in place); Rtot = R of X_in = X_out Rtot.  With
   // gram_ready, G already holds X^T X (from update_gram).  The fused form
-  // computes the second pass' Gram in the first pass' transform.
+  // computes the second pass' Gram in the first pass' transform; with dgks
+  // it also skips the second pass for well-conditioned panels (bchol_kernel).
   void cholqr2(float* X, double* G, double* Rinv, double* R1, double* Rtot, int* brk, bool gram_ready = false) {
     if (fused) {
+      int* skip2 = dgks ? c.ws<int>("beig.skip2", 1) : nullptr;
       if (!gram_ready) dots(X, B, X, nullptr, G);
-      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 1, brk);
+      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 1, brk, skip2);
       GPB_LAUNCH_CHECK();
       {
         KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
@@ -746,16 +780,16 @@ struct BlockOps {
         GPB_LAUNCH_CHECK();
       }
       launch_reduce(c, partial, NC, B * B, G);
-      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 2, brk);
+      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, 2, brk, skip2);
       GPB_LAUNCH_CHECK();
-      KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
-      hipLaunchKernelGGL(btransform_kernel<B>, dim3(vg), dim3(256), 0, c.stream, X, Np, Rinv, Np);
+      KScope ks(c, kKUpdate, skip2 ? 0.0 : double(Np) * 8.0 * B);
+      hipLaunchKernelGGL(btransform_kernel<B>, dim3(vg), dim3(256), 0, c.stream, X, Np, Rinv, Np, skip2);
       GPB_LAUNCH_CHECK();
       return;
     }
     for (int pass = 1; pass <= 2; pass++) {
       if (pass > 1 || !gram_ready) dots(X, B, X, nullptr, G);
-      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, pass, brk);
+      hipLaunchKernelGGL(bchol_kernel<B>, dim3(1), dim3(64), 0, c.stream, G, Rinv, R1, Rtot, pass, brk, nullptr);
       GPB_LAUNCH_CHECK();
       KScope ks(c, kKUpdate, double(Np) * 8.0 * B);
       hipLaunchKernelGGL(btransform_kernel<B>, dim3(vg), dim3(256), 0, c.stream, X, Np, Rinv, Np);
@@ -763,7 +797,7 @@ struct BlockOps {
     }
   }
   bool fused = true;
-  bool dgks = true;  // DGKS-gated second local pass (fused form only)
+  bool dgks = true;  // gated second local pass and CholQR second pass (fused form only)
 };
 
 template <int B>
