@@ -91,6 +91,17 @@ constexpr int kCueRmax = 20;
 constexpr int kCueTileW = kCueTile + 2 * kCueRmax;  // 48
 constexpr int kCueLanes = kCueTile * kCueTile;      // 64
 constexpr int kNbMax = 32;                          // largest histogram (textons)
+// Histogram rows per slice group: kNbMax bins plus one sink bin that the
+// padding entries of the slice-sorted offset lists update and no snapshot reads.
+constexpr int kNbStride = kNbMax + 1;
+// Sentinel rows below the label tile (value kNbMax = the sink bin): a padding
+// entry's offset reaches them from every lane's centre.
+constexpr int kCueSentinelRows = kCueTile;
+constexpr int kCuePadOffset = (kCueTileW - kCueRmax) * kCueTileW;
+// Slice-sorted rings: 5 < d <= 10 and 10 < d <= 20, each list of a slice
+// padded to a multiple of kCueBatch entries.
+constexpr int kCueRings = 2;
+constexpr int kCueBatch = 8;
 
 struct CueParams {
   const uint8_t* plane[4];  // L, a, b, texton labels (H x W, unmirrored)
@@ -98,6 +109,8 @@ struct CueParams {
   const int* off;           // disc offsets (LDS tile offset) sorted by radius
   const uint8_t* slice;     // slice index per offset
   int end_by_r[21];         // number of offsets with d^2 <= r^2
+  const int* ring_off;      // rings 5 < d <= 10 and 10 < d <= 20, offsets sorted by slice, padded
+  int ring_seg[kCueRings * kNumSlices + 1];  // start of (ring, slice) list i in ring_off
   float smooth[3][9];       // half kernels: [0] bg (sigma 2.5, 17 taps), [1] cg (sigma 1.25, 9 taps), [2] none
   float w_mpb[12], w_gpb[12];
   double smooth_d[3][9];    // the same in FP64 (the all-FP64 variant)
@@ -154,7 +167,7 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
     const CT cb = csum[b];
 #pragma unroll
     for (int grp = 0; grp < 4; grp++) {
-      const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+      const uint32_t w = hist[(grp * kNbStride + b) * kCueLanes + lane];
 #pragma unroll
       for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(cb, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
     }
@@ -171,7 +184,7 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
     CT u = CT(0), v = CT(0);
 #pragma unroll
     for (int grp = 0; grp < 4; grp++) {
-      const uint32_t w = hist[(grp * kNbMax + b) * kCueLanes + lane];
+      const uint32_t w = hist[(grp * kNbStride + b) * kCueLanes + lane];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int s = 4 * grp + q;
@@ -229,8 +242,8 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
       // slice o + 8 from the left to the right
       const CT a0 = CT(2) * (oh ? alpha[4 + q] : alpha[q]);
       const CT a8 = -CT(2) * (oh ? alpha[kNumOri + 4 + q] : alpha[kNumOri + q]);
-      const uint32_t* h0 = hist + ((oh >> 2) * kNbMax) * kCueLanes + lane;
-      const uint32_t* h8 = hist + (((oh >> 2) + 2) * kNbMax) * kCueLanes + lane;
+      const uint32_t* h0 = hist + ((oh >> 2) * kNbStride) * kCueLanes + lane;
+      const uint32_t* h8 = hist + (((oh >> 2) + 2) * kNbStride) * kCueLanes + lane;
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         const CT c0 = CT((h0[b * kCueLanes] >> (8 * q)) & 0xffu);
@@ -255,7 +268,7 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
     if (H == 1) gy = 0; else { const int p2 = 2 * H; gy %= p2; if (gy < 0) gy += p2; if (gy >= H) gy = p2 - 1 - gy; }
     tile[i] = plane[size_t(gy) * W + gx];
   }
-  for (int i = 0; i < 4 * kNbMax; i++) hist[i * kCueLanes + lane] = 0u;
+  for (int i = 0; i < 4 * kNbStride; i++) hist[i * kCueLanes + lane] = 0u;
   const int nb = P.nbins[ch];
   const CT* g = smooth_of<CT>(P, KH == 8 ? 0 : (KH == 4 ? 1 : 2));
   if (lane < kNbMax) {
@@ -276,6 +289,25 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
   for (int o = 0; o < kNumOri; o++) chf[o] = CT(0);
   int i = 0;
   for (int ri = 0; ri < 3; ri++) {
+    if (radii[ri] > 5) {
+      // outer rings from the slice-sorted lists: within a slice the histogram
+      // group is a constant offset and the increment a constant
+      const int ring = radii[ri] == 10 ? 0 : 1;
+      const __attribute__((address_space(4))) int* roff = (const __attribute__((address_space(4))) int*)P.ring_off;
+#pragma unroll
+      for (int sl = 0; sl < kNumSlices; sl++) {
+        uint32_t* hs = hist + (sl >> 2) * kNbStride * kCueLanes + lane;
+        const uint32_t inc = 1u << (8 * (sl & 3));
+        const int e = P.ring_seg[ring * kNumSlices + sl + 1];
+        for (int j = P.ring_seg[ring * kNumSlices + sl]; j < e; j += kCueBatch) {
+          uint32_t lab[kCueBatch];
+#pragma unroll
+          for (int k = 0; k < kCueBatch; k++) lab[k] = centre[roff[j + k]];
+#pragma unroll
+          for (int k = 0; k < kCueBatch; k++) atomicAdd(&hs[lab[k] * kCueLanes], inc);
+        }
+      }
+    } else {
     const int end = P.end_by_r[radii[ri]];
     // 8 label reads are issued before their 8 histogram updates (read and
     // update one at a time, each read waited behind the previous update: the
@@ -308,13 +340,14 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
         for (int k = 0; k < 8; k++) lab[k] = centre[off_c[k]];
 #pragma unroll
         for (int k = 0; k < 8; k++)
-          atomicAdd(&hist[((sl_c[k] >> 2) * kNbMax + lab[k]) * kCueLanes + lane], 1u << (8 * (sl_c[k] & 3)));
+          atomicAdd(&hist[((sl_c[k] >> 2) * kNbStride + lab[k]) * kCueLanes + lane], 1u << (8 * (sl_c[k] & 3)));
       }
     }
     for (; i < end; i++) {
       const int s = P.slice[i];
       const uint32_t lab = centre[P.off[i]];
-      atomicAdd(&hist[((s >> 2) * kNbMax + lab) * kCueLanes + lane], 1u << (8 * (s & 3)));
+      atomicAdd(&hist[((s >> 2) * kNbStride + lab) * kCueLanes + lane], 1u << (8 * (s & 3)));
+    }
     }
     CT chi[kNumOri];
     chi2_snapshot<NB, KH, CT>(hist, lane, nb, g, csum, chi);
@@ -356,10 +389,11 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
 
 template <typename CT>
 __global__ __launch_bounds__(64) void cues_kernel(CueParams P) {
-  __shared__ uint8_t tile[kCueTileW * kCueTileW];
-  __shared__ uint32_t hist[4 * kNbMax * kCueLanes];
+  __shared__ uint8_t tile[kCueTileW * (kCueTileW + kCueSentinelRows)];
+  __shared__ uint32_t hist[4 * kNbStride * kCueLanes];
   __shared__ CT csum[kNbMax];
   const int lane = threadIdx.x;
+  for (int i = lane; i < kCueSentinelRows * kCueTileW; i += kCueLanes) tile[kCueTileW * kCueTileW + i] = uint8_t(kNbMax);
   const int px = blockIdx.x * kCueTile + (lane % kCueTile), py = (blockIdx.y + P.by0) * kCueTile + (lane / kCueTile);
   const bool valid = px < P.W && py < P.H;
   CT mpb[kNumOri], gc[kNumOri];
@@ -463,6 +497,8 @@ __global__ __launch_bounds__(256) void nonmax_kernel(const VT* __restrict__ a, c
 namespace {
 struct CueTables {
   DevBuf<int> off;
+  DevBuf<int> ring_off;
+  int ring_seg[kCueRings * kNumSlices + 1];
   DevBuf<uint8_t> slice;
   DevBuf<double> gamma;
   int end_by_r[21];
@@ -494,6 +530,24 @@ static CueTables& cue_tables(Context& c) {
     int e = 0;
     while (e < int(offs.size()) && offs[size_t(e)][0] <= r * r) e++;
     t->end_by_r[r] = e;
+  }
+  // the two outer rings, by slice, each slice's list padded to kCueBatch
+  // entries with the sentinel offset
+  {
+    std::vector<int> roff;
+    int n = 0;
+    for (int ring = 0; ring < kCueRings; ring++) {
+      const int r0 = ring == 0 ? 5 : 10, r1 = ring == 0 ? 10 : 20;
+      for (int sidx = 0; sidx < kNumSlices; sidx++) {
+        t->ring_seg[n++] = int(roff.size());
+        for (size_t i = 0; i < offs.size(); i++)
+          if (offs[i][0] > r0 * r0 && offs[i][0] <= r1 * r1 && sl[i] == sidx) roff.push_back(off[i]);
+        while (roff.size() % kCueBatch) roff.push_back(kCuePadOffset);
+      }
+    }
+    t->ring_seg[n] = int(roff.size());
+    t->ring_off.alloc(roff.size());
+    GPB_HIP(hipMemcpy(t->ring_off.p, roff.data(), roff.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   t->off.alloc(off.size());
   t->slice.alloc(sl.size());
@@ -546,6 +600,8 @@ static void local_cues_t(Context& c, const uint32_t* d_rgb, const uint8_t* d_tex
   P.nbins = nb;
   P.off = t.off.p;
   P.slice = t.slice.p;
+  P.ring_off = t.ring_off.p;
+  for (int q = 0; q <= kCueRings * kNumSlices; q++) P.ring_seg[q] = t.ring_seg[q];
   for (int r = 0; r <= kCueRmax; r++) P.end_by_r[r] = t.end_by_r[r];
   for (int a = 0; a < 3; a++)
     for (int k = 0; k < 9; k++) {
