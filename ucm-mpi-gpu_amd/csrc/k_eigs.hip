@@ -539,22 +539,6 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
   }
 }
 
-// After a gated reorthogonalisation: beta_j from ||w||^2 and the coupling.
-// The removed components stay out of T, which keeps the tridiagonal form the
-// three-term steps and the omega recurrence assume (recording them at
-// isolated steps made the restarted iteration diverge).
-__global__ void pro_finish_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ nrm2,
-                                  double* __restrict__ beta, const int* __restrict__ flag) {
-  if (*flag == 0) return;
-  const double bn = sqrt(*nrm2);
-  beta[j] = bn;
-  beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
-  if (j + 1 < m) {
-    T[size_t(j) * m + j + 1] = bn;
-    T[size_t(j + 1) * m + j] = bn;
-  }
-}
-
 // omega_{j+1}[k] = omega_j[k] = eps for k <= j after a full
 // Gram-Schmidt step j (the first of a cycle); pending cleared.
 __global__ void omega_reset_kernel(double* __restrict__ om_a, double* __restrict__ om_b, int j,
@@ -569,6 +553,36 @@ __global__ void omega_reset_kernel(double* __restrict__ om_a, double* __restrict
 __global__ __launch_bounds__(256) void scale_kernel(const float* __restrict__ w, const double* __restrict__ beta, int j,
                                                     float* __restrict__ v, size_t Np) {
   const double ib = beta[kMaxBasis + j];
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
+    v[p] = float(double(w[p]) * ib);
+}
+
+// Last launch of a partially reorthogonalised step: v_{j+1} = w / beta_j.
+// When the gated whole-basis pass ran (*flag), every workgroup takes beta_j
+// from the norm that pass left (the same expression in each) and workgroup 0
+// records it in beta and T; otherwise beta_j is the one omega_kernel recorded.
+// The components the gated pass removed stay out of T, which keeps the
+// tridiagonal form the three-term steps and the omega recurrence assume
+// (recording them at isolated steps made the restarted iteration diverge).
+__global__ __launch_bounds__(256) void pro_scale_kernel(const float* __restrict__ w, double* __restrict__ beta, int j,
+                                                        float* __restrict__ v, size_t Np, double* __restrict__ T,
+                                                        int m, const double* __restrict__ nrm2,
+                                                        const int* __restrict__ flag) {
+  double ib;
+  if (*flag) {  // uniform
+    const double bn = sqrt(*nrm2);
+    ib = bn > 1e-300 ? 1.0 / bn : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      beta[j] = bn;
+      beta[kMaxBasis + j] = ib;
+      if (j + 1 < m) {
+        T[size_t(j) * m + j + 1] = bn;
+        T[size_t(j + 1) * m + j] = bn;
+      }
+    }
+  } else {
+    ib = beta[kMaxBasis + j];
+  }
   for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
     v[p] = float(double(w[p]) * ib);
 }
@@ -964,8 +978,10 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         GPB_LAUNCH_CHECK();
         red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
         red.update(V, Np, n, h2, w, nrm + 1, flag);
-        hipLaunchKernelGGL(pro_finish_kernel, dim3(1), dim3(1), 0, c.stream, T, m, j, nrm + 1, beta, flag);
+        hipLaunchKernelGGL(pro_scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np, T,
+                           m, nrm + 1, flag);
         GPB_LAUNCH_CHECK();
+        continue;
       } else {
         // three-term step, one full projection pass, and a second pass only
         // when the DGKS test asks for it
