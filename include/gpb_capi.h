@@ -45,6 +45,7 @@ typedef struct gpb_stats {
   int reorth_second_passes; /* Gram-Schmidt steps that needed a second pass */
   int warm_mode;        /* eigensolve start: 0 plain, 1 learned block, 2 random block */
   double warm_ms;       /* warm-start network inference time (mode 1) */
+  int reorth_redone_steps; /* optimistic-cycle steps repeated with the gated pass (counted in lanczos_matvecs) */
 } gpb_stats;
 
 /* Context lifetime.  Replaces the per-call device selection and
@@ -198,13 +199,24 @@ int gpb_get_spmv_tile(gpb_ctx* ctx, int width, int height, int* tile);
 int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
 
 /* Reorthogonalisation of the single-vector Lanczos steps for later solves on
- * ctx: 1 (default) = partial (Simon's omega recurrence on the device: the
+ * ctx.  1 and 2 = partial (Simon's omega recurrence on the device: the
  * whole-basis Gram-Schmidt pass runs only when the estimated loss of
  * orthogonality exceeds sqrt(2^-24)) for residual targets >= 1e-4, and the
  * whole-basis pass every step below that; 0 = the whole-basis pass every
- * step.  No counterpart in the reference (Spectra reorthogonalises fully,
- * SymEigsSolver.h:200-280; damascene not at all, lanczos.cu:192-282). */
+ * step.  With 1 the gated pass is launched on every step and exits at once
+ * when not needed.  With 2 (default) a restart cycle runs without those
+ * launches and is redone from the first step whose recurrence triggered,
+ * with them; the results equal those of 1.  The block and batched solvers
+ * treat 1 and 2 alike.  No counterpart in the reference (Spectra
+ * reorthogonalises fully, SymEigsSolver.h:200-280; damascene not at all,
+ * lanczos.cu:192-282). */
 int gpb_set_reorth(gpb_ctx* ctx, int mode);
+
+/* Trigger level of the partial reorthogonalisation's omega recurrence for
+ * later solves on ctx (0 = sqrt(2^-24), the default).  Lower levels make the
+ * gated pass run more often; for verification.  No counterpart in the
+ * reference. */
+int gpb_set_reorth_threshold(gpb_ctx* ctx, double level);
 
 /* Single-vector Lanczos for later solves on ctx: 1 (default) replays each
  * restart cycle after the first from a hipGraph captured once per basis

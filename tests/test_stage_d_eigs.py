@@ -257,3 +257,38 @@ def test_block_fusion_identical_gpu(gpb, orc, ctx, block, partial):
     assert out[True][2] == out[False][2]
     np.testing.assert_array_equal(out[True][0], out[False][0])
     np.testing.assert_array_equal(out[True][1], out[False][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tol,level", [(1e-3, 0.0), (1e-3, 1e-6), (1e-4, 1e-5)])
+def test_optimistic_cycles_identical_gpu(gpb, orc, ctx, tol, level):
+    """Partial reorthogonalisation with optimistic restart cycles (mode 2, the
+    default: no gated launches, a cycle redone from the first step whose omega
+    recurrence triggered) against the gated launches on every step (mode 1):
+    identical eigenvalues and eigenvectors, and the operator applications of
+    mode 1 plus the redone steps.  Lowered trigger levels make the recurrence
+    fire, so that the redo path runs."""
+    print(f"optimistic cycles tol {tol} level {level}", flush=True)
+    nm = _tiny_w(orc, gpb, 256, 224, 4)
+    ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol, max_matvecs=40000)
+    out = {}
+    for mode in (1, 2):
+        ctx.set_reorth(mode)
+        ctx.set_reorth_threshold(level)
+        try:
+            ev, vec = ctx.eigs(nm.astype(np.float32))
+            st = ctx.stats()
+        finally:
+            ctx.set_reorth(True)
+            ctx.set_reorth_threshold(0.0)
+        print(f"  mode {mode}: matvecs {st['matvecs']} restarts {st['restarts']} gated passes "
+              f"{st['second_passes']} redone steps {st['redone_steps']} residual {st['max_residual']:.2e}", flush=True)
+        out[mode] = (ev, vec, st)
+    s1, s2 = out[1][2], out[2][2]
+    assert s1["redone_steps"] == 0
+    assert s2["matvecs"] == s1["matvecs"] + s2["redone_steps"]
+    assert s2["second_passes"] == s1["second_passes"]
+    if level > 0:
+        assert s1["second_passes"] > 0 and s2["redone_steps"] > 0
+    np.testing.assert_array_equal(out[2][0], out[1][0])
+    np.testing.assert_array_equal(out[2][1], out[1][1])

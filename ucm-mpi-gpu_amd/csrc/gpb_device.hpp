@@ -87,6 +87,7 @@ struct LanczosStats {
   double max_residual = 0;  // max relative generalized residual of returned pairs
   int fallback = 0;         // warm start rejected, plain solve used
   int second_passes = 0;    // Gram-Schmidt steps that needed a second pass
+  int redone_steps = 0;     // optimistic-cycle steps repeated with the gated pass
   double pass2_bytes[2] = {0, 0};
 };
 
@@ -165,7 +166,8 @@ struct Context {
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
   // single-vector Lanczos steps: 1 (default) partial reorthogonalisation for
   // residual targets >= 1e-4, 0 whole-basis Gram-Schmidt every step
-  int reorth_mode = 1;
+  int reorth_mode = 2;         // 0 every step, 1 partial (gated launches), 2 partial (optimistic cycles)
+  double pro_threshold = 0;    // omega trigger level, 0 = sqrt(eps_32)
   // single-vector Lanczos: replay restart cycles from captured hipGraphs
   bool lanczos_graph = true;
   int block_fusion = 2;  // block Lanczos: 0 separate passes, 1 fused, 2 fused + DGKS-gated second local pass

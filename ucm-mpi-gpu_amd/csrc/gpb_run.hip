@@ -372,8 +372,15 @@ int gpb_set_spmv_tile(gpb_ctx* ctx, int tile) {
 
 int gpb_set_reorth(gpb_ctx* ctx, int mode) {
   return guarded2(ctx, [&] {
-    if (mode != 0 && mode != 1) throw ArgError("reorthogonalisation mode must be 0 (every step) or 1 (partial)");
+    if (mode < 0 || mode > 2) throw ArgError("reorthogonalisation mode must be 0 (every step), 1 or 2 (partial)");
     ctx->c.reorth_mode = mode;
+  });
+}
+
+int gpb_set_reorth_threshold(gpb_ctx* ctx, double level) {
+  return guarded2(ctx, [&] {
+    if (!(level >= 0) || level >= 1) throw ArgError("reorthogonalisation threshold must be in [0, 1)");
+    ctx->c.pro_threshold = level;
   });
 }
 
@@ -600,6 +607,7 @@ int gpb_get_stats(gpb_ctx* ctx, gpb_stats* out) {
     out->max_residual = ctx->c.last_lanczos.max_residual;
     out->fallback = ctx->c.last_lanczos.fallback;
     out->reorth_second_passes = ctx->c.last_lanczos.second_passes;
+    out->reorth_redone_steps = ctx->c.last_lanczos.redone_steps;
     out->warm_mode = ctx->c.warm_mode;
     out->warm_ms = ctx->c.warm_ms;
     for (int i = 0; i < kMaxNvec; i++) out->evals[i] = ctx->c.last_evals[i];

@@ -787,7 +787,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
   double* nsecond = c.ws<double>("bat.nsecond", 1);
   double* Ydev = c.ws<double>("bat.Y", size_t(np) * m * keep);
   // partial reorthogonalisation for targets the FP64 refinement does not follow
-  const bool pro = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  const bool pro = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
   double* om = c.ws<double>("bat.omega", size_t(np) * 2 * kOmS);
   int* pend = c.ws<int>("bat.pending", size_t(np));
   int* rows_dev = c.ws<int>("bat.rows", size_t(K + 1));

@@ -873,7 +873,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   bool converged = false;
   int kcur = 0;
   // semi-orthogonal block steps for targets the FP64 refinement does not follow
-  const bool semi = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol;
+  const bool semi = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol;
   int* gflag = c.ws<int>("beig.gflag", 1);
   double* gacc = c.ws<double>("beig.gacc", 1);
   int* dflag = c.ws<int>("beig.dflag", 1);

@@ -486,12 +486,18 @@ __global__ __launch_bounds__(256) void three_term_kernel(const float* __restrict
 // reorthogonalisation gate; T[j][j] = alpha and, without a reorthogonalisation,
 // beta[j] and the coupling T[j][j+1].  om1 overwrites om_1 in place.
 // acc: byte and pass counters as for the DGKS gate.
+//
+// omp holds omega_{j-1}, om0 omega_j; omega_{j+1} goes to om_1 (which may
+// alias omp: each thread reads its entry before writing it).  thr is the
+// trigger level (sqrt(eps) by default).  With first_viol (the optimistic
+// cycle, no gated launches), the first step whose omega exceeds thr is
+// recorded there and the pass counters are left alone.
 __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int m, int j,
                                                     const double* __restrict__ alpha, double* __restrict__ beta,
                                                     const double* __restrict__ nrm_partial, int G,
-                                                    const double* __restrict__ om0, double* __restrict__ om_1,
+                                                    const double* __restrict__ om0, const double* omp, double* om_1,
                                                     int* __restrict__ flag, int* __restrict__ pending, double np,
-                                                    double* __restrict__ acc) {
+                                                    double* __restrict__ acc, double thr, int* __restrict__ first_viol) {
   __shared__ double red4[4];
   __shared__ double mx[4];
   const int t = threadIdx.x;
@@ -504,7 +510,7 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
   if (t < j) {
     double tw = T[size_t(t) * m + j];  // om0[j] = 1
     for (int l = 0; l < j; l++) tw += T[size_t(t) * m + l] * om0[l];
-    const double prev = (t == j - 1) ? 1.0 : om_1[t];
+    const double prev = (t == j - 1) ? 1.0 : omp[t];
     const double e = tw - al * om0[t] - bc * prev;
     const double noise = kProEps * (bn + fabs(T[size_t(t) * m + t + 1]));
     om = (e + (e >= 0 ? noise : -noise)) / (bn > 1e-300 ? bn : 1e-300);
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
   __syncthreads();
   const double worst = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
   const int pend = *pending;
-  const bool viol = worst > kProSqrtEps;
+  const bool viol = worst > thr;
   const bool f = viol || pend;
   // after a reorthogonalisation the new vector is orthogonal to rounding level
   if (t < j) om_1[t] = f ? kProEps : om;
@@ -524,11 +530,12 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
     T[size_t(j) * m + j] = al;
     *flag = f ? 1 : 0;
     *pending = (viol && !pend) ? 1 : 0;
-    if (f) {
+    if (first_viol && viol && *first_viol > j) *first_viol = j;
+    if (f && !first_viol) {
       acc[0] += np * (4.0 * (j + 1) + 4);
       acc[1] += np * (4.0 * (j + 1) + 8);
       acc[2] += 1.0;
-    } else {
+    } else if (!f) {
       beta[j] = bn;
       beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
       if (j + 1 < m) {
@@ -881,11 +888,20 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   // partial reorthogonalisation for targets the FP64 refinement does not
   // follow (see kRefineBelowTol); whole-basis Gram-Schmidt every step otherwise
-  const bool pro = c.reorth_mode == 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
-  double* om = c.ws<double>("eig.omega", size_t(2) * (kMaxBasis + 1));
+  const bool pro = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  // optimistic cycles (mode 2): no gated launches; a cycle whose omega
+  // recurrence triggered is redone from that step with them (below)
+  const bool optimistic = pro && c.reorth_mode == 2;
+  const double pro_thr = c.pro_threshold > 0 ? c.pro_threshold : kProSqrtEps;
+  // one omega vector per step, so that a cycle can be resumed at any step
+  double* om = c.ws<double>("eig.omega", size_t(kMaxBasis + 2) * (kMaxBasis + 1));
+  int* first_viol = c.ws<int>("eig.first_viol", 1);
+  int* zero_flag = c.ws<int>("eig.zero", 1);
+  hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, zero_flag, 0);
+  GPB_LAUNCH_CHECK();
   int* pend = c.ws<int>("eig.pending", 1);
   double* rnrm = c.ws<double>("eig.rnrm", 1);
-  auto om_slot = [&](int i) { return om + size_t(i & 1) * (kMaxBasis + 1); };
+  auto om_slot = [&](int i) { return om + size_t(i) * (kMaxBasis + 1); };
   Reducer red(c, Np);
   const unsigned vg = vec_grid(Np);
   launch_fill(c, w, Np, 0.f);
@@ -931,10 +947,16 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   // which of V / V2 is current, so it is captured once per buffer into a
   // hipGraph and replayed; the first cycle, and every cycle while per-kernel
   // timing is on, launches directly.
-  auto cycle = [&](int k0, float* Vc) {
+  // j0 > k0 resumes a cycle at step j0 (its steps before j0 already done),
+  // launching the gated whole-basis pass on every step.
+  auto cycle = [&](int k0, float* Vc, int j0, bool opt_cycle) {
     float* const V = Vc;
     const int k = k0;
-    for (int j = k; j < m; j++) {
+    if (opt_cycle) {
+      hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, first_viol, kMaxBasis + 1);
+      GPB_LAUNCH_CHECK();
+    }
+    for (int j = j0; j < m; j++) {
       float* vj = V + size_t(j) * Np;
       const int n = j + 1;
       if (j == k || m > kProjectMaxN) {
@@ -974,12 +996,15 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
           GPB_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(omega_kernel, dim3(1), dim3(256), 0, c.stream, T, m, j, alpha, beta, red.npart, red.GU,
-                           om_slot(j), om_slot(j + 1), flag, pend, double(Np), gacc);
+                           om_slot(j), om_slot(j - 1), om_slot(j + 1), flag, pend, double(Np), gacc, pro_thr,
+                           opt_cycle ? first_viol : nullptr);
         GPB_LAUNCH_CHECK();
-        red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
-        red.update(V, Np, n, h2, w, nrm + 1, flag);
+        if (!opt_cycle) {
+          red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
+          red.update(V, Np, n, h2, w, nrm + 1, flag);
+        }
         hipLaunchKernelGGL(pro_scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np, T,
-                           m, nrm + 1, flag);
+                           m, nrm + 1, opt_cycle ? zero_flag : flag);
         GPB_LAUNCH_CHECK();
         continue;
       } else {
@@ -1020,7 +1045,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         GPB_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         c.capturing = true;
         try {
-          cycle(k, V);
+          cycle(k, V, k, optimistic);
         } catch (...) {
           c.capturing = false;
           hipGraph_t dummy;
@@ -1034,13 +1059,34 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       }
       GPB_HIP(hipGraphLaunch(ge, c.stream));
     } else {
-      cycle(k, V);
+      cycle(k, V, k, optimistic);
     }
     matvecs += m - k;
     // Rayleigh-Ritz on the host
+    int viol_h = kMaxBasis + 1;
     GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    if (optimistic) GPB_HIP(hipMemcpyAsync(&viol_h, first_viol, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
+    if (optimistic && viol_h < m) {
+      // the omega recurrence triggered at step viol_h: every step before it
+      // matches the gated form, so redo the cycle from there with the gated
+      // pass.  The steps after it are discarded; of what they leave behind,
+      // the twice-rule flag is cleared (the gated form has it clear at that
+      // step) and so is w: the SpMV writes interior pixels only, so w's zero
+      // halo must not carry their values (unnormalisable vectors there scale
+      // to inf and NaN).  Basis vectors up to that step predate it, later ones
+      // are rewritten whole.
+      hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, pend, 0);
+      GPB_LAUNCH_CHECK();
+      launch_fill(c, w, Np, 0.f);
+      cycle(k, V, viol_h, false);
+      matvecs += m - viol_h;
+      stats.redone_steps += m - viol_h;
+      GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
+      GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+      GPB_HIP(hipStreamSynchronize(c.stream));
+    }
     std::vector<double> A = Th;
     {
       // smallest Ritz values first

@@ -63,7 +63,8 @@ class Stats(C.Structure):
     _fields_ = [("stage_ms", C.c_double * 6), ("lanczos_matvecs", C.c_int), ("lanczos_restarts", C.c_int),
                 ("lanczos_basis", C.c_int), ("lanczos_block", C.c_int), ("converged", C.c_int),
                 ("max_residual", C.c_double), ("evals", C.c_double * 32), ("fallback", C.c_int),
-                ("reorth_second_passes", C.c_int), ("warm_mode", C.c_int), ("warm_ms", C.c_double)]
+                ("reorth_second_passes", C.c_int), ("warm_mode", C.c_int), ("warm_ms", C.c_double),
+                ("reorth_redone_steps", C.c_int)]
 
 
 _sig("gpb_get_stats", C.c_int, [_ctxp, C.POINTER(Stats)])
@@ -75,6 +76,7 @@ _sig("gpb_set_spmv_precision", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_reorth", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_lanczos_graph", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_set_block_fusion", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_reorth_threshold", C.c_int, [_ctxp, C.c_double])
 _sig("gpb_set_verbose", C.c_int, [_ctxp, C.c_int])
 _sig("gpb_compute_f64", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p])
@@ -255,7 +257,8 @@ class Context:
     def _stats_dict(s) -> dict:
         return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
                     basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
-                    evals=list(s.evals), fallback=s.fallback, second_passes=s.reorth_second_passes)
+                    evals=list(s.evals), fallback=s.fallback, second_passes=s.reorth_second_passes,
+                    redone_steps=s.reorth_redone_steps)
 
     def patch_stats(self, patch: int) -> dict:
         s = Stats()
@@ -289,7 +292,8 @@ class Context:
         return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
                     basis=s.lanczos_basis, block=s.lanczos_block, converged=s.converged, max_residual=s.max_residual,
                     evals=list(s.evals), fallback=s.fallback,
-                    second_passes=s.reorth_second_passes, warm_mode=s.warm_mode, warm_ms=s.warm_ms)
+                    second_passes=s.reorth_second_passes, warm_mode=s.warm_mode, warm_ms=s.warm_ms,
+                    redone_steps=s.reorth_redone_steps)
 
     def build_w(self, mpb_nmax: np.ndarray):
         m = np.ascontiguousarray(mpb_nmax, np.float32)
@@ -486,9 +490,15 @@ class Context:
         """Replay single-vector Lanczos restart cycles from hipGraphs (default) or launch directly."""
         self._chk(lib.gpb_set_lanczos_graph(self.h, 1 if on else 0))
 
-    def set_reorth(self, partial: bool):
-        """Partial reorthogonalisation (default, residual targets >= 1e-4) or the whole-basis pass every step."""
-        self._chk(lib.gpb_set_reorth(self.h, 1 if partial else 0))
+    def set_reorth(self, partial):
+        """Partial reorthogonalisation (True: the default, optimistic cycles; 1: gated launches every step) or
+        the whole-basis pass every step (False / 0)."""
+        mode = (2 if partial else 0) if isinstance(partial, bool) else int(partial)
+        self._chk(lib.gpb_set_reorth(self.h, mode))
+
+    def set_reorth_threshold(self, level: float):
+        """Trigger level of the omega recurrence (0 = sqrt(2^-24), the default)."""
+        self._chk(lib.gpb_set_reorth_threshold(self.h, C.c_double(level)))
 
     def spmv_tile(self, width: int, height: int) -> int:
         r = C.c_int(0)
