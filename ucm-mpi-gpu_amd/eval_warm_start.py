@@ -131,7 +131,7 @@ def main():
                           max_residual=float(max(r[k]["residual"] for r in per_image)),
                           mean_warm_ms=float(np.mean([r[k]["warm_ms"] for r in per_image])))
     for m in modes:
-        if m == 0:
+        if m == 0 or 0 not in modes:
             continue
         k = names[m]
         sp = [r["plain"]["ms"] / r[k]["ms"] for r in per_image]
