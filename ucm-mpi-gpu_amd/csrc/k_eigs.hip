@@ -636,8 +636,9 @@ __global__ __launch_bounds__(256) void ritz_kernel(const float* __restrict__ V, 
 // ncp = nk rounded up to NC columns, so that each row's NC coefficients are
 // read unconditionally as 16-byte pairs.  LDS bandwidth bounds a broadcast
 // read per FMA, so each coefficient pair read serves PX pixels per thread
-// (pixels p0 + k * stride, coalesced per k); 4 basis rows x PX pixels of
-// loads are in flight before their FMAs.  One pass over V per NC columns.
+// (pixels p0 + k * stride, coalesced per k); the loads of the next 4 basis
+// rows x PX pixels are issued before the current rows' FMAs where registers
+// allow (NC * PX <= 64).  One pass over V per NC columns.
 template <int NC, int PX>
 __global__ __launch_bounds__(256) void ritz_lds_kernel(const float* __restrict__ V, size_t ld, int m, int nk,
                                                        const double* __restrict__ Y, float* __restrict__ out,
@@ -663,12 +664,34 @@ __global__ __launch_bounds__(256) void ritz_lds_kernel(const float* __restrict__
       for (int k = 0; k < PX; k++)
 #pragma unroll
         for (int c = 0; c < NC; c++) acc[k][c] = 0.0;
-      for (int j0 = 0; j0 < m; j0 += 4) {
-        float v[4][PX];
+      // with kPrefetch, rows j0 .. j0+3 are consumed while rows j0+4 .. j0+7
+      // load (only where the extra registers keep the occupancy)
+      constexpr bool kPrefetch = NC * PX <= 64;
+      float v[4][PX], vn[4][PX];
+      if constexpr (kPrefetch) {
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
-          for (int k = 0; k < PX; k++) v[t][k] = V[size_t(j0 + t < m ? j0 + t : m - 1) * ld + pp[k]];
+          for (int k = 0; k < PX; k++) vn[t][k] = V[size_t(t < m ? t : m - 1) * ld + pp[k]];
+      }
+      for (int j0 = 0; j0 < m; j0 += 4) {
+        if constexpr (kPrefetch) {
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int k = 0; k < PX; k++) v[t][k] = vn[t][k];
+          if (j0 + 4 < m) {  // uniform
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+              for (int k = 0; k < PX; k++) vn[t][k] = V[size_t(j0 + 4 + t < m ? j0 + 4 + t : m - 1) * ld + pp[k]];
+          }
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int k = 0; k < PX; k++) v[t][k] = V[size_t(j0 + t < m ? j0 + t : m - 1) * ld + pp[k]];
+        }
 #pragma unroll
         for (int t = 0; t < 4; t++) {
           if (j0 + t >= m) break;  // uniform
