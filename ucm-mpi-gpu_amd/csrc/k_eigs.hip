@@ -781,7 +781,7 @@ void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const dou
     if (nc == 16)
       hipLaunchKernelGGL((ritz_lds_kernel<16, 4>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
     else if (nc == 24)
-      hipLaunchKernelGGL((ritz_lds_kernel<24, 4>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
+      hipLaunchKernelGGL((ritz_lds_kernel<24, 2>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
     else
       hipLaunchKernelGGL((ritz_lds_kernel<32, 2>), grid, dim3(256), lds, c.stream, V, ld, m, nk, Ydev, out, ld, Np);
     GPB_LAUNCH_CHECK();
