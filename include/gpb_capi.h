@@ -297,9 +297,13 @@ int gpb_spb_from_evecs(gpb_ctx* ctx, int width, int height, int nvec, const floa
 
 /* Whole gPb path on host buffers.  gpb_max[N]: max over orientations of gPb,
  * 0 where mPb_nmax <= 0.05; gpb_all[8][W][H]: calibration sigmoid of each
- * oriented gPb, column-major (index x*H + y), orientation o = the split angle
- * o*pi/8 of the half-disc cues; gpb_orient[8][N]: the oriented gPb before
- * the sigmoid (GlobalPb's output); mpb_nmax[N].  Any output may be NULL.
+ * oriented gPb, column-major (index x*H + y), block i holding orientation
+ * 7 - i (orientation o = the split angle o*pi/8 of the half-disc cues), the
+ * block order of computeGPb's hostGPbAllConcat (oriMap {4,5,6,7,0,1,2,3},
+ * damascene.cu:598-603, over filter slots s that hold orientation
+ * (3 - s) mod 8, spectralPb.cu:101,706-712); gpb_orient[8][N]: the oriented
+ * gPb before the sigmoid, orientation o in block o (GlobalPb's output);
+ * mpb_nmax[N].  Any output may be NULL.
  * Replaces computeGPb (damascene/include/globalPb.h:37,
  * damascene/src/damascene.cu:249-661). */
 int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, float* gpb_max,

@@ -144,10 +144,12 @@ void write_opencv_yaml(const std::string& path, const std::vector<std::pair<std:
 /* Whole gPb of one packed image on GPU rank % getCudaDeviceCount():
  * data = H*W words 0x00BBGGRR (r in bits 0-7); *hostGPb = malloc'd H*W
  * floats (max over orientations, 0 where mPb_nmax <= 0.05); *hostGPbAllConcat
- * = malloc'd 8 blocks of W*H floats, block o column-major (index x*H + y),
- * the calibration sigmoid of the oriented gPb at split angle o * pi / 8 (the
- * reference's oriMap {4,5,6,7,0,1,2,3} over its internal orientations lands
- * on the same order).  The caller frees both with free().  9 eigenvectors to
+ * = malloc'd 8 blocks of W*H floats, block i column-major (index x*H + y),
+ * the calibration sigmoid of the oriented gPb at split angle (7 - i) * pi / 8:
+ * the reference's oriMap {4,5,6,7,0,1,2,3} (damascene.cu:598-603) over filter
+ * slots s that hold the split angle ((3 - s) mod 8) * pi / 8 (the filter for
+ * pi/2 + (7 - s) pi/8 applied row-major, spectralPb.cu:101,706-712).  The
+ * caller frees both with free().  9 eigenvectors to
  * residual 1e-3 (damascene/src/damascene.cu:307-309).  Calls on one device
  * are serialised.  Replaces computeGPb (damascene/include/globalPb.h:37,
  * damascene/src/damascene.cu:249-661). */

@@ -971,7 +971,11 @@ void orc_gpb(const double* cues /* [12][8][N] */, const double* spb /* [8][N] */
 
 /* computeGPb output transforms (damascene globalPb.cu:5-49, skeleton.cu:229-249,
  * damascene.cu:594-604): max over orientations masked by mPb > 0.05, and the
- * calibration sigmoid of each oriented map, written column-major. */
+ * calibration sigmoid of each oriented map, written column-major with
+ * orientation o in block 7 - o (block i = slot oriMap[i] = (i + 4) mod 8,
+ * damascene.cu:598-603; slot s = the filter built for angle
+ * pi/2 + (7 - s) pi/8, spectralPb.cu:706-712, applied row-major,
+ * spectralPb.cu:101, which is orientation (3 - s) mod 8 here). */
 void orc_gpb_outputs(const double* gpb /* [8][N] */, const double* mpb_nmax, int W, int H, float* gmax, float* gall) {
   size_t N = (size_t)W * H;
   for (size_t p = 0; p < N; p++) {
@@ -991,6 +995,6 @@ void orc_gpb_outputs(const double* gpb /* [8][N] */, const double* mpb_nmax, int
         if (value < 0) value = 0;
         value = 1.0f / (1.0f + expf(2.6433 - 10.7998 * value));
         value = (value - 0.0667) / 0.9333;
-        gall[(size_t)o * N + (size_t)x * H + y] = value;
+        gall[(size_t)(7 - o) * N + (size_t)x * H + y] = value;
       }
 }

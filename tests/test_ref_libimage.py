@@ -74,6 +74,58 @@ def test_nonmax(refl, gpb):
     np.testing.assert_array_equal(refl.nonmax(mpb_all), refl.ref_nonmax(mpb_all))
 
 
+def damascene_block_orientations(orc):
+    """Which of our orientations lands in each block of computeGPb's
+    hostGPbAllConcat, derived step by step from the reference's own code:
+
+    1. slot rule: the loop at damascene/src/spectralPb/spectralPb.cu:706-712
+       writes gaussian_2D(angle pi/2 + k pi/8) into slot 7 - k;
+    2. kernel indexing: spectralPb_kernel (spectralPb.cu:83-662) multiplies the
+       texel at (ix + u, iy + v) by slot[(3 + u) + (3 + v) * 7], i.e. it
+       correlates the image with the slot's 49 floats read as a row-major
+       7 x 7 array (row = y offset v, column = x offset u), the same
+       convention as our filters F[o][a][b] (a = y offset, b = x offset);
+    3. oriMap {4,5,6,7,0,1,2,3} (damascene/src/damascene.cu:598-603): block i
+       is copied from slot oriMap[i].
+
+    Returns (block -> our orientation, slot -> our orientation); each slot is
+    matched to the one filter of ours equal to it up to sign (sPb sums
+    absolute responses), to FP32 rounding."""
+    mine = orc.spb_filters().reshape(8, 7, 7)
+    slot_ori = {}
+    for k in range(8):
+        slot = 7 - k
+        applied = orc.ref_spb_filter(np.pi / 2 + k * np.pi / 8).astype(np.float64).reshape(7, 7)
+        tol = 1e-6 * np.abs(applied).max()
+        hits = [o for o in range(8)
+                if min(np.abs(mine[o] - applied).max(), np.abs(mine[o] + applied).max()) <= tol]
+        assert len(hits) == 1, (k, hits)
+        slot_ori[slot] = hits[0]
+    ori_map = [4, 5, 6, 7, 0, 1, 2, 3]
+    return [slot_ori[ori_map[i]] for i in range(8)], slot_ori
+
+
+def test_gpb_all_block_order_vs_damascene(orc, gpb):
+    """hostGPbAllConcat block i holds our orientation (7 - i) mod 8 (derived
+    from the reference's filters, slot rule, kernel indexing and oriMap), and
+    gpb_kernel / the oracle's output transform write that order: through the
+    oracle's transform, block i of a map whose orientation o is the constant
+    o reads (7 - i) mod 8 after the calibration sigmoid is inverted."""
+    if not orc.REF_FILTERS.exists():
+        pytest.skip("reference filters not built (oracle/_ref needs the reference tree)")
+    blocks, slots = damascene_block_orientations(orc)
+    assert slots == {s: (3 - s) % 8 for s in range(8)}
+    assert blocks == [(7 - i) % 8 for i in range(8)]
+    assert blocks == list(gpb.GPB_ALL_BLOCK_ORIENTATION)
+    H, W = 3, 5
+    g = np.stack([np.full((H, W), 0.01 * (o + 1)) for o in range(8)])
+    _, gall = orc.gpb_outputs(g, np.ones((H, W)))
+    # the calibration is monotone: rank the blocks by value
+    order = np.argsort(gall[:, 0, 0])
+    got = [int(np.where(order == i)[0][0]) for i in range(8)]
+    assert got == blocks
+
+
 def test_spb_filters_vs_damascene(orc):
     """The oracle's (and the GPU's) 8 sPb filters are damascene's gaussian_2D
     filters (spectralPb.cu:706-712), which damascene stores x-major at angle

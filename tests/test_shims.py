@@ -140,3 +140,37 @@ def test_shims_match_capi_gpu(gpb, ctx, tmp_path):
     d1, d2 = np.abs(gmax - out9["gpb_max"]).max(), np.abs(gall - out9["gpb_all"]).max()
     print(f"  computeGPb: |d gmax| {d1:.2e} |d gall| {d2:.2e}", flush=True)
     assert d1 <= 1e-4 and d2 <= 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("edge", ["vertical", "horizontal"])
+def test_compute_gpb_block_order_gpu(tmp_path, edge):
+    """computeGPb (through gpb_cli) on a step edge: at the pixels where the
+    masked gPb maximum lies on the edge, the largest hostGPbAllConcat block is
+    the one that holds the edge's orientation in damascene's order, block
+    7 - o for orientation o (a vertical edge is orientation 0, a horizontal
+    one orientation 4: the half-disc split runs along the edge)."""
+    W, H = 64, 48
+    print(f"computeGPb block order, {edge} edge {W}x{H}", flush=True)
+    rng = np.random.default_rng(21)
+    yy, xx = np.mgrid[0:H, 0:W]
+    step = (xx >= W // 2) if edge == "vertical" else (yy >= H // 2)
+    img = np.where(step, 190.0, 70.0)[..., None] + rng.normal(0, 3.0, (H, W, 3))
+    rgb = np.clip(np.rint(img), 0, 255).astype(np.uint8)
+    _write_ppm(tmp_path / "step.ppm", rgb)
+    _cli("compute", tmp_path / "step.ppm", tmp_path / "out.bin")
+    raw = np.fromfile(tmp_path / "out.bin", np.float32)
+    gmax = raw[:W * H].reshape(H, W)
+    gall = raw[W * H:].reshape(8, W, H).transpose(0, 2, 1)  # block, y, x
+    o = 0 if edge == "vertical" else 4
+    want = 7 - o
+    if edge == "vertical":
+        ys = np.arange(8, H - 8)
+        xs = gmax[ys].argmax(axis=1)
+        assert np.all(np.abs(xs - (W // 2 - 0.5)) <= 1.5), xs
+    else:
+        xs = np.arange(8, W - 8)
+        ys = gmax[:, xs].argmax(axis=0)
+        assert np.all(np.abs(ys - (H // 2 - 0.5)) <= 1.5), ys
+    blocks = gall[:, ys, xs].argmax(axis=0)
+    assert np.all(blocks == want), np.bincount(blocks, minlength=8)

@@ -171,7 +171,11 @@ void launch_spb_f64(Context& c, const Stencil64& st, const double* d_evecs, cons
 }
 
 // gPb_o = gcue_o + w_sPb * sPb_o; max masked by mPb_nmax > 0.05; sigmoid maps
-// written column-major through an LDS transpose.
+// written column-major through an LDS transpose, orientation o into block
+// 7 - o: computeGPb's block i holds the slot oriMap[i] = (i + 4) mod 8
+// (damascene/src/damascene.cu:598-603), and slot s holds the filter that
+// damascene builds for angle pi/2 + (7 - s) pi/8 (spectralPb.cu:706-712) and
+// applies row-major (spectralPb.cu:101), which is our orientation (3 - s) mod 8.
 constexpr int kGT = 32;
 __global__ __launch_bounds__(kGT * 8) void gpb_kernel(const float* __restrict__ gcue, const float* __restrict__ spb,
                                                       const float* __restrict__ nmax, int W, int H, float wspb,
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(kGT * 8) void gpb_kernel(const float* __restrict__ 
     const int x = bx + r, y = by + threadIdx.x;
     if (x < W && y < H)
 #pragma unroll
-      for (int o = 0; o < kNumOri; o++) gall[o * N + size_t(x) * H + y] = tr[o][threadIdx.x][r];
+      for (int o = 0; o < kNumOri; o++) gall[(kNumOri - 1 - o) * N + size_t(x) * H + y] = tr[o][threadIdx.x][r];
   }
 }
 

@@ -122,6 +122,12 @@ def warm_layers() -> list:
                what="warm_layer_info")
         out.append((ci.value, co.value, r.value, wo.value, bo.value))
     return out
+# computeGPb's hostGPbAllConcat (gpb_all): block i holds orientation
+# GPB_ALL_BLOCK_ORIENTATION[i] (split angle o * pi / 8 of the half-disc cues),
+# damascene's order (oriMap {4,5,6,7,0,1,2,3} over filter slots that hold
+# orientation (3 - s) mod 8; damascene/src/damascene.cu:598-603,
+# spectralPb/spectralPb.cu:101,706-712).
+GPB_ALL_BLOCK_ORIENTATION = (7, 6, 5, 4, 3, 2, 1, 0)
 KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb", "warm"]
 
 
