@@ -13,8 +13,16 @@ step) and names the class with the largest device time; in the timed steps
 events bracket only that class's launches on the library's stream, and
 `roofline` describes it from those; its `copy_gbs` is the rate (bytes read +
 written) of a plain 1 GiB device copy on the same GPU, the practical streaming
-ceiling next to the published peak.  `cpu_baseline` times the CPU oracle (one
-thread) on a bounded crop of the same image.
+ceiling next to the published peak.
+
+`cpu_baseline` times the reference's own CPU path (oracle/ref_cpu_path.py: its
+lib_image cues, buildW affinity and Spectra eigensolver, compiled from the
+reference tree into oracle/_ref, with the oracle's port for the texton and
+sPb stages the reference cannot build here) on 256x256 patches of the same
+image -- the reference's deployment shape, one process per patch (an 8x8
+Img2Pieces cut of the 2048x2048 image) -- on the host's cores: one patch
+alone on one thread, and up to 16 patches at once, one process each.  It runs
+first, in child processes, before this process touches the GPU.
 """
 from __future__ import annotations
 
@@ -60,18 +68,59 @@ def copy_rate_gbs(torch, mib: int = 1024, reps: int = 10) -> float:
     return rate
 
 
-def cpu_baseline(packed, codebook, crop: int, tol: float):
-    """Time the CPU oracle (single thread) on the top-left crop x crop pixels."""
+def cpu_gen(path: str, size: int, regions: int, patch: int, count: int):
+    """Child mode: the bench image's first `count` patch x patch tiles (row-major
+    over the image) and its codebook, written to an .npz for the CPU run."""
     import numpy as np
 
-    orc = _load("gpb_oracle_binding", ROOT / "oracle" / "oracle.py")
-    sub = np.ascontiguousarray(packed[:crop, :crop])
-    t0 = time.perf_counter()
-    orc.pipeline(sub, codebook, K=9, tol=tol, m=48)
-    dt = time.perf_counter() - t0
-    return dict(value=round(crop * crop / dt / 1e6, 5), unit="Mpixel/s", cores=1, kind="port",
-                sample=f"CPU oracle (C, FP64, one thread), whole gPb path on the top-left {crop}x{crop} crop of the "
-                       f"same image, eigen-residual <= {tol:g}; {dt:.2f} s")
+    gpb = _load("gpb_amd_binding", ROOT / "ucm-mpi-gpu_amd" / "gpb.py")
+    packed = gpb.pack_rgb(gpb.synth_voronoi(size, size, regions, 8.0, 2))
+    cb = gpb.make_codebook(packed, seed=2, iters=10)
+    g = size // patch
+    tiles = [packed[(i // g) * patch:(i // g + 1) * patch, (i % g) * patch:(i % g + 1) * patch]
+             for i in range(min(count, g * g))]
+    np.savez(path, patches=np.stack(tiles), codebook=cb)
+
+
+def cpu_baseline(size: int, regions: int, patch: int = 256, workers: int = 16):
+    """The reference's own CPU path on patch x patch tiles of the bench image,
+    run in child processes (see the module docstring); None when oracle/_ref is
+    not built."""
+    import subprocess
+    import tempfile
+
+    if not all((ROOT / "oracle" / "_ref" / n).exists()
+               for n in ("libref_libimage.so", "libref_buildw.so", "libref_spectra.so")):
+        return None
+    workers = max(1, min(workers, os.cpu_count() or 1, len(os.sched_getaffinity(0))))
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    with tempfile.TemporaryDirectory() as td:
+        npz, res = os.path.join(td, "patches.npz"), os.path.join(td, "cpu.json")
+        # the children's output (buildW prints progress bars) stays off this process' stdout
+        for cmd in ([sys.executable, str(ROOT / "bench.py"), "--cpu-gen", npz, "--size", str(size), "--regions",
+                     str(regions), "--cpu-patch", str(patch), "--cpu-workers", str(workers)],
+                    [sys.executable, str(ROOT / "oracle" / "ref_cpu_path.py"), "--input", npz, "--workers",
+                     str(workers), "--out", res]):
+            p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+            if p.returncode != 0:
+                raise RuntimeError(f"CPU baseline step failed ({p.returncode}): {' '.join(cmd)}\n{p.stdout[-2000:]}")
+        r = json.loads(Path(res).read_text())
+    px = patch * patch
+    mean_patch = sum(r["parallel_patch_seconds"]) / len(r["parallel_patch_seconds"])
+    n_c2 = (size // patch) ** 2
+    return dict(value=round(r["workers"] * px / r["parallel_wall_seconds"] / 1e6, 6), unit="Mpixel/s",
+                cores=r["workers"], kind="reference",
+                sample=(f"{r['workers']} of the {n_c2} {patch}x{patch} patches of the {size}x{size} image, one process "
+                        f"(one thread) per patch as the reference runs one MPI rank per patch: the reference's own "
+                        f"lib_image cues, buildW and Spectra SymGEigsSolver (nev 9, ncv 18, tol 1e-10) compiled "
+                        f"from its tree, the oracle's port for textons and sPb/gPb; "
+                        f"{r['parallel_wall_seconds']:.1f} s wall"),
+                single_thread=dict(value=round(px / r["single_seconds"] / 1e6, 6), seconds=round(r["single_seconds"], 2),
+                                   stage_seconds={k: round(v, 2) for k, v in r["single_stages"].items()},
+                                   spectra_operations=r["single_operations"], converged=r["single_converged"]),
+                mean_patch_seconds=round(mean_patch, 2), host_nproc=r["nproc"], host_affinity=r["affinity"],
+                whole_image_seconds_est=round(n_c2 * mean_patch / r["workers"], 1),
+                c3_4096_patch_split_seconds_est=round((4096 // patch) ** 2 * mean_patch / r["workers"], 1))
 
 
 def measured_traffic(kernel_class: str, size: int):
@@ -172,7 +221,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--regions", type=int, default=400)
-    ap.add_argument("--cpu-crop", type=int, default=320)
+    ap.add_argument("--cpu-patch", type=int, default=256)
+    ap.add_argument("--cpu-workers", type=int, default=16)
+    ap.add_argument("--cpu-gen", default="", help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=1, help="Lanczos block size (1 = single-vector)")
     ap.add_argument("--basis", type=int, default=0)
@@ -187,10 +238,17 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.cpu_gen:
+        cpu_gen(args.cpu_gen, args.size, args.regions, args.cpu_patch, args.cpu_workers)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the gPb path runs one image (or one patch batch) per GPU, and this "
               "build has no multi-GPU launcher; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline first, in child processes, before this process touches the GPU
+    cpu = None if (args.no_cpu or args.copy_probe) else cpu_baseline(args.size, args.regions, args.cpu_patch,
+                                                                     args.cpu_workers)
 
     import numpy as np
     import torch
@@ -296,8 +354,8 @@ def main():
         line["forms"]["warm_start"] = bench_warm(gpb, ctx, step, S, args.steps, tol)
     if "batched" in forms:
         line["forms"]["batched_patches"] = bench_patches(gpb, ctx, tol)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(packed, codebook, args.cpu_crop, tol)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
     ctx.close()
 
