@@ -83,6 +83,28 @@ int gpb_make_codebook(const uint32_t* packed, int width, int height, uint64_t se
  * (src/SPFeatures.cpp:277-360).  Host pointers. */
 int gpb_texton(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, uint8_t* labels);
 
+/* Texton histogram features of superpixels, for the superpixel classifier
+ * (GLSVM): the texton map of stage (a) (gpb_texton on packed, codebook), then
+ * per superpixel l and texton t: descr[l][t] = 10 * (pixels of l with texton
+ * t) / (pixels of l), FP64, divided then scaled as the reference does;
+ * superpixels[N] holds labels in [0, nlabels) (GPB_ERR_ARG otherwise), and a
+ * label with no pixels gets zeros.  Replaces SPFeatures::genTextonFeatures
+ * (src/SPFeatures.cpp:185-276), whose pixel-to-codeword assignment never
+ * updates its running minimum (:234-242) -- here the nearest codeword, as in
+ * gpb_texton.  Host pointers. */
+int gpb_texton_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                        const int32_t* superpixels, int nlabels, double* descr);
+
+/* Texton histograms of the window of up to 65 x 65 pixels around each centre
+ * (row, col) in centres[2 * ncentres], clipped at the image border (north =
+ * min(32, row), south = min(32, H - 1 - row), likewise west / east):
+ * descr[k][t] = 10 * (window pixels with texton t) / (window pixels).
+ * Replaces the texton part of SPFeatures::genRecPatchFeatures
+ * (src/SPFeatures.cpp:478-545, textonDescrs); centres outside the image are
+ * rejected with GPB_ERR_ARG.  Host pointers. */
+int gpb_texton_patch_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                              int ncentres, const int32_t* centres, double* descr);
+
 /* ---- stage (b): local cues and mPb ------------------------------------------ */
 
 /* Gamma 2.5, RGB->Lab (D65), normalise, quantise to 25 bins; bins[3][N]
@@ -107,6 +129,7 @@ int gpb_local_cues(gpb_ctx* ctx, int width, int height, const uint32_t* packed, 
  * and the clip of src/multiscalePb.cpp:251-261. */
 int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float* nmax);
 
+
 /* ---- stages (c)-(e): affinity, eigenvectors, sPb, gPb ---------------------- */
 
 /* Eigensolver settings for later calls on ctx: nvec eigenpairs (2..32,
@@ -116,6 +139,17 @@ int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float*
  * applications.  Replaces the hard-coded nEigNum/fEigTolerance of computeGPb
  * (damascene.cu:307-309) and nvec of spectralPb (src/globalPb.cpp:65). */
 int gpb_set_eig_options(gpb_ctx* ctx, int nvec, int basis, int keep, double tol, int max_matvecs);
+
+/* Convergence rule for later solves on ctx.  0 (default): a pair has
+ * converged when its residual ||(D-W)v - lambda D v|| / ||D v|| is at most the
+ * tol of gpb_set_eig_options (damascene's fixed tolerance, damascene.cu:309,
+ * as a relative residual).  1: when that residual is at most tol times the
+ * pair's Ritz value (the trivial pair: times the smallest nontrivial one), the
+ * rule of the reference's CPU eigensolver (Spectra num_converged,
+ * Spectra/SymEigsSolver.h:325-334, called with tol 1e-10 from
+ * src/spectralPb.cpp:236).  The eigenvalues then come out of an FP64
+ * Rayleigh-Ritz pass over the returned vectors. */
+int gpb_set_eig_convergence(gpb_ctx* ctx, int rel);
 
 /* Lanczos block size for later calls on ctx: 1 = single-vector thick-restart
  * Lanczos (default); 4, 8, 12 or 16 = block thick-restart Lanczos, which
@@ -223,6 +257,17 @@ int gpb_set_reorth_threshold(gpb_ctx* ctx, double level);
  * buffer, 0 launches every kernel directly (the same kernels in the same
  * order).  No counterpart in the reference. */
 int gpb_set_lanczos_graph(gpb_ctx* ctx, int on);
+
+/* Single-vector Lanczos steps with partial reorthogonalisation for later
+ * solves on ctx: 1 (default) fuses each three-term step into one pass over
+ * the pixels -- the stencil SpMV with the previous step's scaling in its tile
+ * staging and this step's update and dot products in its epilogue -- and one
+ * small kernel for alpha, beta and the omega recurrence; 0 launches the
+ * separate kernels (SpMV, reduction, three-term update, omega, scaling).  The
+ * two differ only in FP32 rounding order.  No counterpart in the reference
+ * (damascene's step is several launches with blocking host reductions,
+ * lanczos.cu:192-282). */
+int gpb_set_lanczos_fusion(gpb_ctx* ctx, int on);
 
 /* Block Lanczos for later solves on ctx.  mode 0 runs each local
  * orthogonalisation pass and the dot products that follow it as separate
@@ -362,6 +407,17 @@ int gpb_compute_patches_device(gpb_ctx* ctx, int width, int height, const uint32
  * (residual, restarts). */
 int gpb_get_patch_stats(gpb_ctx* ctx, int patch, gpb_stats* out);
 
+/* The spectral result of patch `patch` of the last patch call on ctx, for
+ * patch rectangle rects[patch] (w x h): evecs[nvec][h][w] (generalised
+ * eigenvectors, v' D v = 1, eigenvalue order as gpb_get_patch_stats reports
+ * them), degree[h][w] (D), w_upper[40][h][w] (the affinities to the 40 upper
+ * disc offsets, as gpb_build_w returns them) and mpb_nmax[h][w]; any may be
+ * NULL.  A later patch call replaces them.  No counterpart in the reference
+ * (each rank keeps its eigenvectors internal, src/main.cpp:161-301): it serves
+ * checks of the batched solve. */
+int gpb_patch_eigenvectors(gpb_ctx* ctx, int patch, int nvec, float* evecs, float* degree, float* w_upper,
+                           float* mpb_nmax);
+
 /* ---- the all-FP64 path (precision reference) ---------------------------------------- */
 
 /* Whole gPb with every stage in FP64 on the GPU: the same texton labels, then
@@ -379,6 +435,29 @@ int gpb_compute_f64(gpb_ctx* ctx, int width, int height, const uint32_t* packed,
 /* The FP64 eigensolve alone on a given mpb_nmax (FP64 W and Lanczos):
  * evals[nvec], evecs[nvec][N] (may be NULL). */
 int gpb_eigs_f64(gpb_ctx* ctx, int width, int height, const double* mpb_nmax, double* evals, double* evecs);
+
+/* ---- matrix framing (moving outputs between processes) ------------------------- */
+
+/* Byte framing of the reference's matrix messages for transports other than
+ * MPI.  The reference sends a matrix as a header of four ints {rows, cols,
+ * channels, OpenCV type} followed by rows * cols * channels elements of the
+ * type's depth (MpiSendMat / MpiRecvMat, src/utilities.cpp:382-413), and a
+ * vector of matrices as one int, the count, followed by those message pairs
+ * (MpiSendMatVec / MpiRecvMatVec, :415-460).  A frame is those messages back to
+ * back in one buffer: int32 count, then per matrix the four int32 header words
+ * and the payload, host byte order, no padding (a single MpiSendMat message
+ * pair is a one-matrix frame without its count).  dims[4 * i + 0..3] = rows,
+ * cols, channels, type with type = depth + 8 * (channels - 1) and depth one
+ * of OpenCV's 0..6 (8U, 8S, 16U, 16S, 32S, 32F, 64F; the gPb maps are CV_64FC1
+ * = 6).  No counterpart beyond the MPI calls it stands in for. */
+/* Bytes of the frame of count matrices (0 for invalid dims). */
+size_t gpb_frame_size(int count, const int* dims);
+/* Writes the frame of count matrices (payload i at data[i]) into out. */
+int gpb_frame_mats(int count, const int* dims, const void* const* data, void* out, size_t out_size);
+/* Reads a frame: *count matrices (GPB_ERR_ARG above max_count), their
+ * dims[4 * i ..] and the byte offset of each payload in buf; GPB_ERR_ARG for a
+ * truncated or malformed frame, or one with trailing bytes. */
+int gpb_unframe_mats(const void* buf, size_t size, int max_count, int* count, int* dims, size_t* offsets);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 

@@ -139,6 +139,13 @@ bool read_ppm(const std::string& path, int& width, int& height, std::vector<uint
  * (%YAML:1.0, "name: !!opencv-matrix" with rows, cols, dt: d, data). */
 void write_opencv_yaml(const std::string& path, const std::vector<std::pair<std::string, const Mat*>>& mats);
 
+/* The byte frame of a vector of CV_64F matrices (gpb_frame_mats): what
+ * MpiSendMatVec sends (src/utilities.cpp:415-438), in one buffer. */
+std::vector<uint8_t> frame_mat_vec(const std::vector<Mat>& mats);
+/* Inverse of frame_mat_vec (MpiRecvMatVec, src/utilities.cpp:440-460);
+ * throws std::invalid_argument on a malformed frame or a non-CV_64F matrix. */
+std::vector<Mat> unframe_mat_vec(const std::vector<uint8_t>& frame);
+
 }  // namespace gpb_ucm
 
 /* Whole gPb of one packed image on GPU rank % getCudaDeviceCount():

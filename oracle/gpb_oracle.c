@@ -233,12 +233,24 @@ int orc_gaussian(double sigma, double* out /* >= 2*ceil(3 sigma)+1 */) {
  * Histogram length = max(label)+1 over the whole input.  Only positions at
  * least r from the input edge are evaluated (all others are trimmed by the
  * caller); out[o][(row-b)*(Wm-2b) + (col-b)] for b = border. */
+static void hist_gradient_nb(const uint8_t* labels, int Wm, int Hm, int border, int r, const double* kern,
+                             int klen, int nb_force, double* out);
 void orc_hist_gradient(const uint8_t* labels, int Wm, int Hm, int border, int r, const double* kern, int klen,
                        double* out) {
+  hist_gradient_nb(labels, Wm, Hm, border, r, kern, klen, 0, out);
+}
+
+/* As orc_hist_gradient with the histogram length given (nb_force > 0) instead
+ * of max(label) + 1 over the input: the length of a whole image, for a
+ * window computed on a crop of it (the smoothing of the bins depends on the
+ * length at the top bins). */
+static void hist_gradient_nb(const uint8_t* labels, int Wm, int Hm, int border, int r, const double* kern,
+                             int klen, int nb_force, double* out) {
   int size = 2 * r + 1;
   int nb = 0;
   for (size_t i = 0; i < (size_t)Wm * Hm; i++)
     if (labels[i] + 1 > nb) nb = labels[i] + 1;
+  if (nb_force > 0) nb = nb_force > nb ? nb_force : nb;
   double* weights = (double*)calloc((size_t)size * size, sizeof(double));
   int* smap = (int*)malloc(sizeof(int) * size * size);
   for (int ox = 0; ox < size; ox++)
@@ -331,7 +343,16 @@ void orc_hist_gradient(const uint8_t* labels, int Wm, int Hm, int border, int r,
  * cues[12][8][N] (bg1..3, cga1..3, cgb1..3, tg1..3), mpb_all[8][N]. */
 static const double MPB_W[12] = {0.0146, 0.0145, 0.0163, 0.0210, 0.0243, 0.0287,
                                  0.0166, 0.0185, 0.0204, 0.0101, 0.0111, 0.0141};
+void orc_local_cues_nb(const uint32_t* packed, const uint8_t* textons, int W, int H, const int* nb4, double* cues,
+                       double* mpb_all);
 void orc_local_cues(const uint32_t* packed, const uint8_t* textons, int W, int H, double* cues, double* mpb_all) {
+  orc_local_cues_nb(packed, textons, W, H, NULL, cues, mpb_all);
+}
+
+/* nb4 (optional): histogram lengths of the L, a, b and texton channels (see
+ * hist_gradient_nb). */
+void orc_local_cues_nb(const uint32_t* packed, const uint8_t* textons, int W, int H, const int* nb4, double* cues,
+                       double* mpb_all) {
   const int B = 30;
   size_t N = (size_t)W * H;
   int Wm = W + 2 * B, Hm = H + 2 * B;
@@ -347,10 +368,10 @@ void orc_local_cues(const uint32_t* packed, const uint8_t* textons, int W, int H
   int lbg = orc_gaussian(0.1 * 25, kbg), lcg = orc_gaussian(0.05 * 25, kcg);
   static const int rbg[3] = {3, 5, 10}, rcg[3] = {5, 10, 20};
   for (int s = 0; s < 3; s++) {
-    orc_hist_gradient(q, Wm, Hm, B, rbg[s], kbg, lbg, cues + (size_t)(0 + s) * NORI * N);
-    orc_hist_gradient(q + Nm, Wm, Hm, B, rcg[s], kcg, lcg, cues + (size_t)(3 + s) * NORI * N);
-    orc_hist_gradient(q + 2 * Nm, Wm, Hm, B, rcg[s], kcg, lcg, cues + (size_t)(6 + s) * NORI * N);
-    orc_hist_gradient(tm, Wm, Hm, B, rcg[s], NULL, 0, cues + (size_t)(9 + s) * NORI * N);
+    hist_gradient_nb(q, Wm, Hm, B, rbg[s], kbg, lbg, nb4 ? nb4[0] : 0, cues + (size_t)(0 + s) * NORI * N);
+    hist_gradient_nb(q + Nm, Wm, Hm, B, rcg[s], kcg, lcg, nb4 ? nb4[1] : 0, cues + (size_t)(3 + s) * NORI * N);
+    hist_gradient_nb(q + 2 * Nm, Wm, Hm, B, rcg[s], kcg, lcg, nb4 ? nb4[2] : 0, cues + (size_t)(6 + s) * NORI * N);
+    hist_gradient_nb(tm, Wm, Hm, B, rcg[s], NULL, 0, nb4 ? nb4[3] : 0, cues + (size_t)(9 + s) * NORI * N);
   }
   if (mpb_all)
     for (int o = 0; o < NORI; o++)
@@ -997,4 +1018,46 @@ void orc_gpb_outputs(const double* gpb /* [8][N] */, const double* mpb_nmax, int
         value = (value - 0.0667) / 0.9333;
         gall[(size_t)(7 - o) * N + (size_t)x * H + y] = value;
       }
+}
+
+/* ---- texton histogram features of the superpixel classifier (test
+ * infrastructure): SPFeatures::genTextonFeatures (src/SPFeatures.cpp:259-272)
+ * and the texton part of genRecPatchFeatures (:478-545), on a given texton
+ * map.  descr[l][t] = count / size * 10 in that order of operations; a label
+ * with no pixels keeps zeros.  Returns 1 for a label outside [0, nlabels). */
+int orc_texton_features(const uint8_t* textons, const int32_t* sp, size_t N, int nlabels, double* descr) {
+  unsigned* cnt = (unsigned*)calloc((size_t)nlabels * 32, sizeof(unsigned));
+  for (size_t p = 0; p < N; p++) {
+    if (sp[p] < 0 || sp[p] >= nlabels) { free(cnt); return 1; }
+    cnt[(size_t)sp[p] * 32 + textons[p]]++;
+  }
+  for (int l = 0; l < nlabels; l++) {
+    unsigned size = 0;
+    for (int t = 0; t < 32; t++) size += cnt[(size_t)l * 32 + t];
+    for (int t = 0; t < 32; t++) {
+      double v = (double)cnt[(size_t)l * 32 + t];
+      if (size > 0) { v /= (double)size; v *= 10.0; }
+      descr[(size_t)l * 32 + t] = v;
+    }
+  }
+  free(cnt);
+  return 0;
+}
+
+void orc_texton_patch_features(const uint8_t* textons, int W, int H, const int32_t* centres, int n, double* descr) {
+  for (int k = 0; k < n; k++) {
+    int row = centres[2 * k], col = centres[2 * k + 1];
+    int north = row < 32 ? row : 32, south = (H - 1 - row) < 32 ? (H - 1 - row) : 32;
+    int west = col < 32 ? col : 32, east = (W - 1 - col) < 32 ? (W - 1 - col) : 32;
+    unsigned h[32] = {0};
+    for (int y = row - north; y <= row + south; y++)
+      for (int x = col - west; x <= col + east; x++) h[textons[(size_t)y * W + x]]++;
+    int area = (north + south + 1) * (west + east + 1);
+    for (int t = 0; t < 32; t++) {
+      double v = (double)h[t];
+      v /= (double)area;
+      v *= 10.0;
+      descr[(size_t)k * 32 + t] = v;
+    }
+  }
 }

@@ -47,6 +47,7 @@ _sig("orc_mirror_u8", None, [_u8p, C.c_int, C.c_int, C.c_int, _u8p])
 _sig("orc_gaussian", C.c_int, [C.c_double, _f64p])
 _sig("orc_hist_gradient", None, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, _f64p])
 _sig("orc_local_cues", None, [_u32p, _u8p, C.c_int, C.c_int, _f64p, C.c_void_p])
+_sig("orc_local_cues_nb", None, [_u32p, _u8p, C.c_int, C.c_int, C.c_void_p, _f64p, C.c_void_p])
 _sig("orc_nonmax", None, [_f64p, C.c_int, C.c_int, _f64p])
 
 
@@ -113,14 +114,20 @@ def hist_gradient(labels_mirrored: np.ndarray, border: int, r: int, kernel: np.n
     return out
 
 
-def local_cues(packed: np.ndarray, textons: np.ndarray):
-    """Returns cues[12, 8, H, W] and mpb_all[8, H, W] (FP64)."""
+def local_cues(packed: np.ndarray, textons: np.ndarray, nbins=None):
+    """Returns cues[12, 8, H, W] and mpb_all[8, H, W] (FP64).  nbins (optional,
+    4 ints): the histogram lengths of L, a, b and textons to use instead of
+    max(label) + 1 over this input (those of a whole image, for a crop)."""
     packed = np.ascontiguousarray(packed, np.uint32)
     H, W = packed.shape
     cues = np.empty((12, 8, H, W), np.float64)
     mpb = np.empty((8, H, W), np.float64)
-    lib.orc_local_cues(packed.reshape(-1), np.ascontiguousarray(textons, np.uint8).reshape(-1), W, H,
-                       cues.reshape(-1), mpb.ctypes.data)
+    t = np.ascontiguousarray(textons, np.uint8).reshape(-1)
+    if nbins is None:
+        lib.orc_local_cues(packed.reshape(-1), t, W, H, cues.reshape(-1), mpb.ctypes.data)
+    else:
+        nb = np.ascontiguousarray(nbins, np.int32)
+        lib.orc_local_cues_nb(packed.reshape(-1), t, W, H, nb.ctypes.data, cues.reshape(-1), mpb.ctypes.data)
     return cues, mpb
 
 
@@ -222,6 +229,30 @@ def gpb_outputs(gpb_maps: np.ndarray, mpb_nmax: np.ndarray):
     lib.orc_gpb_outputs(g.reshape(-1), np.ascontiguousarray(mpb_nmax, np.float64).reshape(-1), W, H,
                         gmax.reshape(-1), gall.reshape(-1))
     return gmax, gall
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def texton_features(textons: np.ndarray, superpixels: np.ndarray, nlabels: int) -> np.ndarray:
+    """Per-superpixel texton histograms (SPFeatures::genTextonFeatures) -> [nlabels, 32]."""
+    t = np.ascontiguousarray(textons, np.uint8)
+    sp = np.ascontiguousarray(superpixels, np.int32)
+    out = np.zeros((nlabels, 32), np.float64)
+    if lib.orc_texton_features(_ptr(t), _ptr(sp), C.c_size_t(t.size), C.c_int(nlabels), _ptr(out)) != 0:
+        raise ValueError("superpixel label outside [0, nlabels)")
+    return out
+
+
+def texton_patch_features(textons: np.ndarray, centres: np.ndarray) -> np.ndarray:
+    """Texton histograms of the clipped 65 x 65 windows around centres [n, 2] (row, col) -> [n, 32]."""
+    t = np.ascontiguousarray(textons, np.uint8)
+    H, W = t.shape
+    c = np.ascontiguousarray(centres, np.int32).reshape(-1, 2)
+    out = np.zeros((c.shape[0], 32), np.float64)
+    lib.orc_texton_patch_features(_ptr(t), C.c_int(W), C.c_int(H), _ptr(c), C.c_int(c.shape[0]), _ptr(out))
+    return out
 
 
 def pipeline(packed: np.ndarray, codebook: np.ndarray, K: int = 9, tol: float = 1e-10, m: int = 40):

@@ -121,6 +121,95 @@ __device__ __forceinline__ double spmv_tile(const float* __restrict__ wup_, cons
   return xy;
 }
 
+// One Lanczos step of the partially reorthogonalised solver in one pass over
+// a tile, the SpMV with the vector updates around it fused in:
+//   XMODE 1: x = fl32((w_in - a v_prev) * ib) formed while the tile is staged
+//            (the previous step's three-term update and scaling), written to
+//            the basis slot v_out for the tile's interior;
+//   XMODE 0: x = v_out as stored (a step resumed from the basis);
+//   w_out = fl32(L x - b v_prev) (this step's three-term update up to the
+//   alpha term, which the next step applies), and the thread's shares of
+//   x . w_out, w_out . w_out and x . x in acc[0..2].
+// a, ib: the previous step's alpha (0 after a whole-basis pass) and 1/beta;
+// b: that beta.  w_in and w_out are distinct buffers (neighbouring tiles read
+// w_in in their halos while this tile writes w_out).  Zero halos stay zero:
+// only interior pixels are written.  xs: LDS tile of the x values (float).
+template <int ROWS, typename AT, int NY, int XMODE>
+__device__ __forceinline__ void lanczos_step_tile(const float* __restrict__ wup_, const float* __restrict__ s_,
+                                                  const float* __restrict__ w_in_, const float* __restrict__ v_prev_,
+                                                  float* __restrict__ v_out_, float* __restrict__ w_out_, double a,
+                                                  double ib, double b, int W, int H, int Wp, int Hp, size_t Np,
+                                                  int x0, int y0, AT* __restrict__ u, float* __restrict__ xs,
+                                                  double acc[3]) {
+  const gptr<const float> s = as_global(s_);
+  const gptr<const float> w_in = as_global(w_in_);
+  const gptr<const float> v_prev = as_global(v_prev_);
+  const gptr<float> v_out = as_global_mut(v_out_);
+  const gptr<float> w_out = as_global_mut(w_out_);
+  constexpr int kSpmvTY = NY * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
+  for (int i = tid; i < kSpmvTileH * kSpmvTileW; i += kSpmvTX * NY) {
+    const int ty = i / kSpmvTileW, tx = i - ty * kSpmvTileW;
+    const int py = y0 + ty, px = x0 + tx;  // padded coordinates (tile starts at interior - 5)
+    float xv = 0.f;
+    AT uv = 0;
+    if (py < Hp && px < Wp) {
+      const size_t q = size_t(py) * Wp + px;
+      if (XMODE == 1) {
+        xv = float((double(w_in[q]) - a * double(v_prev[q])) * ib);
+        // the tile's own interior pixels go to the basis
+        if (ty >= kICRadius && ty < kICRadius + kSpmvTY && tx >= kICRadius && tx < kICRadius + kSpmvTX &&
+            py - kICRadius < H && px - kICRadius < W)
+          v_out[q] = xv;
+      } else {
+        xv = v_out[q];
+      }
+      uv = AT(s[q]) * AT(xv);
+    }
+    u[i] = uv;
+    xs[i] = xv;
+  }
+  __syncthreads();
+  const int xi = x0 + threadIdx.x;
+  acc[0] = acc[1] = acc[2] = 0.0;
+  if (xi < W) {
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int ty = threadIdx.y + NY * r;
+      const int yi = y0 + ty;
+      if (yi >= H) break;
+      const uint32_t p = uint32_t(yi + kICRadius) * uint32_t(Wp) + uint32_t(xi + kICRadius);
+      const int c = (ty + kICRadius) * kSpmvTileW + threadIdx.x + kICRadius;
+      const uint32_t p4 = p * 4u;
+      float wa[kStencilHalf], wb[kStencilHalf];
+#pragma unroll
+      for (int j = 0; j < kStencilHalf; j++) {
+        const gptr<const char> plane = as_global(reinterpret_cast<const char*>(wup_ + size_t(j) * Np));
+        const uint32_t o4 = uint32_t(kUpper.dy[j] * Wp + kUpper.dx[j]) * 4u;
+        wa[j] = *reinterpret_cast<gptr<const float>>(plane + p4);
+        wb[j] = *reinterpret_cast<gptr<const float>>(plane + (p4 - o4));
+      }
+      const float vp = v_prev[p];
+      const AT up = u[c];
+      AT sum = 0;
+#pragma unroll
+      for (int j = 0; j < kStencilHalf; j++) {
+        constexpr int kW = kSpmvTileW;
+        const int lo = kUpper.dy[j] * kW + kUpper.dx[j];
+        sum = fma_t(AT(wa[j]), up - u[c + lo], sum);
+        sum = fma_t(AT(wb[j]), up - u[c - lo], sum);
+      }
+      const float yp = float(AT(s[p]) * sum);
+      const float wv = float(double(yp) - b * double(vp));
+      w_out[p] = wv;
+      const double xd = double(xs[c]), wd = double(wv);
+      acc[0] += xd * wd;
+      acc[1] += wd * wd;
+      acc[2] += xd * xd;
+    }
+  }
+}
+
 // Block sum of one double per thread over NW wavefronts (blockDim.x * y =
 // 64 * NW); every thread gets the result.
 template <int NW>

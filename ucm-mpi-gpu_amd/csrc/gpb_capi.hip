@@ -126,6 +126,59 @@ int gpb_texton(gpb_ctx* ctx, int width, int height, const uint32_t* packed, cons
   });
 }
 
+int gpb_texton_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                        const int32_t* superpixels, int nlabels, double* descr) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!packed || !codebook || !superpixels || !descr || nlabels < 1) throw ArgError("null pointer or no labels");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    uint8_t* d_lab = c.ws<uint8_t>("io.labels", N);
+    int32_t* d_sp = c.ws<int32_t>("io.superpixels", N);
+    double* d_descr = c.ws<double>("io.descr", size_t(nlabels) * kTextonK);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_sp, superpixels, N * 4, hipMemcpyHostToDevice, c.stream));
+    launch_texton(c, d_rgb, width, height, d_cb, d_lab);
+    launch_texton_features(c, d_lab, width, height, d_sp, nlabels, d_descr);
+    GPB_HIP(hipMemcpyAsync(descr, d_descr, sizeof(double) * size_t(nlabels) * kTextonK, hipMemcpyDeviceToHost,
+                           c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gpb_texton_patch_features(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook,
+                              int ncentres, const int32_t* centres, double* descr) {
+  if (!ctx) return GPB_ERR_ARG;
+  return guarded(ctx, [&] {
+    check_image(width, height);
+    if (!packed || !codebook || ncentres < 0 || (ncentres > 0 && (!centres || !descr))) throw ArgError("null pointer");
+    for (int k = 0; k < ncentres; k++)
+      if (centres[2 * k] < 0 || centres[2 * k] >= height || centres[2 * k + 1] < 0 || centres[2 * k + 1] >= width)
+        throw ArgError("centre outside the image");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    uint32_t* d_rgb = c.ws<uint32_t>("io.rgb", N);
+    float* d_cb = c.ws<float>("io.codebook", kSchmidNF * kTextonK);
+    uint8_t* d_lab = c.ws<uint8_t>("io.labels", N);
+    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
+    launch_texton(c, d_rgb, width, height, d_cb, d_lab);
+    if (ncentres > 0) {
+      int32_t* d_ctr = c.ws<int32_t>("io.centres", size_t(2) * ncentres);
+      double* d_descr = c.ws<double>("io.descr", size_t(ncentres) * kTextonK);
+      GPB_HIP(hipMemcpyAsync(d_ctr, centres, sizeof(int32_t) * 2 * size_t(ncentres), hipMemcpyHostToDevice, c.stream));
+      launch_texton_patch_features(c, d_lab, width, height, d_ctr, ncentres, d_descr);
+      GPB_HIP(hipMemcpyAsync(descr, d_descr, sizeof(double) * size_t(ncentres) * kTextonK, hipMemcpyDeviceToHost,
+                             c.stream));
+    }
+    GPB_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 }  // extern "C"
 
 // ---- stage (b) ------------------------------------------------------------------

@@ -162,6 +162,7 @@ struct Context {
   int eig_m = 48;
   int eig_keep = 16;
   double eig_tol = 1e-3;
+  bool eig_rel = false;  // tol relative to the Ritz value (gpb_set_eig_convergence)
   int eig_max_matvecs = 20000;
   int eig_block = 1;  // 1: single-vector Lanczos; 4, 8, 12, 16: block Lanczos
   // single-vector Lanczos steps: 1 (default) partial reorthogonalisation for
@@ -170,6 +171,9 @@ struct Context {
   double pro_threshold = 0;    // omega trigger level, 0 = sqrt(eps_32)
   // single-vector Lanczos: replay restart cycles from captured hipGraphs
   bool lanczos_graph = true;
+  // single-vector Lanczos with partial reorthogonalisation: fused steps (one
+  // pass and one small kernel per step) or the separate kernels
+  bool lanczos_fused = true;
   int block_fusion = 2;  // block Lanczos: 0 separate passes, 1 fused, 2 fused + DGKS-gated second local pass
   // SpMV tile rows per thread: tuned per image shape (W << 32 | H), or forced
   std::map<uint64_t, int> spmv_rows;
@@ -180,7 +184,9 @@ struct Context {
   int verbose = 0;         // > 0: solvers report each restart on stderr
   // per-patch solver statistics and eigenvalues of the last patch call
   std::vector<LanczosStats> patch_stats;
-  std::vector<double> patch_evals;  // a hipGraph capture is open on the stream (no event timing)
+  std::vector<double> patch_evals;
+  std::vector<int> patch_dims;  // w, h of each patch of the last patch call (its "bp<p>." buffers)
+  int patch_K = 0;
   // learned warm start: network, optimiser and training set; the mode used
   // by the eigensolve (0 plain, 1 learned block, 2 block from a random start)
   std::shared_ptr<WarmState> warm;

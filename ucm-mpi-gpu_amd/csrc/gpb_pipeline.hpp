@@ -1,5 +1,6 @@
 // Internal launch interface between the C-ABI layer and the stage kernels.
 #pragma once
+#include <cmath>
 
 #include "gpb_device.hpp"
 
@@ -15,6 +16,12 @@ struct BlockBreakdown : std::runtime_error {
 // stage (a)
 void upload_schmid(Context& c);
 void launch_texton(Context& c, const uint32_t* d_rgb, int W, int H, const float* d_codebook, uint8_t* d_labels);
+
+// texton histogram features (superpixel classifier)
+void launch_texton_features(Context& c, const uint8_t* d_tex, int W, int H, const int32_t* d_sp, int nlabels,
+                            double* d_descr);
+void launch_texton_patch_features(Context& c, const uint8_t* d_tex, int W, int H, const int32_t* d_centres,
+                                  int ncentres, double* d_descr);
 
 // stage (b)
 void launch_lab_quantize(Context& c, const uint32_t* d_rgb, size_t N, uint8_t* d_q /* 3N */, int* d_nb /* 3 */);
@@ -89,7 +96,27 @@ struct EigOptions {
   int keep = 24;         // vectors kept at a restart
   double tol = 1e-3;     // relative generalised residual target
   int max_matvecs = 20000;
+  // false: pair i has converged when ||(D-W)v - theta D v|| / ||D v|| <= tol;
+  // true: when that residual is <= tol * max(|theta_i|, |theta_1|), the rule
+  // of Spectra's num_converged (Spectra/SymEigsSolver.h:325-334: residual
+  // below tol times the Ritz value), with the trivial pair held to the
+  // smallest nontrivial Ritz value instead of Spectra's eps^(2/3) floor.
+  bool rel = false;
 };
+// Residual target of pair i given the current Ritz values theta (ascending,
+// K of them).
+inline double conv_target(const EigOptions& o, const double* theta, int i) {
+  if (!o.rel) return o.tol;
+  const double t1 = std::fabs(theta[o.K > 1 ? 1 : 0]);
+  const double ti = std::fabs(theta[i]);
+  return o.tol * (ti > t1 ? ti : t1);
+}
+// Number of the K pairs whose residual meets its target.
+inline int conv_count(const EigOptions& o, const double* theta, const double* resid) {
+  int n = 0;
+  for (int i = 0; i < o.K; i++) n += resid[i] <= conv_target(o, theta, i);
+  return n;
+}
 // Generalised eigenpairs of (D - W, D): evals ascending (host), evecs [K][Np]
 // padded layout, generalised normalisation v^T D v = 1.  start_evecs
 // (optional, [start_count][Np] generalised vectors, padded, may alias d_evecs)
@@ -114,11 +141,22 @@ void refine_pairs(Context& c, const Stencil& st, double* evals, float* d_evecs, 
 // Residual targets below this get the FP64 refinement: there the FP32
 // storage of the Krylov vectors, not the target, limits the eigenvalues.
 constexpr double kRefineBelowTol = 1e-4;
+// Targets relative to the Ritz value ask for residuals of a small multiple of
+// the smallest eigenvalues, always in the range where the FP32 storage limits
+// the eigenvalues: they always get the refinement.  The single-vector,
+// block and batched solvers keep their partial reorthogonalisation there
+// (semi-orthogonality bounds the error of the projected matrix, and the
+// refinement recomputes the eigenvalues from the vectors in FP64).
+inline bool eig_refine(const EigOptions& o) { return o.rel || o.tol < kRefineBelowTol; }
+inline bool eig_partial_reorth(const EigOptions& o) { return o.rel || o.tol >= kRefineBelowTol; }
 // Warm-started solves with a target below this run the block phase to this
 // residual and finish with the single-vector solver started from the block's
 // Ritz vectors: block Lanczos with a few block steps per restart converges
 // slowly once the wanted pairs are resolved, single-vector Lanczos does not.
 constexpr double kWarmHandoverTol = 1e-3;
+// Measured-residual judge of a warm-started solve whose target is relative to
+// the Ritz value: plain Lanczos takes over above this.
+constexpr double kWarmJudgeRel = 1e-3;
 // Measured relative generalised residuals of K <= 9 returned pairs; returns
 // the largest (resid, optional, receives all K).
 double measured_max_residual(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K,

@@ -787,7 +787,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
   double* nsecond = c.ws<double>("bat.nsecond", 1);
   double* Ydev = c.ws<double>("bat.Y", size_t(np) * m * keep);
   // partial reorthogonalisation for targets the FP64 refinement does not follow
-  const bool pro = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  const bool pro = c.reorth_mode >= 1 && eig_partial_reorth(opt) && m <= kProjectMaxN;
   double* om = c.ws<double>("bat.omega", size_t(np) * 2 * kOmS);
   int* pend = c.ws<int>("bat.pending", size_t(np));
   int* rows_dev = c.ws<int>("bat.rows", size_t(K + 1));
@@ -955,23 +955,20 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       if (!act[size_t(p)]) continue;
       const double bm = betah[size_t(p) * 2 * kMaxBasis + m - 1];
       double worst = 0;
+      std::vector<double> rp(static_cast<size_t>(K));
       for (int i = 0; i < K; i++) {
-        const double r = std::fabs(bm * Y[size_t(p)][size_t(m - 1) * m + i]) * std::sqrt(nrmh[size_t(p) * kPS + K]) /
-                         std::sqrt(std::max(nrmh[size_t(p) * kPS + i], 1e-300));
-        worst = std::max(worst, r);
+        rp[size_t(i)] = std::fabs(bm * Y[size_t(p)][size_t(m - 1) * m + i]) * std::sqrt(nrmh[size_t(p) * kPS + K]) /
+                        std::sqrt(std::max(nrmh[size_t(p) * kPS + i], 1e-300));
+        worst = std::max(worst, rp[size_t(i)]);
       }
       stats[size_t(p)].max_residual = worst;
       stats[size_t(p)].restarts = cycles - 1;
       stats[size_t(p)].iterations = mv[size_t(p)];
-      if (worst <= opt.tol || mv[size_t(p)] >= opt.max_matvecs) {
+      const int nconv = conv_count(opt, th[size_t(p)].data(), rp.data());
+      if (nconv == K || mv[size_t(p)] >= opt.max_matvecs) {
         // done: eigenpairs out, patch leaves the batch
         for (int i = 0; i < K; i++) evals[size_t(p) * K + i] = th[size_t(p)][size_t(i)];
-        stats[size_t(p)].converged = 0;
-        for (int i = 0; i < K; i++) {
-          const double r = std::fabs(bm * Y[size_t(p)][size_t(m - 1) * m + i]) * std::sqrt(nrmh[size_t(p) * kPS + K]) /
-                           std::sqrt(std::max(nrmh[size_t(p) * kPS + i], 1e-300));
-          stats[size_t(p)].converged += r <= opt.tol;
-        }
+        stats[size_t(p)].converged = nconv;
         const Stencil& st = sts[size_t(p)];
         hipLaunchKernelGGL(bto_general_kernel, dim3(std::min<unsigned>(2048, div_up(st.Np, 256))), dim3(256), 0,
                            c.stream, V, Nt, K, st.s, ph[size_t(p)].voff, st.Np, d_evecs[size_t(p)]);
@@ -1009,7 +1006,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
   GPB_HIP(hipStreamSynchronize(c.stream));
   for (auto& s : stats) s.second_passes = int(ns);
   for (int p = 0; p < np; p++)
-    if (stats[size_t(p)].max_residual > opt.tol)
+    if (stats[size_t(p)].converged < K)
       throw NoConvergence("batched Lanczos: patch " + std::to_string(p) + " stopped after " +
                           std::to_string(stats[size_t(p)].iterations) + " operator applications with residual " +
                           std::to_string(stats[size_t(p)].max_residual));

@@ -873,7 +873,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   bool converged = false;
   int kcur = 0;
   // semi-orthogonal block steps for targets the FP64 refinement does not follow
-  const bool semi = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol;
+  const bool semi = c.reorth_mode >= 1 && eig_partial_reorth(opt);
   int* gflag = c.ws<int>("beig.gflag", 1);
   double* gacc = c.ws<double>("beig.gacc", 1);
   int* dflag = c.ws<int>("beig.dflag", 1);
@@ -1001,7 +1001,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     if (c.verbose)
       std::fprintf(stderr, "gpb: block Lanczos %dx%d (block %d): %d operator applications, residual %.3e\n", st.W,
                    st.H, B, matvecs, worst);
-    converged = worst <= opt.tol;
+    converged = conv_count(opt, th.data(), resid.data()) == K;
     if (converged || matvecs >= opt.max_matvecs) break;
     std::vector<double> T0(size_t(M) * M, 0.0);
     for (int i = 0; i < keep; i++) T0[size_t(i) * M + i] = th[size_t(i)];
@@ -1031,7 +1031,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   }
   stats.iterations = matvecs;
   stats.converged = 0;
-  for (int i = 0; i < K; i++) stats.converged += resid[size_t(i)] <= opt.tol;
+  stats.converged = th.empty() ? 0 : conv_count(opt, th.data(), resid.data());
   for (int i = 0; i < K; i++) {
     evals[i] = th[size_t(i)];
     launch_to_general(c, V + size_t(i) * Np, st.s, d_evecs + size_t(i) * Np, Np);

@@ -64,6 +64,45 @@ __global__ __launch_bounds__(kSpmvTX * NY) void spmv_kernel(const float* __restr
   if (threadIdx.x == 0 && threadIdx.y == 0) xy_partial[tile] = t;
 }
 
+// One fused Lanczos step over the whole image (lanczos_step_tile per tile, the
+// SpMV's XCD-aware tile order): partial[k * ntiles + tile] for k = x . w,
+// w . w, x . x.  a = aeff[j - 1], 1/beta and beta from beta[] at j - 1.
+template <int ROWS, typename AT, int NY, int XMODE>
+__global__ __launch_bounds__(kSpmvTX * NY) void lanczos_step_kernel(
+    const float* __restrict__ wup, const float* __restrict__ s, const float* __restrict__ w_in,
+    const float* __restrict__ v_prev, float* __restrict__ v_out, float* __restrict__ w_out,
+    const double* __restrict__ aeff, const double* __restrict__ beta, int j, int W, int H, int Wp, int Hp, size_t Np,
+    double* __restrict__ partial) {
+  constexpr int kSpmvTY = NY * ROWS, kSpmvTileH = kSpmvTY + 2 * kICRadius;
+  __shared__ AT u[kSpmvTileH * kSpmvTileW];
+  __shared__ float xs[kSpmvTileH * kSpmvTileW];
+  __shared__ double red[3][NY];
+  const int tiles_x = (W + kSpmvTX - 1) / kSpmvTX, ntiles = tiles_x * ((H + kSpmvTY - 1) / kSpmvTY);
+  const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
+  const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
+  if (tile >= ntiles) return;  // whole workgroup
+  const int x0 = (tile % tiles_x) * kSpmvTX, y0 = (tile / tiles_x) * kSpmvTY;
+  const double a = XMODE == 1 ? aeff[j - 1] : 0.0, ib = XMODE == 1 ? beta[kMaxBasis + j - 1] : 0.0;
+  const double b = beta[j - 1];
+  double acc[3];
+  lanczos_step_tile<ROWS, AT, NY, XMODE>(wup, s, w_in, v_prev, v_out, w_out, a, ib, b, W, H, Wp, Hp, Np, x0, y0, u,
+                                         xs, acc);
+  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double v = acc[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) red[k][tid >> 6] = v;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double t = 0;
+#pragma unroll
+    for (int w = 0; w < NY; w++) t += red[tid][w];
+    partial[size_t(tid) * ntiles + tile] = t;
+  }
+}
+
 // SpMV tile codes: 10 * NY + ROWS (NY wavefronts of 64 threads, ROWS rows per
 // thread; tile 64 x NY*ROWS pixels).  Every tile is at least 8 rows tall.
 constexpr int kSpmvTiles[] = {42, 44, 48, 82, 84, 161, 162};
@@ -169,6 +208,47 @@ void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double
 }
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) { launch_spmv(c, st, x, y, nullptr); }
+
+// One fused step (lanczos_step_kernel) with the tile the shape was tuned to;
+// xmode 1 forms v_j from w_in and v_{j-1}, 0 reads it from v_out.
+static void launch_lanczos_step(Context& c, const Stencil& st, int xmode, const float* w_in, const float* v_prev,
+                                float* v_out, float* w_out, const double* aeff, const double* beta, int j,
+                                double* partial) {
+  const int code = tune_spmv(c, st);
+  // W planes, w_in, v_{j-1}, s read; v_j (xmode 1) and w written
+  KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + (xmode ? 20 : 16)));
+  const int ntiles = spmv_blocks(st, code);
+  const dim3 grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
+#define GPB_STEP(NY, R, XM)                                                                                          \
+  if (c.spmv_fp64)                                                                                                 \
+    hipLaunchKernelGGL((lanczos_step_kernel<R, double, NY, XM>), grid, dim3(kSpmvTX, NY), 0, c.stream, st.wup, st.s, \
+                       w_in, v_prev, v_out, w_out, aeff, beta, j, st.W, st.H, st.Wp, st.Hp, st.Np, partial);       \
+  else                                                                                                             \
+    hipLaunchKernelGGL((lanczos_step_kernel<R, float, NY, XM>), grid, dim3(kSpmvTX, NY), 0, c.stream, st.wup, st.s,  \
+                       w_in, v_prev, v_out, w_out, aeff, beta, j, st.W, st.H, st.Wp, st.Hp, st.Np, partial);
+#define GPB_STEP_CODE(NY, R)  \
+  case 10 * NY + R:           \
+    if (xmode) {              \
+      GPB_STEP(NY, R, 1)      \
+    } else {                  \
+      GPB_STEP(NY, R, 0)      \
+    }                         \
+    break;
+  switch (code) {
+    GPB_STEP_CODE(4, 2)
+    GPB_STEP_CODE(4, 4)
+    GPB_STEP_CODE(4, 8)
+    GPB_STEP_CODE(8, 2)
+    GPB_STEP_CODE(8, 4)
+    GPB_STEP_CODE(16, 1)
+    GPB_STEP_CODE(16, 2)
+    default:
+      throw ArgError("unsupported SpMV tile");
+  }
+#undef GPB_STEP_CODE
+#undef GPB_STEP
+  GPB_LAUNCH_CHECK();
+}
 
 // Sum of 4 values over the 256-thread block; result valid in thread 0.
 __device__ __forceinline__ void block_sum4(double v[4], double* sh /* [4][4] */) {
@@ -479,7 +559,7 @@ __global__ __launch_bounds__(256) void three_term_kernel(const float* __restrict
   if (threadIdx.x == 0) nrm_partial[blockIdx.x] = r;
 }
 
-// Step j (one block of 256 threads, j < m <= kProjectMaxN): beta_j from the
+// Step j (one block of 256 threads, j < m < kMaxBasis): beta_j from the
 // partial norms, the omega recurrence for k < j,
 //   b_new om1[k] = (T om0)_k - alpha_j om0[k] - b_cur om_1[k] +- eps (b_new + |T[k][k+1]|),
 // with om0 = omega_j (om0[j] = 1), om_1 = omega_{j-1} (om_1[j-1] = 1); the
@@ -492,20 +572,16 @@ __global__ __launch_bounds__(256) void three_term_kernel(const float* __restrict
 // trigger level (sqrt(eps) by default).  With first_viol (the optimistic
 // cycle, no gated launches), the first step whose omega exceeds thr is
 // recorded there and the pass counters are left alone.
-__global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int m, int j,
-                                                    const double* __restrict__ alpha, double* __restrict__ beta,
-                                                    const double* __restrict__ nrm_partial, int G,
-                                                    const double* __restrict__ om0, const double* omp, double* om_1,
-                                                    int* __restrict__ flag, int* __restrict__ pending, double np,
-                                                    double* __restrict__ acc, double thr, int* __restrict__ first_viol) {
-  __shared__ double red4[4];
-  __shared__ double mx[4];
+// The omega recurrence and the gate for step j, given alpha_j (al) and
+// beta_j (bn); one block of 256 threads.  aeff (optional): receives alpha_j
+// when no whole-basis pass follows (the fused steps apply it in the next
+// step's staging).
+__device__ void omega_body(double* __restrict__ T, int m, int j, double al, double bn, double* __restrict__ beta,
+                           const double* __restrict__ om0, const double* omp, double* om_1, int* __restrict__ flag,
+                           int* __restrict__ pending, double np, double* __restrict__ acc, double thr,
+                           int* __restrict__ first_viol, double* __restrict__ aeff, double* mx) {
   const int t = threadIdx.x;
-  double a = 0;
-  for (int b = t; b < G; b += 256) a += nrm_partial[b];
-  const double nrm = block_sum_waves<4>(a, red4);
-  const double bn = sqrt(nrm);
-  const double al = *alpha, bc = beta[j - 1];
+  const double bc = beta[j - 1];
   double om = 0;
   if (t < j) {
     double tw = T[size_t(t) * m + j];  // om0[j] = 1
@@ -538,6 +614,7 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
     } else if (!f) {
       beta[j] = bn;
       beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+      if (aeff) aeff[j] = al;
       if (j + 1 < m) {
         T[size_t(j) * m + j + 1] = bn;
         T[size_t(j + 1) * m + j] = bn;
@@ -546,10 +623,96 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
   }
 }
 
+__global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int m, int j,
+                                                    const double* __restrict__ alpha, double* __restrict__ beta,
+                                                    const double* __restrict__ nrm_partial, int G,
+                                                    const double* __restrict__ om0, const double* omp, double* om_1,
+                                                    int* __restrict__ flag, int* __restrict__ pending, double np,
+                                                    double* __restrict__ acc, double thr, int* __restrict__ first_viol) {
+  __shared__ double red4[4];
+  __shared__ double mx[4];
+  double a = 0;
+  for (int b = threadIdx.x; b < G; b += 256) a += nrm_partial[b];
+  const double nrm = block_sum_waves<4>(a, red4);
+  omega_body(T, m, j, *alpha, sqrt(nrm), beta, om0, omp, om_1, flag, pending, np, acc, thr, first_viol, nullptr, mx);
+}
+
+// Step j of the fused form: alpha_j = x . w and beta_j^2 = ||w - alpha x||^2 =
+// w . w - 2 alpha (x . w) + alpha^2 (x . x) from the tile partials of the
+// fused kernel (partial[3][G], summed in a fixed order), then the omega
+// recurrence and the gate.  alpha_j also goes to alpha_out for a gated pass.
+__global__ __launch_bounds__(256) void omega_fused_kernel(double* __restrict__ T, int m, int j,
+                                                          const double* __restrict__ partial, int G,
+                                                          double* __restrict__ beta, double* __restrict__ aeff,
+                                                          double* __restrict__ alpha_out,
+                                                          const double* __restrict__ om0, const double* omp,
+                                                          double* om_1, int* __restrict__ flag,
+                                                          int* __restrict__ pending, double np,
+                                                          double* __restrict__ acc, double thr,
+                                                          int* __restrict__ first_viol) {
+  __shared__ double red4[4];
+  __shared__ double mx[4];
+  double s3[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < G; b += 256) {
+    s3[0] += partial[b];
+    s3[1] += partial[size_t(G) + b];
+    s3[2] += partial[size_t(2) * G + b];
+  }
+  const double xw = block_sum_waves<4>(s3[0], red4);
+  __syncthreads();
+  const double ww = block_sum_waves<4>(s3[1], red4);
+  __syncthreads();
+  const double xx = block_sum_waves<4>(s3[2], red4);
+  const double al = xw;
+  const double b2 = ww - 2.0 * al * xw + al * al * xx;
+  const double bn = sqrt(b2 > 0 ? b2 : 0.0);
+  if (threadIdx.x == 0) *alpha_out = al;
+  omega_body(T, m, j, al, bn, beta, om0, omp, om_1, flag, pending, np, acc, thr, first_viol, aeff, mx);
+}
+
+// Gated whole-basis pass of a fused step, first and last parts: w -= alpha v_j
+// (the update the fused form leaves to the next step), and after the
+// projection beta_j from the norm that pass left, with alpha_eff = 0 (the next
+// step's staging then only scales).  Skipped when *flag == 0.
+__global__ __launch_bounds__(256) void materialise_kernel(float* __restrict__ w, const float* __restrict__ vj,
+                                                          const double* __restrict__ alpha, size_t Np,
+                                                          const int* __restrict__ flag) {
+  if (*flag == 0) return;
+  const double a = *alpha;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
+    w[p] = float(double(w[p]) - a * double(vj[p]));
+}
+
+__global__ void gated_beta_kernel(double* __restrict__ T, int m, int j, double* __restrict__ beta,
+                                  double* __restrict__ aeff, const double* __restrict__ nrm2,
+                                  const int* __restrict__ flag) {
+  if (*flag == 0) return;
+  const double bn = sqrt(*nrm2);
+  beta[j] = bn;
+  beta[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+  aeff[j] = 0.0;
+  if (j + 1 < m) {
+    T[size_t(j) * m + j + 1] = bn;
+    T[size_t(j + 1) * m + j] = bn;
+  }
+}
+
+// The residual direction at the end of a fused cycle: v_m = (w - alpha_eff
+// v_{m-1}) / beta_{m-1}, the scaling the next step would have done.
+__global__ __launch_bounds__(256) void fused_last_kernel(const float* __restrict__ w, const float* __restrict__ vprev,
+                                                         const double* __restrict__ aeff,
+                                                         const double* __restrict__ beta, int j,
+                                                         float* __restrict__ v, size_t Np) {
+  const double a = aeff[j], ib = beta[kMaxBasis + j];
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x)
+    v[p] = float((double(w[p]) - a * double(vprev[p])) * ib);
+}
+
 // omega_{j+1}[k] = omega_j[k] = eps for k <= j after a full
 // Gram-Schmidt step j (the first of a cycle); pending cleared.
 __global__ void omega_reset_kernel(double* __restrict__ om_a, double* __restrict__ om_b, int j,
-                                   int* __restrict__ pending) {
+                                   int* __restrict__ pending, double* __restrict__ aeff) {
+  if (threadIdx.x == 0) aeff[j] = 0.0;  // this step's w is fully orthogonalised: the next staging only scales
   for (int k = threadIdx.x; k <= j; k += blockDim.x) {
     om_a[k] = kProEps;
     om_b[k] = kProEps;
@@ -854,6 +1017,13 @@ struct Reducer {
     hipLaunchKernelGGL(reduce_kernel, dim3(n + 1), dim3(kRedThreads), 0, c.stream, partial, GP, dst, gate);
     GPB_LAUNCH_CHECK();
   }
+  // dst[0..n) = V_i . w, dst[n] = ||w||^2 for any n < kMaxBasis, in chunks of
+  // kProjectMaxN vectors (each chunk's norm slot is overwritten by the next
+  // chunk's first dot; the last chunk's stays); gated when gate is non-null.
+  void project_any(const float* V, size_t ld, int n, float* w, const int* gate, double* dst) {
+    for (int i0 = 0; i0 < n; i0 += kProjectMaxN)
+      project(V + size_t(i0) * ld, ld, std::min(kProjectMaxN, n - i0), w, nullptr, nullptr, 0, gate, dst + i0);
+  }
   // w -= V h; nrm_dst = ||w||^2 afterwards (if non-null); gated when gate is non-null
   void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst,
               const int* gate = nullptr) {
@@ -911,11 +1081,28 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   // partial reorthogonalisation for targets the FP64 refinement does not
   // follow (see kRefineBelowTol); whole-basis Gram-Schmidt every step otherwise
-  const bool pro = c.reorth_mode >= 1 && opt.tol >= kRefineBelowTol && m <= kProjectMaxN;
+  const bool pro = c.reorth_mode >= 1 && eig_partial_reorth(opt);
   // optimistic cycles (mode 2): no gated launches; a cycle whose omega
   // recurrence triggered is redone from that step with them (below)
   const bool optimistic = pro && c.reorth_mode == 2;
   const double pro_thr = c.pro_threshold > 0 ? c.pro_threshold : kProSqrtEps;
+  // fused steps (gpb_set_lanczos_fusion): one pass per three-term step (the
+  // SpMV with the previous step's scaling and this step's update around it,
+  // lanczos_step_tile) and one small kernel for alpha, beta and omega.  Their
+  // w alternates between two buffers (neighbouring tiles read the previous w
+  // in their halos while a tile writes the new one), and they write only
+  // interior pixels, so every basis halo starts at zero.
+  const bool fused = pro && c.lanczos_fused;
+  float* wbuf[2] = {w, fused ? c.ws<float>("eig.w2", Np) : w};
+  auto wj = [&](int j) { return wbuf[j & 1]; };
+  double* aeff = c.ws<double>("eig.aeff", kMaxBasis);
+  double* alpha_f = c.ws<double>("eig.alpha_f", 1);
+  double* part3 = c.ws<double>("eig.part3", size_t(3) * spmv_g);
+  if (fused) {
+    launch_fill(c, wbuf[1], Np, 0.f);
+    launch_fill(c, V, size_t(m + 1) * Np, 0.f);
+    launch_fill(c, V2, size_t(m + 1) * Np, 0.f);
+  }
   // one omega vector per step, so that a cycle can be resumed at any step
   double* om = c.ws<double>("eig.omega", size_t(kMaxBasis + 2) * (kMaxBasis + 1));
   int* first_viol = c.ws<int>("eig.first_viol", 1);
@@ -981,8 +1168,9 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     }
     for (int j = j0; j < m; j++) {
       float* vj = V + size_t(j) * Np;
+      float* const w = wj(j);
       const int n = j + 1;
-      if (j == k || m > kProjectMaxN) {
+      if (j == k || (!pro && m > kProjectMaxN)) {
         // first step of a cycle (its column couples to every kept Ritz
         // vector): classical Gram-Schmidt twice, FP64 accumulation
         launch_spmv(c, st, vj, w);
@@ -998,14 +1186,36 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
           red.dots(V, Np, n, w, nullptr, false, h2);
         }
         red.update(V, Np, n, h2, w, nrm + 1);
-        hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, int(m <= kProjectMaxN), nullptr,
+        hipLaunchKernelGGL(tcol_kernel, dim3(1), dim3(kMaxBasis), 0, c.stream, T, m, j, int(pro || m <= kProjectMaxN),
+                           nullptr,
                            h1, h2, one_flag,
                            nrm + 1, nrm + 1, beta);
         GPB_LAUNCH_CHECK();
         if (pro) {
-          hipLaunchKernelGGL(omega_reset_kernel, dim3(1), dim3(256), 0, c.stream, om_slot(j), om_slot(j + 1), j, pend);
+          hipLaunchKernelGGL(omega_reset_kernel, dim3(1), dim3(256), 0, c.stream, om_slot(j), om_slot(j + 1), j, pend,
+                             aeff);
           GPB_LAUNCH_CHECK();
         }
+      } else if (fused) {
+        // v_j formed from the previous step's w while the tile is staged --
+        // or, for the first step of a resumed cycle, read from the basis,
+        // which holds it -- then w = L v_j - beta_{j-1} v_{j-1} and alpha_j,
+        // beta_j and omega from its partial sums
+        const bool resumed = j == j0 && j0 > k;
+        launch_lanczos_step(c, st, resumed ? 0 : 1, wj(j - 1), V + size_t(j - 1) * Np, vj, w, aeff, beta, j, part3);
+        hipLaunchKernelGGL(omega_fused_kernel, dim3(1), dim3(256), 0, c.stream, T, m, j, part3, spmv_g, beta, aeff,
+                           alpha_f, om_slot(j), om_slot(j - 1), om_slot(j + 1), flag, pend, double(Np), gacc,
+                           pro_thr, opt_cycle ? first_viol : nullptr);
+        GPB_LAUNCH_CHECK();
+        if (!opt_cycle) {
+          hipLaunchKernelGGL(materialise_kernel, dim3(vg), dim3(256), 0, c.stream, w, vj, alpha_f, Np, flag);
+          GPB_LAUNCH_CHECK();
+          red.project_any(V, Np, n, w, flag, h2);
+          red.update(V, Np, n, h2, w, nrm + 1, flag);
+          hipLaunchKernelGGL(gated_beta_kernel, dim3(1), dim3(1), 0, c.stream, T, m, j, beta, aeff, nrm + 1, flag);
+          GPB_LAUNCH_CHECK();
+        }
+        continue;
       } else if (pro) {
         // three-term step; the whole-basis pass only when the omega
         // recurrence says semi-orthogonality is about to be lost
@@ -1023,7 +1233,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
                            opt_cycle ? first_viol : nullptr);
         GPB_LAUNCH_CHECK();
         if (!opt_cycle) {
-          red.project(V, Np, n, w, nullptr, nullptr, j, flag, h2);
+          red.project_any(V, Np, n, w, flag, h2);
           red.update(V, Np, n, h2, w, nrm + 1, flag);
         }
         hipLaunchKernelGGL(pro_scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np, T,
@@ -1047,6 +1257,12 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         GPB_LAUNCH_CHECK();
       }
       hipLaunchKernelGGL(scale_kernel, dim3(vg), dim3(256), 0, c.stream, w, beta, j, V + size_t(j + 1) * Np, Np);
+      GPB_LAUNCH_CHECK();
+    }
+    if (fused) {
+      // the residual direction v_m: the scaling a next step would have done
+      hipLaunchKernelGGL(fused_last_kernel, dim3(vg), dim3(256), 0, c.stream, wj(m - 1), V + size_t(m - 1) * Np, aeff,
+                         beta, m - 1, V + size_t(m) * Np, Np);
       GPB_LAUNCH_CHECK();
     }
   };
@@ -1154,7 +1370,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     stats.max_residual = worst;
     if (c.verbose)
       std::fprintf(stderr, "gpb: Lanczos %dx%d: %d operator applications, residual %.3e\n", st.W, st.H, matvecs, worst);
-    converged = worst <= opt.tol;
+    converged = conv_count(opt, th.data(), resid.data()) == K;
     if (converged || matvecs >= opt.max_matvecs) break;
     // thick restart: basis = Ritz vectors [0, keep) and v_m; T = diag(theta)
     std::vector<double> T0(size_t(m) * m, 0.0);
@@ -1169,7 +1385,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       // rounding level): Gram-Schmidt it against them twice, then normalise.
       float* vk = V + size_t(keep) * Np;
       for (int pass = 0; pass < 2; pass++) {
-        red.project(V, Np, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+        red.project_any(V, Np, keep, vk, nullptr, h1);
         red.update(V, Np, keep, h1, vk, rnrm);
       }
       launch_scale_by(c, vk, rnrm, Np);
@@ -1177,7 +1393,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   }
   stats.iterations = matvecs;
   stats.converged = 0;
-  for (int i = 0; i < K; i++) stats.converged += resid[i] <= opt.tol;
+  stats.converged = th.empty() ? 0 : conv_count(opt, th.data(), resid.data());
   for (int i = 0; i < K; i++) {
     evals[i] = th[size_t(i)];
     hipLaunchKernelGGL(to_general_kernel, dim3(vg), dim3(256), 0, c.stream, V + size_t(i) * Np, st.s,

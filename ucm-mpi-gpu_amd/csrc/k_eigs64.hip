@@ -341,20 +341,18 @@ void lanczos_solve_f64(Context& c, const Stencil64& st, const EigOptions& opt_in
     if (c.verbose)
       std::fprintf(stderr, "gpb: FP64 Lanczos %dx%d: %d operator applications, residual %.3e\n", st.W, st.H, matvecs,
                    worst);
-    if (worst <= opt.tol || matvecs >= opt.max_matvecs) break;
+    if (conv_count(opt, th.data(), resid.data()) == K || matvecs >= opt.max_matvecs) break;
     std::fill(T.begin(), T.end(), 0.0);
     for (int i = 0; i < keep; i++) T[size_t(i) * m + i] = th[size_t(i)];
     k = keep;
     stats.restarts++;
   }
   stats.iterations = matvecs;
-  for (int i = 0; i < K; i++) {
-    stats.converged += resid[size_t(i)] <= opt.tol;
-    evals[i] = th[size_t(i)];
-  }
+  stats.converged = conv_count(opt, th.data(), resid.data());
+  for (int i = 0; i < K; i++) evals[i] = th[size_t(i)];
   hipLaunchKernelGGL(general64_kernel, dim3(2048), dim3(256), 0, c.stream, V, st.s, K, Np, d_evecs);
   GPB_LAUNCH_CHECK();
-  if (stats.max_residual > opt.tol)
+  if (stats.converged < K)
     throw NoConvergence("FP64 Lanczos stopped after " + std::to_string(matvecs) +
                         " operator applications with residual " + std::to_string(stats.max_residual));
 }

@@ -286,6 +286,47 @@ void buildW(std::vector<double>& valD, std::vector<double>& val, std::vector<int
     }
 }
 
+std::vector<uint8_t> frame_mat_vec(const std::vector<Mat>& mats) {
+  const int n = int(mats.size());
+  std::vector<int> dims(size_t(4) * n);
+  std::vector<const void*> data(static_cast<size_t>(n));
+  for (int i = 0; i < n; i++) {
+    const Mat& m = mats[size_t(i)];
+    dims[size_t(4) * i] = m.rows;
+    dims[size_t(4) * i + 1] = m.cols;
+    dims[size_t(4) * i + 2] = m.channels;
+    dims[size_t(4) * i + 3] = 6 + 8 * (m.channels - 1);  // CV_64FC(channels)
+    data[size_t(i)] = m.data.data();
+  }
+  const size_t bytes = gpb_frame_size(n, dims.data());
+  if (bytes == 0) throw std::invalid_argument("frame_mat_vec: bad matrix dimensions");
+  std::vector<uint8_t> out(bytes);
+  if (gpb_frame_mats(n, dims.data(), data.data(), out.data(), out.size()) != GPB_OK)
+    throw std::invalid_argument("frame_mat_vec: framing failed");
+  return out;
+}
+
+std::vector<Mat> unframe_mat_vec(const std::vector<uint8_t>& frame) {
+  if (frame.size() < sizeof(int32_t)) throw std::invalid_argument("unframe_mat_vec: truncated frame");
+  int32_t n = 0;
+  std::memcpy(&n, frame.data(), sizeof n);
+  if (n < 0 || size_t(n) > frame.size() / 16) throw std::invalid_argument("unframe_mat_vec: bad count");
+  std::vector<int> dims(size_t(4) * size_t(n) + 4);
+  std::vector<size_t> offs(size_t(n) + 1);
+  int count = 0;
+  if (gpb_unframe_mats(frame.data(), frame.size(), n, &count, dims.data(), offs.data()) != GPB_OK)
+    throw std::invalid_argument("unframe_mat_vec: malformed frame");
+  std::vector<Mat> out;
+  for (int i = 0; i < count; i++) {
+    const int* d = dims.data() + size_t(4) * i;
+    if ((d[3] & 7) != 6) throw std::invalid_argument("unframe_mat_vec: not a CV_64F matrix");
+    Mat m(d[0], d[1], d[2]);
+    std::memcpy(m.data.data(), frame.data() + offs[size_t(i)], m.data.size() * sizeof(double));
+    out.push_back(std::move(m));
+  }
+  return out;
+}
+
 bool read_ppm(const std::string& path, int& width, int& height, std::vector<uint8_t>& rgb) {
   std::ifstream f(path, std::ios::binary);
   if (!f) return false;

@@ -43,10 +43,18 @@ _sig("gpb_pack_rgb", None, [_u8p, C.c_size_t, _u32p])
 _sig("gpb_synth_voronoi", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, _u8p, C.c_void_p])
 _sig("gpb_make_codebook", C.c_int, [_u32p, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, _f32p])
 _sig("gpb_texton", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _f32p, _u8p])
+_sig("gpb_texton_features", C.c_int,
+     [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p])
+_sig("gpb_texton_patch_features", C.c_int,
+     [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p])
 _sig("gpb_lab_quantize", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, _i32p])
 _sig("gpb_local_cues", C.c_int, [_ctxp, C.c_int, C.c_int, _u32p, _u8p, C.c_void_p, _f32p])
 _sig("gpb_nonmax", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p])
 _sig("gpb_set_eig_options", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int])
+_sig("gpb_set_eig_convergence", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_set_lanczos_fusion", C.c_int, [_ctxp, C.c_int])
+_sig("gpb_patch_eigenvectors", C.c_int,
+     [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
 _sig("gpb_build_w", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, _f32p])
 _sig("gpb_eigs", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f64p, C.c_void_p])
 _sig("gpb_spectral_pb", C.c_int, [_ctxp, C.c_int, C.c_int, _f32p, _f32p, C.c_void_p])
@@ -129,6 +137,49 @@ def warm_layers() -> list:
 # spectralPb/spectralPb.cu:101,706-712).
 GPB_ALL_BLOCK_ORIENTATION = (7, 6, 5, 4, 3, 2, 1, 0)
 KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb", "warm"]
+
+
+_sig("gpb_frame_size", C.c_size_t, [C.c_int, C.c_void_p])
+_sig("gpb_frame_mats", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t])
+_sig("gpb_unframe_mats", C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p])
+
+_CV_DEPTH = {np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16): 2, np.dtype(np.int16): 3,
+             np.dtype(np.int32): 4, np.dtype(np.float32): 5, np.dtype(np.float64): 6}
+
+
+def frame_mats(mats) -> bytes:
+    """The byte frame (count, then per matrix {rows, cols, channels, OpenCV type} and the raw elements) of
+    2-D (rows, cols) or 3-D (rows, cols, channels) arrays: what MpiSendMatVec sends, in one buffer."""
+    arrs = [np.ascontiguousarray(m) for m in mats]
+    dims = np.zeros((max(1, len(arrs)), 4), np.int32)
+    for i, a in enumerate(arrs):
+        ch = a.shape[2] if a.ndim == 3 else 1
+        dims[i] = (a.shape[0], a.shape[1], ch, _CV_DEPTH[a.dtype] + 8 * (ch - 1))
+    size = lib.gpb_frame_size(len(arrs), dims.ctypes.data)
+    if size == 0:
+        raise GpbError(GPB_ERR_ARG, "frame_mats: unsupported matrix")
+    ptrs = (C.c_void_p * max(1, len(arrs)))(*[a.ctypes.data for a in arrs])
+    out = np.empty(size, np.uint8)
+    _check(lib.gpb_frame_mats(len(arrs), dims.ctypes.data, ptrs, out.ctypes.data, size), what="frame_mats")
+    return out.tobytes()
+
+
+def unframe_mats(frame: bytes, max_count: int = 1024) -> list:
+    """Inverse of frame_mats: the matrices of a frame, as arrays."""
+    buf = np.frombuffer(frame, np.uint8)
+    dims = np.zeros((max_count, 4), np.int32)
+    offs = np.zeros(max_count, np.uint64)
+    n = C.c_int(0)
+    _check(lib.gpb_unframe_mats(buf.ctypes.data, buf.size, max_count, C.byref(n), dims.ctypes.data, offs.ctypes.data),
+           what="unframe_mats")
+    inv = {v: k for k, v in _CV_DEPTH.items()}
+    out = []
+    for i in range(n.value):
+        r, c, ch, t = (int(v) for v in dims[i])
+        dt = inv[t & 7]
+        a = np.frombuffer(frame, dt, count=r * c * ch, offset=int(offs[i])).reshape((r, c, ch) if ch > 1 else (r, c))
+        out.append(a.copy())
+    return out
 
 
 def ic_program(left: int, right: int, top: int, bottom: int):
@@ -231,6 +282,28 @@ class Context:
                                  np.ascontiguousarray(codebook, np.float32).reshape(-1), out.reshape(-1)))
         return out
 
+    def texton_features(self, packed: np.ndarray, codebook: np.ndarray, superpixels: np.ndarray,
+                        nlabels: int) -> np.ndarray:
+        """Per-superpixel texton histograms [nlabels, 32] (10 * count / size)."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        sp = np.ascontiguousarray(superpixels, np.int32)
+        out = np.empty((nlabels, 32), np.float64)
+        self._chk(lib.gpb_texton_features(self.h, W, H, packed.ctypes.data, np.ascontiguousarray(codebook, np.float32)
+                                          .ctypes.data, sp.ctypes.data, nlabels, out.ctypes.data))
+        return out
+
+    def texton_patch_features(self, packed: np.ndarray, codebook: np.ndarray, centres: np.ndarray) -> np.ndarray:
+        """Texton histograms [n, 32] of the clipped 65 x 65 windows around centres [n, 2] (row, col)."""
+        packed = np.ascontiguousarray(packed, np.uint32)
+        H, W = packed.shape
+        c = np.ascontiguousarray(centres, np.int32).reshape(-1, 2)
+        out = np.empty((c.shape[0], 32), np.float64)
+        self._chk(lib.gpb_texton_patch_features(self.h, W, H, packed.ctypes.data,
+                                                np.ascontiguousarray(codebook, np.float32).ctypes.data, c.shape[0],
+                                                c.ctypes.data, out.ctypes.data))
+        return out
+
     def lab_quantize(self, packed: np.ndarray):
         packed = np.ascontiguousarray(packed, np.uint32)
         H, W = packed.shape
@@ -259,6 +332,14 @@ class Context:
         self._chk(lib.gpb_set_eig_options(self.h, nvec, basis, keep, tol, max_matvecs))
         self.nvec = nvec
 
+    def set_lanczos_fusion(self, on: bool):
+        """Fused single-vector Lanczos steps (default) or the separate kernels."""
+        self._chk(lib.gpb_set_lanczos_fusion(self.h, 1 if on else 0))
+
+    def set_eig_convergence(self, rel: bool):
+        """False: residual <= tol; True: residual <= tol * Ritz value (Spectra's rule)."""
+        self._chk(lib.gpb_set_eig_convergence(self.h, 1 if rel else 0))
+
     @staticmethod
     def _stats_dict(s) -> dict:
         return dict(stage_ms=list(s.stage_ms), matvecs=s.lanczos_matvecs, restarts=s.lanczos_restarts,
@@ -270,6 +351,17 @@ class Context:
         s = Stats()
         self._chk(lib.gpb_get_patch_stats(self.h, patch, C.byref(s)))
         return self._stats_dict(s)
+
+    def patch_eigenvectors(self, patch: int, w: int, h: int, nvec: int = 9):
+        """Eigenvectors [nvec, h, w], degree [h, w], W planes [40, h, w] and mPb_nmax [h, w] of a patch
+        of the last patch call (w, h: that patch's rectangle)."""
+        ev = np.empty((nvec, h, w), np.float32)
+        deg = np.empty((h, w), np.float32)
+        wup = np.empty((40, h, w), np.float32)
+        nm = np.empty((h, w), np.float32)
+        self._chk(lib.gpb_patch_eigenvectors(self.h, patch, nvec, ev.ctypes.data, deg.ctypes.data, wup.ctypes.data,
+                                             nm.ctypes.data))
+        return ev, deg, wup, nm
 
     def compute_patches(self, packed: np.ndarray, codebook: np.ndarray, rects: np.ndarray, cores: np.ndarray,
                         mode: int = 1):
