@@ -28,6 +28,7 @@ extern "C" {
 #define GPB_ERR_HIP (-2)     /* HIP runtime error (allocation, launch) */
 #define GPB_ERR_NOCONV (-3)  /* eigensolver stopped before the residual target */
 #define GPB_ERR_INTERNAL (-4)
+#define GPB_ERR_TOO_LARGE (-5) /* image beyond GPB_NOALLOC_MAX_PIXELS (gpb_nonmax_noalloc) */
 
 typedef struct gpb_ctx gpb_ctx;
 
@@ -129,6 +130,21 @@ int gpb_local_cues(gpb_ctx* ctx, int width, int height, const uint32_t* packed, 
  * and the clip of src/multiscalePb.cpp:251-261. */
 int gpb_nonmax(gpb_ctx* ctx, int width, int height, const float* mpb_all, float* nmax);
 
+/* gpb_nonmax on device buffers without allocating: d_mpb_all[8][N] ->
+ * d_nmax[N] on the current HIP device, enqueued on stream (a hipStream_t; NULL
+ * = the null stream); returns once enqueued, the caller synchronises.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree and
+ * needs no context; its scratch (5 bytes per pixel) is a file-scope device
+ * array of 20 MiB that each GPU holds until the process ends, so
+ * width * height <= GPB_NOALLOC_MAX_PIXELS (2048 x 2048) and height <= 65535;
+ * larger images return GPB_ERR_TOO_LARGE (GPB_ERR_ARG for bad arguments,
+ * GPB_ERR_HIP for a failed launch).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two streams, two host threads) race,
+ * so serialise them.  Results equal gpb_nonmax's bit for bit. */
+#define GPB_NOALLOC_MAX_PIXELS (2048 * 2048)
+int gpb_nonmax_noalloc(int width, int height, const float* d_mpb_all, float* d_nmax, void* stream);
 
 /* ---- stages (c)-(e): affinity, eigenvectors, sPb, gPb ---------------------- */
 

@@ -52,6 +52,8 @@ int gpb_ctx_create(int device, gpb_ctx** out) {
   const int rc = guarded(ctx, [&] {
     GPB_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
     upload_tables(ctx->c);
+    // this device's address of the no-allocation entry point's scratch array
+    if (noalloc_scratch_setup(device) != GPB_OK) throw HipError("hipGetSymbolAddress(scratch) failed");
   });
   if (rc != GPB_OK) {
     delete ctx;

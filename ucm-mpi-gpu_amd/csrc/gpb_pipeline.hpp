@@ -80,6 +80,13 @@ void launch_fill_f64(Context& c, double* p, size_t n, double v);
 void launch_local_cues_f64(Context& c, const uint32_t* d_rgb, const uint8_t* d_text, int W, int H, double* d_mpb_all,
                            double* d_gcue, double* d_cues);
 void launch_nonmax_f64(Context& c, const double* d_mpb_all, int W, int H, double* d_nmax);
+// The FP32 nonmax of launch_nonmax on caller scratch (a: N floats, k: N
+// bytes) and a caller stream: no context, no allocation.
+void launch_nonmax_scratch(hipStream_t stream, const float* d_mpb_all, int W, int H, float* a, uint8_t* k,
+                           float* d_nmax);
+// Looks up this device's copy of the no-allocation scratch array (the device
+// must be current); GPB_OK or GPB_ERR_HIP.
+int noalloc_scratch_setup(int device);
 void launch_spb_f64(Context& c, const Stencil64& st, const double* d_evecs, const double* evals, int K,
                     double* d_spb);
 void launch_gpb_f64(Context& c, int W, int H, const double* d_gcue, const double* d_spb, const double* d_nmax,

@@ -677,4 +677,20 @@ void launch_nonmax_f64(Context& c, const double* d_mpb_all, int W, int H, double
   nonmax_t<double>(c, d_mpb_all, W, H, d_nmax);
 }
 
+void launch_nonmax_scratch(hipStream_t stream, const float* d_mpb_all, int W, int H, float* a, uint8_t* k,
+                           float* d_nmax) {
+  NmsTable T;
+  const NonmaxTable& nt = nonmax_table();
+  for (int o = 0; o < kNumOri; o++) {
+    T.branch[o] = nt.branch[o];
+    T.d[o] = nt.d[o];
+    for (int j = 0; j < kNumOri; j++) T.cosd[o][j] = nt.cosdiff[o][j];
+  }
+  const size_t N = size_t(W) * H;
+  hipLaunchKernelGGL(max_ori_kernel<float>, dim3(div_up(N, 256)), dim3(256), 0, stream, d_mpb_all, N, a, k);
+  GPB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(nonmax_kernel<float>, dim3(div_up(W, 256), H), dim3(256), 0, stream, a, k, W, H, T, d_nmax);
+  GPB_LAUNCH_CHECK();
+}
+
 }  // namespace gpb
