@@ -214,6 +214,88 @@ def bench_warm(gpb, ctx, step, S: int, steps: int, tol: float):
     return out
 
 
+def bench_parity(gpb, ctx, step, S: int, tol_rel: float = 3e-4):
+    """The plain form at the parity setting: convergence when every pair's
+    residual is within tol_rel times its Ritz value (the reference CPU solver's
+    rule, Spectra num_converged), where the eigenvalue (1e-5 relative),
+    subspace (1e-3 rad) and gPb (1e-3) bars hold against the all-FP64 solve on
+    this image (tests/test_configs_scale.py); one timed step."""
+    import torch
+
+    ctx.set_eig_convergence(True)
+    ctx.set_eig_options(nvec=9, basis=48, keep=16, tol=tol_rel, max_matvecs=80000)
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+    finally:
+        ctx.set_eig_convergence(False)
+        ctx.set_eig_options(nvec=9, basis=48, keep=16, tol=1e-3, max_matvecs=40000)
+    return dict(value=round(S * S / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3, 1), matvecs=st["matvecs"],
+                restarts=st["restarts"], stage_ms=[round(x, 1) for x in st["stage_ms"]],
+                workload=f"the plain form's image, 9 eigenvectors, residual <= {tol_rel:g} x Ritz value (basis 48, "
+                         f"keep 16), FP64 Rayleigh-Ritz refinement")
+
+
+def bench_host_to_host(gpb, ctx, packed, codebook, steps: int):
+    """The plain form through the host-buffer entry point: upload of the packed
+    image and codebook, the whole path, download of the gPb maximum and the 8
+    oriented maps (computeGPb's outputs)."""
+    gpb_out = None
+    ctx.compute(packed, codebook)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpb_out = ctx.compute(packed, codebook)
+    dt = (time.perf_counter() - t0) / steps
+    S = packed.shape[0]
+    del gpb_out
+    return dict(value=round(S * S / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3, 1),
+                workload="the plain form's image, host buffers in and out (4 B/px up; 4 + 32 B/px of gPb maps and "
+                         "the mPb_nmax map down)")
+
+
+def bench_c3(gpb, ctx, tol: float, cpu):
+    """The 4096x4096 three-class image (1600 regions, sigma 8, seed 3): the plain
+    form's whole path, device-resident, beside the reference CPU path's time for
+    the same image in its deployment shape (16 x 16 patches of 256 x 256, one
+    process per patch on the host's cores, estimated from the patches the CPU
+    baseline timed)."""
+    import numpy as np
+    import torch
+
+    S = 4096
+    packed = gpb.pack_rgb(gpb.synth_voronoi(S, S, 1600, 8.0, 3, 3))
+    cb = gpb.make_codebook(packed, seed=3, iters=10)
+    dev = torch.device("cuda:0")
+    d_rgb = torch.from_numpy(packed.view(np.int32)).to(dev)
+    d_cb = torch.from_numpy(cb).to(dev)
+    d_gmax = torch.empty((S, S), dtype=torch.float32, device=dev)
+    d_gall = torch.empty((8, S, S), dtype=torch.float32, device=dev)
+    run = lambda: ctx.compute_device(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), d_gmax.data_ptr(), d_gall.data_ptr())
+    run()  # warm-up (tile tuning for the shape, workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    out = dict(value=round(S * S / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3, 1), matvecs=st["matvecs"],
+               stage_ms=[round(x, 1) for x in st["stage_ms"]],
+               workload=f"4096x4096 three-class Voronoi (1600 regions, sigma 8, seed 3), 9 eigenvectors, residual "
+                        f"<= {tol:g}")
+    if cpu is not None:
+        est = cpu["c3_4096_patch_split_seconds_est"]
+        out["cpu_reference_seconds_est"] = est
+        out["speedup_vs_cpu_reference"] = round(est / dt, 1)
+        out["cpu_cores"] = cpu["cores"]
+    del d_rgb, d_gmax, d_gall
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,8 +315,9 @@ def main():
     ap.add_argument("--full-reorth", action="store_true",
                     help="whole-basis Gram-Schmidt on every Lanczos step (default: partial reorthogonalisation)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without per-kernel HIP events")
-    ap.add_argument("--forms", default="batched,warm",
-                    help="comma list of extra forms to time: batched (C4), warm (learned warm start), none")
+    ap.add_argument("--forms", default="parity,warm,batched,host,c3",
+                    help="comma list of extra forms to time: parity (the Spectra-rule setting), warm (learned warm "
+                         "start), batched (C4), host (host buffers in and out), c3 (4096x4096), none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -350,10 +433,16 @@ def main():
     forms = [f for f in args.forms.split(",") if f and f != "none"]
     if forms:
         line["forms"] = {}
+    if "parity" in forms:
+        line["forms"]["parity_setting"] = bench_parity(gpb, ctx, step, S)
     if "warm" in forms:
         line["forms"]["warm_start"] = bench_warm(gpb, ctx, step, S, args.steps, tol)
+    if "host" in forms:
+        line["forms"]["host_to_host"] = bench_host_to_host(gpb, ctx, packed, codebook, args.steps)
     if "batched" in forms:
         line["forms"]["batched_patches"] = bench_patches(gpb, ctx, tol)
+    if "c3" in forms:
+        line["forms"]["c3_4096"] = bench_c3(gpb, ctx, tol, cpu)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

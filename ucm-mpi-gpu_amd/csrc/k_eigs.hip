@@ -209,14 +209,76 @@ void launch_spmv(Context& c, const Stencil& st, const float* x, float* y, double
 
 void launch_spmv(Context& c, const Stencil& st, const float* x, float* y) { launch_spmv(c, st, x, y, nullptr); }
 
-// One fused step (lanczos_step_kernel) with the tile the shape was tuned to;
-// xmode 1 forms v_j from w_in and v_{j-1}, 0 reads it from v_out.
+static void lanczos_step_dispatch(Context& c, const Stencil& st, int code, int xmode, const float* w_in,
+                                  const float* v_prev, float* v_out, float* w_out, const double* aeff,
+                                  const double* beta, int j, double* partial);
+
+__global__ void fill_f64_kernel(double* p, int n, double v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// Tile code of the fused step for this shape: every variant timed (HIP
+// events, 3 steps after one warm-up) the first time a shape is seen, on
+// scratch vectors; the fastest is cached (its own choice: the fused kernel's
+// LDS and register needs differ from the plain SpMV's).
+static int tune_lanczos_step(Context& c, const Stencil& st) {
+  if (c.spmv_rows_forced) return c.spmv_rows_forced;
+  const uint64_t key = ((uint64_t(st.W) << 32) | uint32_t(st.H)) ^ (c.spmv_fp64 ? (1ull << 62) : 0) ^ (1ull << 61);
+  auto it = c.spmv_rows.find(key);
+  if (it != c.spmv_rows.end()) return it->second;
+  float* win = c.ws<float>("tune.x", st.Np);
+  float* vp = c.ws<float>("tune.y", st.Np);
+  float* vo = c.ws<float>("tune.v", st.Np);
+  float* wo = c.ws<float>("tune.w", st.Np);
+  double* bt = c.ws<double>("tune.beta", 2 * kMaxBasis);
+  double* ae = c.ws<double>("tune.aeff", kMaxBasis);
+  double* part = c.ws<double>("tune.part", size_t(3) * spmv_blocks(st));
+  launch_fill(c, win, st.Np, 1.0f);
+  launch_fill(c, vp, st.Np, 0.0f);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(256), 0, c.stream, bt, 2 * kMaxBasis, 1.0);
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(256), 0, c.stream, ae, kMaxBasis, 0.0);
+  GPB_LAUNCH_CHECK();
+  hipEvent_t e0, e1;
+  GPB_HIP(hipEventCreate(&e0));
+  GPB_HIP(hipEventCreate(&e1));
+  int best = 42;
+  float best_ms = 1e30f;
+  for (int code : kSpmvTiles) {
+    lanczos_step_dispatch(c, st, code, 1, win, vp, vo, wo, ae, bt, 1, part);
+    GPB_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < 3; r++) lanczos_step_dispatch(c, st, code, 1, win, vp, vo, wo, ae, bt, 1, part);
+    GPB_HIP(hipEventRecord(e1, c.stream));
+    GPB_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GPB_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (c.verbose)
+      std::fprintf(stderr, "gpb: fused-step tuning %dx%d: tile %d (64x%d): %.1f us\n", st.W, st.H, code,
+                   tile_rows(code), ms * 1e3f / 3);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = code;
+    }
+  }
+  GPB_HIP(hipEventDestroy(e0));
+  GPB_HIP(hipEventDestroy(e1));
+  c.spmv_rows[key] = best;
+  return best;
+}
+
+// One fused step (lanczos_step_kernel) with the tile tuned for it; xmode 1
+// forms v_j from w_in and v_{j-1}, 0 reads it from v_out.
 static void launch_lanczos_step(Context& c, const Stencil& st, int xmode, const float* w_in, const float* v_prev,
                                 float* v_out, float* w_out, const double* aeff, const double* beta, int j,
                                 double* partial) {
-  const int code = tune_spmv(c, st);
+  const int code = tune_lanczos_step(c, st);
   // W planes, w_in, v_{j-1}, s read; v_j (xmode 1) and w written
   KScope ks(c, kKSpmv, double(st.W) * st.H * (4 * kStencilHalf + (xmode ? 20 : 16)));
+  lanczos_step_dispatch(c, st, code, xmode, w_in, v_prev, v_out, w_out, aeff, beta, j, partial);
+}
+
+static void lanczos_step_dispatch(Context& c, const Stencil& st, int code, int xmode, const float* w_in,
+                                  const float* v_prev, float* v_out, float* w_out, const double* aeff,
+                                  const double* beta, int j, double* partial) {
   const int ntiles = spmv_blocks(st, code);
   const dim3 grid(unsigned(div_up(ntiles, kNumXcd) * kNumXcd));
 #define GPB_STEP(NY, R, XM)                                                                                          \
@@ -1077,6 +1139,9 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   double* beta = c.ws<double>("eig.beta", 2 * kMaxBasis);
   double* alpha = c.ws<double>("eig.alpha", 1);
   const int spmv_g = spmv_blocks(st, tune_spmv(c, st));
+  const int step_g = (c.reorth_mode >= 1 && eig_partial_reorth(opt) && c.lanczos_fused)
+                         ? spmv_blocks(st, tune_lanczos_step(c, st))
+                         : 1;
   double* xy_part = c.ws<double>("eig.xy", size_t(spmv_g));
   double* Ydev = c.ws<double>("eig.Y", size_t(m) * m);
   // partial reorthogonalisation for targets the FP64 refinement does not
@@ -1097,7 +1162,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   auto wj = [&](int j) { return wbuf[j & 1]; };
   double* aeff = c.ws<double>("eig.aeff", kMaxBasis);
   double* alpha_f = c.ws<double>("eig.alpha_f", 1);
-  double* part3 = c.ws<double>("eig.part3", size_t(3) * spmv_g);
+  double* part3 = c.ws<double>("eig.part3", size_t(3) * step_g);
   if (fused) {
     launch_fill(c, wbuf[1], Np, 0.f);
     launch_fill(c, V, size_t(m + 1) * Np, 0.f);
@@ -1203,7 +1268,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         // beta_j and omega from its partial sums
         const bool resumed = j == j0 && j0 > k;
         launch_lanczos_step(c, st, resumed ? 0 : 1, wj(j - 1), V + size_t(j - 1) * Np, vj, w, aeff, beta, j, part3);
-        hipLaunchKernelGGL(omega_fused_kernel, dim3(1), dim3(256), 0, c.stream, T, m, j, part3, spmv_g, beta, aeff,
+        hipLaunchKernelGGL(omega_fused_kernel, dim3(1), dim3(256), 0, c.stream, T, m, j, part3, step_g, beta, aeff,
                            alpha_f, om_slot(j), om_slot(j - 1), om_slot(j + 1), flag, pend, double(Np), gacc,
                            pro_thr, opt_cycle ? first_viol : nullptr);
         GPB_LAUNCH_CHECK();
