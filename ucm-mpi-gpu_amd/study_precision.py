@@ -15,11 +15,13 @@ Per output it reports the error of prod against f64: mPb_nmax max abs error
 (and pixels whose nonmax survival flips), eigenvalue relative error of pairs
 1..8, the largest principal angle between the eigenvector subspaces (D inner
 product), and gPb max abs error (oriented, and the masked maximum where both
-masks agree).  The eigensolves run at the residual target given (the bench's
-1e-3) and at 1e-6, where the solver's convergence error is below the
-precision effects.  Prints one JSON object.
+masks agree).  Residual targets are "abs:T" (residual <= T, the bench's rule at
+T = 1e-3) or "rel:T" (residual <= T times the Ritz value, the reference CPU
+solver's rule); the production solves run at each target given, the FP64
+reference solves at --ref-tol.  Prints one JSON object.
 
     python ucm-mpi-gpu_amd/study_precision.py [--size 4096] [--regions 1600] [--seed 3]
+        [--tols abs:1e-3,rel:3e-4] [--ref-tol rel:1e-6] [--basis 48 --keep 16]
 """
 from __future__ import annotations
 
@@ -60,13 +62,23 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--regions", type=int, default=1600)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--tols", default="1e-3,1e-6", help="residual targets of the production solves")
-    ap.add_argument("--ref-tol", type=float, default=1e-6, help="residual target of the FP64 reference solves")
-    ap.add_argument("--basis", type=int, default=40, help="Lanczos basis of every solve (clustered spectra need more)")
-    ap.add_argument("--keep", type=int, default=20)
-    ap.add_argument("--max-matvecs", type=int, default=40000)
+    ap.add_argument("--tols", default="abs:1e-3,rel:3e-4", help="residual targets of the production solves")
+    ap.add_argument("--ref-tol", default="rel:1e-6", help="residual target of the FP64 reference solves")
+    ap.add_argument("--basis", type=int, default=48, help="Lanczos basis of the production solves")
+    ap.add_argument("--keep", type=int, default=16)
+    ap.add_argument("--ref-basis", type=int, default=64, help="Lanczos basis of the FP64 reference solves")
+    ap.add_argument("--ref-keep", type=int, default=32)
+    ap.add_argument("--max-matvecs", type=int, default=80000)
     args = ap.parse_args()
-    tols = sorted({float(t) for t in args.tols.split(",")}, reverse=True)
+
+    def rule(text):
+        kind, _, t = text.partition(":")
+        if kind not in ("abs", "rel"):
+            raise SystemExit(f"target {text}: expected abs:T or rel:T")
+        return kind == "rel", float(t)
+
+    tols = [rule(t) for t in args.tols.split(",")]
+    ref_rel, ref_tol = rule(args.ref_tol)
     gpb = _binding()
     S = args.size
     t0 = time.perf_counter()
@@ -80,7 +92,8 @@ def main():
     res = dict(workload=f"{S}x{S} seeded three-class Voronoi ({args.regions} regions, sigma 8, seed {args.seed}), "
                         f"9 eigenvectors")
     # the all-FP64 reference (tight residual)
-    ctx.set_eig_options(nvec=9, basis=args.basis, keep=args.keep, tol=args.ref_tol, max_matvecs=args.max_matvecs)
+    ctx.set_eig_convergence(ref_rel)
+    ctx.set_eig_options(nvec=9, basis=args.ref_basis, keep=args.ref_keep, tol=ref_tol, max_matvecs=args.max_matvecs)
     t = time.perf_counter()
     f64 = ctx.compute_f64(packed, cb)
     res["f64_ms"] = round((time.perf_counter() - t) * 1e3, 1)
@@ -89,11 +102,13 @@ def main():
     _, D = ctx.build_w(f64["mpb_nmax"].astype(np.float32))
     res["ref_tol"] = args.ref_tol
     res["basis"], res["keep"] = args.basis, args.keep
+    res["ref_basis"], res["ref_keep"] = args.ref_basis, args.ref_keep
     # the FP64 solve on the production mPb (the mPb does not depend on the
     # residual target, so it runs once)
     ev_s = vec_s = None
-    for tol in tols:
-        key = f"tol_{tol:g}"
+    for rel, tol in tols:
+        key = f"{'rel' if rel else 'abs'}_{tol:g}"
+        ctx.set_eig_convergence(rel)
         ctx.set_eig_options(nvec=9, basis=args.basis, keep=args.keep, tol=tol, max_matvecs=args.max_matvecs)
         ctx.set_spmv_fp64(True)
         t = time.perf_counter()
@@ -102,11 +117,17 @@ def main():
         ev, vec = ctx.eigs(out["mpb_nmax"])
         matvecs = ctx.stats()["matvecs"]
         ctx.set_spmv_fp64(False)
-        ev_b, vec_b = ctx.eigs(out["mpb_nmax"])
+        try:
+            ev_b, vec_b = ctx.eigs(out["mpb_nmax"])
+        except gpb.GpbError as e:  # FP32 arithmetic may not reach a tight target
+            print(f"{key}: FP32-SpMV solve: {e}", file=sys.stderr, flush=True)
+            ev_b = vec_b = None
         ctx.set_spmv_fp64(True)
         print(f"{key}: production and FP32-SpMV solves done", file=sys.stderr, flush=True)
         if ev_s is None:
-            ctx.set_eig_options(nvec=9, basis=args.basis, keep=args.keep, tol=args.ref_tol, max_matvecs=args.max_matvecs)
+            ctx.set_eig_convergence(ref_rel)
+            ctx.set_eig_options(nvec=9, basis=args.ref_basis, keep=args.ref_keep, tol=ref_tol,
+                                max_matvecs=args.max_matvecs)
             ev_s, vec_s = ctx.eigs_f64(out["mpb_nmax"].astype(np.float64))
             print(f"FP64 solve on the production mPb done", file=sys.stderr, flush=True)
         same = (out["mpb_nmax"] > 0.05) == (f64["mpb_nmax"] > 0.05)
@@ -119,8 +140,9 @@ def main():
         r["subspace_angle_end_to_end"] = subspace_angle(vec[1:], f64["evecs"][1:], D)
         r["eig_rel_err_solver"] = float(np.max(np.abs(ev[1:] - ev_s[1:]) / ev_s[1:]))
         r["subspace_angle_solver"] = subspace_angle(vec[1:], vec_s[1:], D)
-        r["eig_rel_err_solver_spmv32"] = float(np.max(np.abs(ev_b[1:] - ev_s[1:]) / ev_s[1:]))
-        r["subspace_angle_solver_spmv32"] = subspace_angle(vec_b[1:], vec_s[1:], D)
+        if ev_b is not None:
+            r["eig_rel_err_solver_spmv32"] = float(np.max(np.abs(ev_b[1:] - ev_s[1:]) / ev_s[1:]))
+            r["subspace_angle_solver_spmv32"] = subspace_angle(vec_b[1:], vec_s[1:], D)
         r["gpb_orient_max_abs_err"] = float(np.abs(out["gpb_orient"] - f64["gpb_orient"]).max())
         r["gpb_max_max_abs_err"] = float(np.abs(out["gpb_max"] - f64["gpb_max"])[same].max())
         r["gpb_mask_flips"] = int(np.count_nonzero(~same))
