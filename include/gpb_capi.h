@@ -452,6 +452,57 @@ int gpb_compute_f64(gpb_ctx* ctx, int width, int height, const uint32_t* packed,
  * evals[nvec], evecs[nvec][N] (may be NULL). */
 int gpb_eigs_f64(gpb_ctx* ctx, int width, int height, const double* mpb_nmax, double* evals, double* evecs);
 
+/* ---- OWT-UCM: the consumer of the 8 oriented gPb maps --------------------------- */
+
+/* Ultrametric contour map from the oriented gPb maps pb_oriented[8][H][W]
+ * (double, the order GlobalPb returns them in): the finest partition (the
+ * maximum over orientations scaled to 8 bits, its 8-connected watershed, the
+ * watershed pixels' contours subdivided into near-straight edges, each edge's
+ * pixels taking the map of its chord's orientation), the doubled grid and its
+ * cleaning, greedy merging of regions by mean boundary strength, and the
+ * logistic calibration.  ucm: (2H+1) x (2W+1) row-major when double_size
+ * (fmt "doubleSize"), else (H+1) x (W+1) (the even sites; last row and
+ * column 0).  Replaces Contours2Ucm (src/contours2ucm.cpp:405-490) with
+ * CreateFinestPartition (:251-404), watershed_yupeng (src/watershed.cpp:28-108),
+ * fit_contour / MexContourSides (src/fit_contour.cpp:6-22,
+ * src/mex_contour_sides.cpp:120-224), SuperContour4C (:24-78),
+ * CleanWatersheds (:80-189), bwlabel (src/MATLAB_functions.cpp:182-222),
+ * UcmMeanPb (src/ucm_mean_pb.cpp:402-449) and NormalizeOutput (:191-249).
+ * Host pointers, no context (all of it is sequential host work).
+ * GPB_ERR_INTERNAL where a pixel site of the cleaned grid is a boundary (the
+ * reference then indexes its regions at -1). */
+int gpb_contours2ucm(int width, int height, const double* pb_oriented, int double_size, double* ucm);
+
+/* The intermediate results of gpb_contours2ucm (each may be NULL): pb8[N]
+ * (the 8-bit map), ws[N] (watershed labels, 0 = watershed pixel), ws_wt[N]
+ * (finest-partition contour strength), ws_wt2 and super_ucm
+ * [(2H+1)(2W+1)] (the cleaned doubled grid, the map before calibration),
+ * labels[N] (initial regions), *regions, and *contours_failed (1 when the
+ * contour subdivision could not be completed and the partition has no
+ * contours, as the reference's exception handler leaves it).  No counterpart
+ * beyond the functions gpb_contours2ucm replaces. */
+int gpb_ucm_stages(int width, int height, const double* pb_oriented, uint8_t* pb8, int32_t* ws, double* ws_wt,
+                   double* ws_wt2, int32_t* labels, double* super_ucm, int* regions, int* contours_failed);
+
+/* 8-connected watershed of an 8-bit image[H][W]: levels in increasing order,
+ * each level's pixels in raster order; labels[N] > 0 for basins (numbered in
+ * order of creation), 0 for watershed pixels.  Replaces watershed_yupeng
+ * (src/watershed.cpp:28-108). */
+int gpb_watershed8(int width, int height, const uint8_t* image, int32_t* labels);
+
+/* Contours of a binary image bw[H][W] (nonzero = contour pixel), as
+ * lib_image's contour_set builds and subdivides them (globalPb/src/math/
+ * libraries/lib_image.cc:5320-5485, 5953-6044, 6138-6326; called from
+ * src/mex_contour_sides.cpp:136-146): vertices[2 * v] = (row, column); edges
+ * [2 * e] = (start vertex, end vertex); the interior points of edge e are
+ * points[2 * k] (row, column) for k in [point_offsets[e], point_offsets[e+1]).
+ * Counts go to *n_vertices, *n_edges, *n_points; GPB_ERR_TOO_LARGE (with the
+ * counts set) when a buffer is too small, GPB_ERR_INTERNAL when the
+ * subdivision cannot be completed (the reference throws there). */
+int gpb_fit_contours(int width, int height, const uint8_t* bw, int max_vertices, int32_t* vertices, int max_edges,
+                     int32_t* edges, int64_t* point_offsets, int64_t max_points, int32_t* points, int* n_vertices,
+                     int* n_edges, int64_t* n_points);
+
 /* ---- matrix framing (moving outputs between processes) ------------------------- */
 
 /* Byte framing of the reference's matrix messages for transports other than

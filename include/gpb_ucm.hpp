@@ -146,6 +146,13 @@ std::vector<uint8_t> frame_mat_vec(const std::vector<Mat>& mats);
  * throws std::invalid_argument on a malformed frame or a non-CV_64F matrix. */
 std::vector<Mat> unframe_mat_vec(const std::vector<uint8_t>& frame);
 
+/* The ultrametric contour map of the 8 oriented gPb maps (each H x W, the
+ * order GlobalPb returns them in): (2H+1) x (2W+1) when fmt is "doubleSize",
+ * else (H+1) x (W+1).  pieceFilePat names the piece in the reference (its
+ * debug images); it is unused here.  Replaces Contours2Ucm
+ * (src/contours2ucm.cpp:405-490; gpb_contours2ucm). */
+Mat Contours2Ucm(const std::string& pieceFilePat, const std::vector<Mat>& pb_oriented, const std::string& fmt);
+
 }  // namespace gpb_ucm
 
 /* Whole gPb of one packed image on GPU rank % getCudaDeviceCount():

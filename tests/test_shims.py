@@ -62,6 +62,7 @@ def test_reference_names_exported():
         " std::vector<int, std::allocator<int> >&, std::vector<int, std::allocator<int> >&, gpb_ucm::Mat&, "
         "gpb_ucm::Mat&)",
         "gpb_ucm::GlobalPb(",
+        "gpb_ucm::Contours2Ucm(",
     ]:
         assert sig in out, sig
 
@@ -70,6 +71,20 @@ def test_cli_yaml_format(tmp_path):
     _cli("yaml", tmp_path / "m.yml")
     m = _parse_yaml(tmp_path / "m.yml")["m"]
     np.testing.assert_array_equal(m, (0.5 * np.arange(6) - 1.0 / 3.0).reshape(2, 3))
+
+
+def test_cli_contours2ucm_matches_capi(gpb, tmp_path):
+    """gpb_ucm::Contours2Ucm (through gpb_cli) against gpb_contours2ucm, both
+    output sizes."""
+    from scipy import ndimage
+    rng = np.random.default_rng(4)
+    H, W = 33, 41
+    pbo = np.stack([np.abs(ndimage.gaussian_filter(rng.standard_normal((H, W)), 1.8)) for _ in range(8)])
+    (tmp_path / "maps.f64").write_bytes(np.ascontiguousarray(pbo, np.float64).tobytes())
+    for fmt, shape in (("doubleSize", (2 * H + 1, 2 * W + 1)), ("", (H + 1, W + 1))):
+        _cli("ucm", tmp_path / "maps.f64", W, H, tmp_path / "u.f64", *([fmt] if fmt else []))
+        got = np.fromfile(tmp_path / "u.f64", np.float64).reshape(shape)
+        np.testing.assert_array_equal(got, gpb.contours2ucm(pbo, fmt == "doubleSize"))
 
 
 def test_cli_rejects_bad_image(tmp_path):

@@ -151,6 +151,26 @@ int main(int argc, char** argv) {
       return 0;
     }
     if (mode == "shims" && argc == 4) return run_shims(argv[2], argv[3]);
+    if (mode == "ucm" && (argc == 6 || argc == 7)) {
+      // 8 oriented H x W maps (raw doubles) -> Contours2Ucm -> raw doubles
+      const int w = std::atoi(argv[3]), h = std::atoi(argv[4]);
+      if (w < 2 || h < 2) throw std::invalid_argument("bad map size");
+      std::vector<Mat> pbo(8, Mat(h, w));
+      FILE* f = std::fopen(argv[2], "rb");
+      if (!f) throw std::invalid_argument("cannot read maps");
+      for (auto& m : pbo)
+        if (std::fread(m.data.data(), sizeof(double), m.data.size(), f) != m.data.size()) {
+          std::fclose(f);
+          throw std::invalid_argument("short maps file");
+        }
+      std::fclose(f);
+      const Mat u = gpb_ucm::Contours2Ucm("piece", pbo, argc == 7 ? argv[6] : "");
+      FILE* o = std::fopen(argv[5], "wb");
+      if (!o) throw std::runtime_error("cannot write output");
+      std::fwrite(u.data.data(), sizeof(double), u.data.size(), o);
+      std::fclose(o);
+      return 0;
+    }
   } catch (const std::invalid_argument& e) {
     std::fprintf(stderr, "gpb_cli: %s\n", e.what());
     return 2;
@@ -163,6 +183,6 @@ int main(int argc, char** argv) {
   }
   std::fprintf(stderr,
                "usage: gpb_cli gpb <in.ppm> <outprefix> | compute <in.ppm> <out.bin> | shims <in.ppm> <outdir> |"
-               " yaml <out.yml> | devices\n");
+               " ucm <maps.f64> <W> <H> <out.f64> [doubleSize] | yaml <out.yml> | devices\n");
   return 2;
 }

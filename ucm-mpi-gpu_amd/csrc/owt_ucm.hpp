@@ -1,0 +1,57 @@
+// OWT-UCM, the consumer of the 8 oriented gPb maps: the ultrametric contour
+// map of src/contours2ucm.cpp (Contours2Ucm :405-490).  Host code: every stage
+// after the 8-bit scaling is a sequential graph or geometry algorithm whose
+// result depends on its visiting order (a level-by-level raster watershed,
+// contour tracing and subdivision, greedy region merging), so it runs on the
+// CPU in the reference's order, with data structures of its own.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace gpb {
+namespace owt {
+
+// Contours of a binary image (nonzero = contour pixel), as lib_image's
+// contour_set builds them (globalPb/src/math/libraries/lib_image.cc:5320-5485
+// vertices and traced edges, :5953-6044 subdivide_local with its default
+// arguments, :6138-6326 subdivide_global).  Coordinates are (x, y) = (row,
+// column); an edge's points are its interior pixels in traversal order.
+struct Contours {
+  std::vector<int32_t> vx, vy;                // vertex coordinates
+  std::vector<int32_t> es, ee;                // edge start / end vertex ids
+  std::vector<std::vector<int32_t>> px, py;   // interior points of each edge
+  bool failed = false;                        // a subdivision could not be completed
+};
+
+// rows x cols row-major binary image -> contours after both subdivisions.
+// When a subdivision cannot make the topology consistent the reference's
+// caller catches the exception and continues with no contours
+// (src/mex_contour_sides.cpp:216-223); then failed is set and the set is empty.
+Contours fit_contours(const uint8_t* bw, int rows, int cols);
+
+// The 8-connected watershed of src/watershed.cpp:28-108 (levels in increasing
+// order, each level's pixels in raster order); labels > 0 for basins, 0 for
+// watershed pixels.
+std::vector<int32_t> watershed8(const uint8_t* a, int rows, int cols);
+
+// Stages of Contours2Ucm on rows x cols maps (row-major, double).
+struct UcmStages {
+  std::vector<uint8_t> pb8;      // max over orientations scaled to 8 bits
+  std::vector<int32_t> ws;       // watershed labels
+  std::vector<double> ws_wt;     // contour strength of the finest partition
+  std::vector<double> ws_wt2;    // (2 rows + 1) x (2 cols + 1): cleaned doubled grid
+  std::vector<int32_t> labels;   // rows x cols initial regions
+  std::vector<double> super_ucm; // (2 rows + 1) x (2 cols + 1), before calibration
+  Contours contours;
+  int regions = 0;
+};
+
+// The whole consumer: pb_oriented[8][rows][cols] -> ucm, (2 rows + 1) x
+// (2 cols + 1) when double_size, else (rows + 1) x (cols + 1).  stages, if
+// given, receives the intermediate results.
+void contours2ucm(const double* pb_oriented, int rows, int cols, bool double_size, double* ucm,
+                  UcmStages* stages = nullptr);
+
+}  // namespace owt
+}  // namespace gpb

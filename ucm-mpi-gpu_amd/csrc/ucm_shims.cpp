@@ -286,6 +286,25 @@ void buildW(std::vector<double>& valD, std::vector<double>& val, std::vector<int
     }
 }
 
+Mat Contours2Ucm(const std::string& pieceFilePat, const std::vector<Mat>& pb_oriented, const std::string& fmt) {
+  (void)pieceFilePat;
+  if (pb_oriented.size() != size_t(kOri)) throw std::invalid_argument("Contours2Ucm: 8 oriented maps expected");
+  const int H = pb_oriented[0].rows, W = pb_oriented[0].cols;
+  const size_t N = size_t(W) * H;
+  std::vector<double> pbo(size_t(kOri) * N);
+  for (int o = 0; o < kOri; o++) {
+    const Mat& m = pb_oriented[size_t(o)];
+    if (m.rows != H || m.cols != W || m.channels != 1)
+      throw std::invalid_argument("Contours2Ucm: the maps must be single-channel and of one size");
+    std::memcpy(pbo.data() + size_t(o) * N, m.data.data(), N * sizeof(double));
+  }
+  const bool dbl = fmt == "doubleSize";
+  Mat out = dbl ? Mat(2 * H + 1, 2 * W + 1) : Mat(H + 1, W + 1);
+  const int rc = gpb_contours2ucm(W, H, pbo.data(), dbl ? 1 : 0, out.data.data());
+  if (rc != GPB_OK) throw std::runtime_error("Contours2Ucm: gpb_contours2ucm returned " + std::to_string(rc));
+  return out;
+}
+
 std::vector<uint8_t> frame_mat_vec(const std::vector<Mat>& mats) {
   const int n = int(mats.size());
   std::vector<int> dims(size_t(4) * n);

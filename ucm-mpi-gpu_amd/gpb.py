@@ -15,6 +15,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libgpb.so"
 
 GPB_OK, GPB_ERR_ARG, GPB_ERR_HIP, GPB_ERR_NOCONV, GPB_ERR_INTERNAL = 0, -1, -2, -3, -4
+GPB_ERR_TOO_LARGE = -5
 
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} is missing: build it with `python ucm-mpi-gpu_amd/build.py`")
@@ -205,6 +206,68 @@ def _check(rc: int, ctx=None, what: str = ""):
 
 def device_count() -> int:
     return lib.gpb_device_count()
+
+
+# ---- OWT-UCM (host code, no context) ----
+_sig("gpb_contours2ucm", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
+_sig("gpb_ucm_stages", C.c_int, [C.c_int, C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p])
+_sig("gpb_watershed8", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p])
+_sig("gpb_fit_contours", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
+
+
+def contours2ucm(pb_oriented: np.ndarray, double_size: bool = True) -> np.ndarray:
+    """Contours2Ucm of the oriented maps [8, H, W]: (2H+1, 2W+1) when double_size, else (H+1, W+1)."""
+    pbo = np.ascontiguousarray(pb_oriented, np.float64)
+    _, H, W = pbo.shape
+    out = np.empty((2 * H + 1, 2 * W + 1) if double_size else (H + 1, W + 1))
+    _check(lib.gpb_contours2ucm(W, H, pbo.ctypes.data, int(double_size), out.ctypes.data), what="contours2ucm")
+    return out
+
+
+def ucm_stages(pb_oriented: np.ndarray) -> dict:
+    """The intermediate results of contours2ucm (see gpb_ucm_stages)."""
+    pbo = np.ascontiguousarray(pb_oriented, np.float64)
+    _, H, W = pbo.shape
+    d = dict(pb8=np.empty((H, W), np.uint8), ws=np.empty((H, W), np.int32), ws_wt=np.empty((H, W)),
+             ws_wt2=np.empty((2 * H + 1, 2 * W + 1)), labels=np.empty((H, W), np.int32),
+             super_ucm=np.empty((2 * H + 1, 2 * W + 1)))
+    reg, fail = C.c_int(), C.c_int()
+    _check(lib.gpb_ucm_stages(W, H, pbo.ctypes.data, d["pb8"].ctypes.data, d["ws"].ctypes.data, d["ws_wt"].ctypes.data,
+                              d["ws_wt2"].ctypes.data, d["labels"].ctypes.data, d["super_ucm"].ctypes.data,
+                              C.byref(reg), C.byref(fail)), what="ucm_stages")
+    d["regions"], d["contours_failed"] = reg.value, bool(fail.value)
+    return d
+
+
+def watershed8(image: np.ndarray) -> np.ndarray:
+    """8-connected watershed labels of an 8-bit image (0 = watershed pixel)."""
+    img = np.ascontiguousarray(image, np.uint8)
+    H, W = img.shape
+    out = np.empty((H, W), np.int32)
+    _check(lib.gpb_watershed8(W, H, img.ctypes.data, out.ctypes.data), what="watershed8")
+    return out
+
+
+def fit_contours(bw: np.ndarray):
+    """Contours of a binary image: (vertices [nv, 2], edges [ne, 2], offsets [ne + 1], points [np, 2]), rows and
+    columns; None where the subdivision cannot be completed."""
+    b = np.ascontiguousarray(bw != 0, np.uint8)
+    H, W = b.shape
+    nv, ne, npt = C.c_int(), C.c_int(), C.c_int64()
+    rc = lib.gpb_fit_contours(W, H, b.ctypes.data, 0, None, 0, None, None, 0, None, C.byref(nv), C.byref(ne),
+                              C.byref(npt))
+    if rc == GPB_ERR_INTERNAL:
+        return None
+    if rc not in (GPB_OK, GPB_ERR_TOO_LARGE):
+        _check(rc, what="fit_contours")
+    V = np.zeros((max(nv.value, 1), 2), np.int32)
+    E = np.zeros((max(ne.value, 1), 2), np.int32)
+    O = np.zeros(ne.value + 1, np.int64)
+    P = np.zeros((max(npt.value, 1), 2), np.int32)
+    _check(lib.gpb_fit_contours(W, H, b.ctypes.data, nv.value, V.ctypes.data, ne.value, E.ctypes.data, O.ctypes.data,
+                                npt.value, P.ctypes.data, C.byref(nv), C.byref(ne), C.byref(npt)), what="fit_contours")
+    return V[:nv.value], E[:ne.value], O, P[:npt.value]
 
 
 def synth_voronoi(width: int, height: int, regions: int, noise_sigma: float, seed: int, classes: int = 1,
