@@ -257,6 +257,54 @@ def bench_host_to_host(gpb, ctx, packed, codebook, steps: int):
                          "the mPb_nmax map down)")
 
 
+def bench_ucm(gpb, ctx, S: int, d_rgb, d_cb, gpb_ms: float):
+    """The consumer of the gPb maps on the plain form's image: gpb_compute_ucm
+    (gPb on the GPU, then OWT-UCM: the per-pixel stages on the GPU, the
+    watershed, contour fitting, cleaning and merging on the host), beside the
+    CPU oracle of the consumer (a Python restatement of the reference's
+    steps; the reference's own Contours2Ucm needs OpenCV) and the host-only
+    consumer on a 160 x 160 crop of the same image's oriented maps."""
+    import importlib.util
+    import numpy as np
+    import torch
+
+    ctx.compute_ucm(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), True)  # warm-up
+    ctx.set_profiling_classes(["ucm"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ucm = ctx.compute_ucm(S, S, d_rgb.data_ptr(), d_cb.data_ptr(), True)
+    dt = time.perf_counter() - t0
+    kt = ctx.kernel_times()["ucm"]
+    ctx.set_profiling_classes([])
+    out = dict(value=round(S * S / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3, 1),
+               ucm_ms=round(dt * 1e3 - gpb_ms, 1), gpb_ms=round(gpb_ms, 1),
+               ucm_kernels=dict(ms=round(kt["ms"], 3), launches=kt["launches"],
+                                gbs=round(kt["bytes"] / (kt["ms"] * 1e-3) / 1e9, 1) if kt["ms"] > 0 else 0.0),
+               ucm_nonzero=int(np.count_nonzero(ucm)),
+               workload=f"the plain form's image: gPb, then the ultrametric contour map ({2 * S + 1} x {2 * S + 1})")
+    # the oracle and the host consumer on a crop of the same maps
+    spec = importlib.util.spec_from_file_location("ucm_oracle_bench", ROOT / "oracle" / "ucm_oracle.py")
+    uo = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(uo)
+    packed = np.empty((S, S), np.uint32)
+    d_rgb_host = d_rgb.cpu().numpy().view(np.uint32)
+    packed[:] = d_rgb_host.reshape(S, S)
+    crop = 160
+    c0 = S // 2 - crop // 2
+    orient = ctx.compute(packed, d_cb.cpu().numpy())["gpb_orient"].astype(np.float64)
+    maps = np.ascontiguousarray(orient[:, c0:c0 + crop, c0:c0 + crop])
+    t0 = time.perf_counter()
+    gpb.contours2ucm(maps, True)
+    t_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    uo.contours2ucm(maps, True)
+    t_orc = time.perf_counter() - t0
+    out["cpu_sample"] = dict(sample=f"{crop}x{crop} crop of the image's oriented gPb maps", oracle_seconds=round(t_orc, 3),
+                             host_consumer_seconds=round(t_host, 4), kind="port",
+                             oracle_mpixel_s=round(crop * crop / t_orc / 1e6, 5))
+    return out
+
+
 def bench_c3(gpb, ctx, tol: float, cpu):
     """The 4096x4096 three-class image (1600 regions, sigma 8, seed 3): the plain
     form's whole path, device-resident, beside the reference CPU path's time for
@@ -315,9 +363,10 @@ def main():
     ap.add_argument("--full-reorth", action="store_true",
                     help="whole-basis Gram-Schmidt on every Lanczos step (default: partial reorthogonalisation)")
     ap.add_argument("--no-kernel-timing", action="store_true", help="time the steps without per-kernel HIP events")
-    ap.add_argument("--forms", default="parity,warm,batched,host,c3",
+    ap.add_argument("--forms", default="parity,warm,batched,host,ucm,c3",
                     help="comma list of extra forms to time: parity (the Spectra-rule setting), warm (learned warm "
-                         "start), batched (C4), host (host buffers in and out), c3 (4096x4096), none")
+                         "start), batched (C4), host (host buffers in and out), ucm (gPb then OWT-UCM), c3 "
+                         "(4096x4096), none")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
@@ -441,6 +490,8 @@ def main():
         line["forms"]["host_to_host"] = bench_host_to_host(gpb, ctx, packed, codebook, args.steps)
     if "batched" in forms:
         line["forms"]["batched_patches"] = bench_patches(gpb, ctx, tol)
+    if "ucm" in forms:
+        line["forms"]["ucm"] = bench_ucm(gpb, ctx, S, d_rgb, d_cb, ms_step)
     if "c3" in forms:
         line["forms"]["c3_4096"] = bench_c3(gpb, ctx, tol, cpu)
     if cpu is not None:

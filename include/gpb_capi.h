@@ -314,6 +314,8 @@ int gpb_set_verbose(gpb_ctx* ctx, int level);
 #define GPB_K_RITZ 7
 #define GPB_K_SPB 8
 #define GPB_K_GPB 9
+#define GPB_K_WARM 10
+#define GPB_K_UCM 11 /* the OWT-UCM per-pixel kernels (gpb_contours2ucm_device) */
 
 /* Enable (1) or disable (0) HIP-event timing of every kernel launch on ctx,
  * and clear the totals.  Replaces the StopWatch stage timers of computeGPb
@@ -472,6 +474,24 @@ int gpb_eigs_f64(gpb_ctx* ctx, int width, int height, const double* mpb_nmax, do
  * GPB_ERR_INTERNAL where a pixel site of the cleaned grid is a boundary (the
  * reference then indexes its regions at -1). */
 int gpb_contours2ucm(int width, int height, const double* pb_oriented, int double_size, double* ucm);
+
+/* gpb_contours2ucm on oriented maps already in HBM, d_pb_oriented[8][H][W]
+ * FP32 (the gpb_orient output of gpb_compute_device): the maximum over
+ * orientations, its 8-bit scaling and the contour pixels' strengths run on
+ * the GPU, the watershed, contour fitting, cleaning and merging on the host,
+ * with the same results as gpb_contours2ucm on the same values as doubles.
+ * ucm on the host.  No counterpart beyond Contours2Ucm, whose input it keeps
+ * on the device. */
+int gpb_contours2ucm_device(gpb_ctx* ctx, int width, int height, const float* d_pb_oriented, int double_size,
+                            double* ucm);
+
+/* The whole consumer chain of one image on ctx: gPb as gpb_compute_device
+ * (device packed image and codebook), then gpb_contours2ucm_device on its
+ * oriented maps; ucm on the host.  Replaces the GlobalPbCuda ->
+ * UCM_generator::generateUCM2 -> Contours2Ucm sequence of one piece
+ * (src/main.cpp:287, :427-458; src/UCMgenerator.cpp:23-51). */
+int gpb_compute_ucm(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
+                    int double_size, double* ucm);
 
 /* The intermediate results of gpb_contours2ucm (each may be NULL): pb8[N]
  * (the 8-bit map), ws[N] (watershed labels, 0 = watershed pixel), ws_wt[N]

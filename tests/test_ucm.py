@@ -280,3 +280,55 @@ def test_ucm_on_gpu_gpb_gpu(gpb, uo, ctx):
     for k in ("pb8", "ws", "ws_wt", "ws_wt2", "labels", "super_ucm"):
         np.testing.assert_array_equal(st[k], ref[k], err_msg=k)
     np.testing.assert_array_equal(gpb.contours2ucm(pbo, True), ucm)
+
+
+class _Hip:
+    """Device buffers through the HIP runtime the library itself uses."""
+
+    def __init__(self):
+        import ctypes
+        self.C = ctypes
+        self.lib = ctypes.CDLL("libamdhip64.so.7")
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.lib.hipFree.argtypes = [ctypes.c_void_p]
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.C.c_void_p()
+        assert self.lib.hipMalloc(self.C.byref(p), a.nbytes) == 0
+        assert self.lib.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return p
+
+    def free(self, p):
+        assert self.lib.hipFree(p) == 0
+
+
+@pytest.mark.gpu
+def test_ucm_device_paths_gpu(gpb, uo, ctx):
+    """gpb_contours2ucm_device (the per-pixel stages on the GPU) and
+    gpb_compute_ucm (gPb then UCM, all on one device image) against the host
+    consumer on the same maps, both output sizes; the host consumer against
+    the oracle on them."""
+    W, H = 120, 90
+    _, packed = synth_packed(gpb, W, H, regions=18, sigma=6.0, seed=9)
+    cb = gpb.make_codebook(packed, seed=9, iters=10)
+    out = ctx.compute(packed, cb)
+    pbo = out["gpb_orient"]
+    hip = _Hip()
+    d_pbo = hip.upload(pbo.astype(np.float32))
+    d_rgb = hip.upload(packed.astype(np.uint32))
+    d_cb = hip.upload(cb.astype(np.float32))
+    try:
+        for dbl in (True, False):
+            host = gpb.contours2ucm(pbo.astype(np.float64), dbl)
+            print(f"{W}x{H}: contours2ucm_device, double_size={dbl}", flush=True)
+            np.testing.assert_array_equal(ctx.contours2ucm_device(d_pbo.value, W, H, dbl), host)
+            print(f"{W}x{H}: compute_ucm, double_size={dbl}", flush=True)
+            np.testing.assert_array_equal(ctx.compute_ucm(W, H, d_rgb.value, d_cb.value, dbl), host)
+        np.testing.assert_array_equal(gpb.contours2ucm(pbo.astype(np.float64), True),
+                                      uo.contours2ucm(pbo.astype(np.float64), True)[0])
+    finally:
+        for p in (d_pbo, d_rgb, d_cb):
+            hip.free(p)
+

@@ -94,7 +94,7 @@ struct LanczosStats {
 // Per-kernel-class timing with HIP events on the context stream (enabled by
 // gpb_set_profiling); durations are folded in when the stream is synchronised.
 enum KernelClass {
-  kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKWarm, kKOther,
+  kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKWarm, kKUcm, kKOther,
   kNumKClasses
 };
 struct KernelTimes {

@@ -5,6 +5,9 @@
 #include "gpb_device.hpp"
 
 namespace gpb {
+namespace owt {
+struct UcmStages;
+}
 
 struct NoConvergence : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -185,5 +188,11 @@ bool spmv_tile_valid(int code);
 void launch_spb(Context& c, const Stencil& st, const float* d_evecs, const double* evals, int K, float* d_spb /* [8][N] */);
 void launch_gpb(Context& c, int W, int H, const float* d_gcue, const float* d_spb, const float* d_nmax,
                 float* d_gpb_orient /* [8][N] or null */, float* d_gmax /* [N] or null */, float* d_gall /* [8][W][H] or null */);
+
+// OWT-UCM of oriented gPb maps in HBM (d_pbo[8][height][width]): the
+// per-pixel stages on the GPU, the sequential ones on the host
+// (k_ucm.hip); ucm on the host.
+void ucm_contours2ucm_device(Context& c, const float* d_pbo, int width, int height, bool double_size, double* ucm,
+                             owt::UcmStages* stages);
 
 }  // namespace gpb

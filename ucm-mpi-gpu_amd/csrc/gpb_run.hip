@@ -770,6 +770,31 @@ int gpb_compute_device(gpb_ctx* ctx, int width, int height, const uint32_t* d_pa
   });
 }
 
+int gpb_contours2ucm_device(gpb_ctx* ctx, int width, int height, const float* d_pb_oriented, int double_size,
+                            double* ucm) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (width < 2 || height < 2 || !d_pb_oriented || !ucm) throw ArgError("null pointer or image below 2 x 2");
+    ucm_contours2ucm_device(ctx->c, d_pb_oriented, width, height, double_size != 0, ucm, nullptr);
+  });
+}
+
+int gpb_compute_ucm(gpb_ctx* ctx, int width, int height, const uint32_t* d_packed, const float* d_codebook,
+                    int double_size, double* ucm) {
+  return guarded2(ctx, [&] {
+    check_dims(width, height);
+    if (width < 2 || height < 2 || !d_packed || !d_codebook || !ucm) throw ArgError("null pointer or image below 2 x 2");
+    Context& c = ctx->c;
+    const size_t N = size_t(width) * height;
+    const EigOptions eo = ctx_eig_options(c);
+    float* d_or = c.ws<float>("io.gorient", 8 * N);
+    double ev[kMaxNvec];
+    run_pipeline(c, DeviceImage{width, height, d_packed, d_codebook}, eo, false, nullptr, nullptr, d_or, ev, nullptr);
+    for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];
+    ucm_contours2ucm_device(c, d_or, width, height, double_size != 0, ucm, nullptr);
+  });
+}
+
 int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, const float* codebook, float* gpb_max,
                 float* gpb_all, float* gpb_orient, float* mpb_nmax) {
   return guarded2(ctx, [&] {

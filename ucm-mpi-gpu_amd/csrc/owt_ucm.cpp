@@ -475,7 +475,7 @@ struct Builder {
   // angle of the part right of the sweep line, then by id).
   void find_crossings(std::vector<Crossing>& out) const {
     const size_t ne = c.es.size();
-    constexpr int kCell = 32;
+    constexpr int kCell = 8;
     const int gx = sx / kCell + 1, gy = sy / kCell + 1;
     std::vector<std::vector<int>> cells(size_t(gx) * gy);
     for (size_t e = 0; e < ne; e++) {
@@ -655,113 +655,148 @@ std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
 }
 
 // Greedy merging of the initial regions by mean boundary strength
-// (ComputeUcm, src/ucm_mean_pb.cpp:254-399): the pair of adjacent regions
-// with the weakest mean boundary (then the lower region ids) merges first;
-// the boundary elements of the absorbed (lower-id) region take the merge's
-// strength; sums of strength and length carry over to the merged region's
-// neighbours.  local: (2 tx + 1) x (2 ty + 1) column-major boundary strengths,
-// part: tx x ty column-major region ids, ucm: same layout as local.
-void merge_regions(const double* local, const int32_t* part, int tx, int ty, int totcc, double* ucm) {
-  const int txc = 2 * tx + 1, tyc = 2 * ty + 1;
+// (ComputeUcm, src/ucm_mean_pb.cpp:254-399).  The pair of adjacent regions
+// whose boundary has the smallest mean strength merges first (ties: the queue
+// entry with the smaller first, then second, region id); the merged region
+// keeps the larger id, and its sums of strength and length with each
+// neighbour are the sums of the two it came from.  After every merge the
+// reference pushes a queue entry for every neighbour of the merged region,
+// stale or not; this keeps that discipline, so the merge order is the
+// reference's, ties included.
+//
+// The reference stores each region's boundary elements and, at every merge,
+// raises those of the absorbed region to the merge's strength.  Merge
+// strengths never decrease (a merged boundary's mean lies between the two
+// means it combines, both at least the current minimum; the reference treats
+// a decrease as an error), so an element ends at the strength of the merge
+// that joins its two sides.  Here each pair of adjacent regions keeps the
+// list of elements between them, and those take the strength when the pair
+// merges.  Labels are a union-find over region ids.
+//
+// local: (2 rows + 1) x (2 cols + 1) row-major boundary strengths; part: rows
+// x cols row-major region ids in [0, totcc); ucm: like local.
+void merge_regions(const double* local, const int32_t* part, int rows, int cols, int totcc, double* ucm) {
+  const int txc = 2 * rows + 1, tyc = 2 * cols + 1;
   std::fill(ucm, ucm + size_t(txc) * tyc, 0.0);
-  struct Bd {
-    int coord, neigh;
+  // boundary elements: (pair, coord) with coord the reference's column-major
+  // doubled-grid index; the sums run over each pair's elements in coord order
+  struct Elem {
+    uint64_t pair;
+    int32_t coord;
   };
-  struct Nb {
-    double energy, total, length;
-  };
-  const size_t nr = static_cast<size_t>(totcc);
-  std::vector<std::vector<Bd>> bdry(nr);
-  std::vector<std::vector<int>> elems(nr);
-  std::vector<std::map<int, Nb>> nbrs(nr);
-  std::vector<int> label(nr);
-  for (int k = 0; k < totcc; k++) {
-    label[size_t(k)] = k;
-    elems[size_t(k)].push_back(k);
-  }
-  const int vx[4] = {1, 0, -1, 0}, vy[4] = {0, 1, 0, -1};
-  for (int p = 0; p < tx * ty; p++) {
-    const int xp = p % tx, yp = p / tx;
-    for (int v = 0; v < 4; v++) {
-      const int nx = xp + vx[v], ny = yp + vy[v];
-      if (ny < 0 || ny >= ty || nx >= tx || nx < 0) continue;
-      const int q = nx + ny * tx;
-      if (part[q] != part[p]) bdry[size_t(part[p])].push_back({(xp + nx + 1) + (yp + ny + 1) * txc, part[q]});
+  std::vector<Elem> el;
+  el.reserve(size_t(rows) * cols / 2);
+  for (int x = 0; x < rows; x++)
+    for (int y = 0; y < cols; y++) {
+      const int32_t a = part[size_t(x) * cols + y];
+      if (x + 1 < rows) {
+        const int32_t b = part[size_t(x + 1) * cols + y];
+        if (a != b)
+          el.push_back({(uint64_t(std::min(a, b)) << 32) | uint32_t(std::max(a, b)), (2 * x + 2) + (2 * y + 1) * txc});
+      }
+      if (y + 1 < cols) {
+        const int32_t b = part[size_t(x) * cols + y + 1];
+        if (a != b)
+          el.push_back({(uint64_t(std::min(a, b)) << 32) | uint32_t(std::max(a, b)), (2 * x + 1) + (2 * y + 2) * txc});
+      }
     }
-  }
+  std::sort(el.begin(), el.end(), [](const Elem& p, const Elem& q) {
+    return p.pair < q.pair || (p.pair == q.pair && p.coord < q.coord);
+  });
+  struct Pair {
+    double energy, total, length;
+    std::vector<std::pair<int32_t, int32_t>> runs;  // [begin, end) ranges of el
+  };
+  std::vector<Pair> pairs;
+  std::vector<std::unordered_map<int32_t, int32_t>> nb(static_cast<size_t>(totcc));
   struct Node {
     double energy;
     int r1, r2;
   };
-  // the top is the smallest (energy, r1, r2)
   auto later = [](const Node& a, const Node& b) {
     return a.energy > b.energy || (a.energy == b.energy && a.r1 > b.r1) ||
            (a.energy == b.energy && a.r1 == b.r1 && a.r2 > b.r2);
   };
-  std::priority_queue<Node, std::vector<Node>, decltype(later)> queue(later);
-  for (int r = 0; r < totcc; r++) {
-    auto& b = bdry[size_t(r)];
-    if (b.empty()) continue;
-    std::sort(b.begin(), b.end(), [](const Bd& a, const Bd& c) {
-      return a.neigh < c.neigh || (a.neigh == c.neigh && a.coord < c.coord);
-    });
-    size_t i = 0;
-    while (i < b.size()) {
-      const int n = b[i].neigh;
-      double len = 0, tot = 0;
-      for (; i < b.size() && b[i].neigh == n; i++) {
-        len++;
-        tot += local[b[i].coord];
-      }
-      nbrs[size_t(r)][n] = Nb{tot / len, tot, len};
-      if (n > r) queue.push(Node{tot / len, r, n});
+  std::vector<Node> heap_store;
+  heap_store.reserve(el.size() + size_t(totcc) * 4);
+  std::priority_queue<Node, std::vector<Node>, decltype(later)> queue(later, std::move(heap_store));
+  for (size_t i = 0; i < el.size();) {
+    const uint64_t key = el[i].pair;
+    const size_t i0 = i;
+    double len = 0, tot = 0;
+    for (; i < el.size() && el[i].pair == key; i++) {
+      len++;
+      const int32_t c = el[i].coord;
+      tot += local[size_t(c % txc) * tyc + c / txc];
     }
+    const int32_t a = int32_t(key >> 32), b = int32_t(key & 0xffffffffu);
+    const int32_t id = int32_t(pairs.size());
+    pairs.push_back(Pair{tot / len, tot, len, {{int32_t(i0), int32_t(i)}}});
+    nb[size_t(a)].emplace(b, id);
+    nb[size_t(b)].emplace(a, id);
+    queue.push(Node{tot / len, a, b});
   }
-  double current = 0.0;
+  std::vector<int32_t> parent(static_cast<size_t>(totcc));
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int32_t r) {
+    while (parent[size_t(r)] != r) {
+      parent[size_t(r)] = parent[size_t(parent[size_t(r)])];
+      r = parent[size_t(r)];
+    }
+    return r;
+  };
   while (!queue.empty()) {
     const Node top = queue.top();
     queue.pop();
-    if (label[size_t(top.r1)] != top.r1 || label[size_t(top.r2)] != top.r2) continue;
-    auto f12 = nbrs[size_t(top.r1)].find(top.r2);
-    if (f12 == nbrs[size_t(top.r1)].end() || top.energy != f12->second.energy) continue;
-    if (current <= top.energy) current = top.energy;
-    const double saliency = f12->second.total / f12->second.length;
-    const int son = std::min(top.r1, top.r2), father = std::max(top.r1, top.r2);
-    // boundary: the father's elements facing the son go; the son's take the
-    // saliency and those facing a third region move to the father
-    auto& fb = bdry[size_t(father)];
-    fb.erase(std::remove_if(fb.begin(), fb.end(), [&](const Bd& e) { return label[size_t(e.neigh)] == son; }),
-             fb.end());
-    for (const Bd& e : bdry[size_t(son)]) {
-      if (ucm[e.coord] < saliency) ucm[e.coord] = saliency;
-      if (label[size_t(e.neigh)] != father) fb.push_back(e);
+    if (find(top.r1) != top.r1 || find(top.r2) != top.r2) continue;
+    auto f12 = nb[size_t(top.r1)].find(top.r2);
+    if (f12 == nb[size_t(top.r1)].end() || pairs[size_t(f12->second)].energy != top.energy) continue;
+    Pair& joined = pairs[size_t(f12->second)];
+    const double saliency = joined.total / joined.length;
+    for (const auto& r : joined.runs)
+      for (int32_t k = r.first; k < r.second; k++) {
+        const int32_t c = el[size_t(k)].coord;
+        double& u = ucm[size_t(c % txc) * tyc + c / txc];
+        if (u < saliency) u = saliency;
+      }
+    joined.runs.clear();
+    joined.runs.shrink_to_fit();
+    const int32_t son = std::min(top.r1, top.r2), father = std::max(top.r1, top.r2);
+    parent[size_t(son)] = father;
+    auto& fmap = nb[size_t(father)];
+    auto& smap = nb[size_t(son)];
+    fmap.erase(son);
+    for (auto it = fmap.begin(); it != fmap.end();) it = (find(it->first) != it->first) ? fmap.erase(it) : std::next(it);
+    for (const auto& kv : smap) {
+      const int32_t n = kv.first;
+      if (n == father || find(n) != n) continue;
+      Pair& q = pairs[size_t(kv.second)];
+      auto& nmap = nb[size_t(n)];
+      nmap.erase(son);
+      auto fit = fmap.find(n);
+      if (fit == fmap.end()) {
+        // a new neighbour of the father: 0 + the son's sums
+        q.total = 0.0 + q.total;
+        q.length = 0.0 + q.length;
+        fmap.emplace(n, kv.second);
+        nmap[father] = kv.second;
+      } else {
+        Pair& f = pairs[size_t(fit->second)];
+        f.total += q.total;
+        f.length += q.length;
+        if (f.runs.size() < q.runs.size()) f.runs.swap(q.runs);
+        f.runs.insert(f.runs.end(), q.runs.begin(), q.runs.end());
+        q.runs.clear();
+        q.runs.shrink_to_fit();
+      }
     }
-    bdry[size_t(son)].clear();
-    bdry[size_t(son)].shrink_to_fit();
-    for (int e : elems[size_t(son)]) label[size_t(e)] = father;
-    auto& fe = elems[size_t(father)];
-    fe.insert(fe.begin(), elems[size_t(son)].begin(), elems[size_t(son)].end());
-    elems[size_t(son)].clear();
-    elems[size_t(son)].shrink_to_fit();
-    auto& fn = nbrs[size_t(father)];
-    for (auto it = fn.begin(); it != fn.end();) it = (label[size_t(it->first)] != it->first) ? fn.erase(it) : ++it;
-    auto& sn = nbrs[size_t(son)];
-    for (auto it = sn.begin(); it != sn.end();)
-      it = (label[size_t(it->first)] != it->first || label[size_t(it->first)] == father) ? sn.erase(it) : ++it;
-    for (const auto& kv : sn) {
-      Nb& a = fn[kv.first];
-      Nb& b = nbrs[size_t(kv.first)][father];
-      a.total += kv.second.total;
-      b.total += kv.second.total;
-      a.length += kv.second.length;
-      b.length += kv.second.length;
-    }
-    sn.clear();
-    for (auto& kv : fn) {
-      const double d = kv.second.total / kv.second.length;
+    smap.clear();
+    std::unordered_map<int32_t, int32_t>().swap(smap);
+    for (const auto& kv : fmap) {
+      Pair& q = pairs[size_t(kv.second)];
+      const double d = q.total / q.length;
       queue.push(Node{d, kv.first, father});
-      kv.second.energy = d;
-      nbrs[size_t(kv.first)][father].energy = d;
+      q.energy = d;
     }
   }
   // complete the map on the even (pixel-corner) sites by the max of the
@@ -769,11 +804,11 @@ void merge_regions(const double* local, const int32_t* part, int tx, int ty, int
   for (int x = 0; x < txc; x += 2)
     for (int y = 0; y < tyc; y += 2) {
       double m = 0.0;
-      for (int v = 0; v < 4; v++) {
-        const int nx = x + vx[v], ny = y + vy[v];
-        if (ny >= 0 && ny < tyc && nx < txc && nx >= 0 && m < ucm[nx + ny * txc]) m = ucm[nx + ny * txc];
-      }
-      ucm[x + y * txc] = m;
+      if (x + 1 < txc && m < ucm[size_t(x + 1) * tyc + y]) m = ucm[size_t(x + 1) * tyc + y];
+      if (y + 1 < tyc && m < ucm[size_t(x) * tyc + y + 1]) m = ucm[size_t(x) * tyc + y + 1];
+      if (x >= 1 && m < ucm[size_t(x - 1) * tyc + y]) m = ucm[size_t(x - 1) * tyc + y];
+      if (y >= 1 && m < ucm[size_t(x) * tyc + y - 1]) m = ucm[size_t(x) * tyc + y - 1];
+      ucm[size_t(x) * tyc + y] = m;
     }
 }
 
@@ -818,11 +853,9 @@ Contours fit_contours(const uint8_t* bw, int rows, int cols) {
   return std::move(b.c);
 }
 
-void contours2ucm(const double* pbo, int rows, int cols, bool double_size, double* ucm, UcmStages* stages) {
+void pb8_from_maps(const double* pbo, int rows, int cols, UcmStages& S) {
+  // CreateFinestPartition, src/contours2ucm.cpp:255-278
   const size_t N = size_t(rows) * cols;
-  UcmStages local_stages;
-  UcmStages& S = stages ? *stages : local_stages;
-  // ---- finest partition (CreateFinestPartition, src/contours2ucm.cpp:251-404)
   std::vector<double> pb(pbo, pbo + N);
   for (int k = 1; k < 8; k++)
     for (size_t i = 0; i < N; i++)
@@ -840,12 +873,17 @@ void contours2ucm(const double* pbo, int rows, int cols, bool double_size, doubl
     const double r = std::nearbyint(pb[i] * alpha + beta);
     S.pb8[i] = !(r >= 0) ? uint8_t(0) : (r > 255 ? uint8_t(255) : uint8_t(r));
   }
+}
+
+void finest_partition(int rows, int cols, UcmStages& S, std::vector<uint8_t>& omask) {
+  // CreateFinestPartition, src/contours2ucm.cpp:280-404
+  const size_t N = size_t(rows) * cols;
   S.ws = watershed8(S.pb8.data(), rows, cols);
   std::vector<uint8_t> bw(N);
   for (size_t i = 0; i < N; i++) bw[i] = S.ws[i] == 0;
   S.contours = fit_contours(bw.data(), rows, cols);
   const Contours& ct = S.contours;
-  S.ws_wt.assign(N, 0.0);
+  omask.assign(N, 0);
   for (size_t e = 0; e < ct.es.size(); e++) {
     const int v1 = ct.es[e], v2 = ct.ee[e];
     const int r1 = ct.vx[size_t(v1)], c1 = ct.vy[size_t(v1)], r2 = ct.vx[size_t(v2)], c2 = ct.vy[size_t(v2)];
@@ -870,15 +908,38 @@ void contours2ucm(const double* pbo, int rows, int cols, bool double_size, doubl
       o = 6;
     else
       o = 7;
-    const double* pbk = pbo + size_t(o) * N;
-    auto put = [&](int r, int c) {
-      const size_t i = size_t(r) * cols + c;
-      S.ws_wt[i] = std::max(pbk[i], S.ws_wt[i]);
-    };
-    for (size_t p = 0; p < ct.px[e].size(); p++) put(ct.px[e][p], ct.py[e][p]);
-    put(r1, c1);
-    put(r2, c2);
+    const uint8_t bit = uint8_t(1u << o);
+    for (size_t p = 0; p < ct.px[e].size(); p++) omask[size_t(ct.px[e][p]) * cols + ct.py[e][p]] |= bit;
+    omask[size_t(r1) * cols + c1] |= bit;
+    omask[size_t(r2) * cols + c2] |= bit;
   }
+}
+
+void gather_strength(const double* pbo, const uint8_t* omask, size_t n, std::vector<double>& ws_wt) {
+  // each contour pixel: the max over its edges' orientation maps, from 0
+  ws_wt.assign(n, 0.0);
+  for (size_t i = 0; i < n; i++) {
+    const unsigned m = omask[i];
+    if (!m) continue;
+    double v = 0.0;
+    for (int o = 0; o < 8; o++)
+      if (m & (1u << o)) v = std::max(pbo[size_t(o) * n + i], v);
+    ws_wt[i] = v;
+  }
+}
+
+void contours2ucm(const double* pbo, int rows, int cols, bool double_size, double* ucm, UcmStages* stages) {
+  UcmStages local_stages;
+  UcmStages& S = stages ? *stages : local_stages;
+  pb8_from_maps(pbo, rows, cols, S);
+  std::vector<uint8_t> omask;
+  finest_partition(rows, cols, S, omask);
+  gather_strength(pbo, omask.data(), size_t(rows) * cols, S.ws_wt);
+  ucm_from_strength(rows, cols, double_size, ucm, S);
+}
+
+void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmStages& S) {
+  const size_t N = size_t(rows) * cols;
   // ---- the doubled grid (SuperContour4C, :24-78): pixels at even sites,
   // the min of two neighbours between them, the last row and column copied
   const int R2 = 2 * rows, C2 = 2 * cols;
@@ -968,18 +1029,9 @@ void contours2ucm(const double* pbo, int rows, int cols, bool double_size, doubl
   for (int i = 0; i < RR; i++)
     for (int j = 0; j < CC; j++)
       S.ws_wt2[size_t(i) * CC + j] = pb2[size_t(i == R2 ? R2 - 1 : i) * C2 + (j == C2 ? C2 - 1 : j)];
-  // ---- merging (UcmMeanPb, src/ucm_mean_pb.cpp:402-449): column-major
-  std::vector<double> local(size_t(RR) * CC);
-  for (int i = 0; i < RR; i++)
-    for (int j = 0; j < CC; j++) local[size_t(i) + size_t(j) * RR] = S.ws_wt2[size_t(i) * CC + j];
-  std::vector<int32_t> part(N);
-  for (int i = 0; i < rows; i++)
-    for (int j = 0; j < cols; j++) part[size_t(i) + size_t(j) * rows] = S.labels[size_t(i) * cols + j];
-  std::vector<double> u(size_t(RR) * CC);
-  merge_regions(local.data(), part.data(), rows, cols, totcc, u.data());
+  // ---- merging (UcmMeanPb, src/ucm_mean_pb.cpp:402-449)
   S.super_ucm.resize(size_t(RR) * CC);
-  for (int i = 0; i < RR; i++)
-    for (int j = 0; j < CC; j++) S.super_ucm[size_t(i) * CC + j] = u[size_t(i) + size_t(j) * RR];
+  merge_regions(S.ws_wt2.data(), S.labels.data(), rows, cols, totcc, S.super_ucm.data());
   // ---- calibration (NormalizeOutput, :191-249) and the output size
   auto calib = [](double v) {
     const double comp = 1 * -2.7487 + v * 11.1189;

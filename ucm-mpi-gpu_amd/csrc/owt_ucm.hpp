@@ -6,6 +6,7 @@
 // CPU in the reference's order, with data structures of its own.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -52,6 +53,20 @@ struct UcmStages {
 // given, receives the intermediate results.
 void contours2ucm(const double* pb_oriented, int rows, int cols, bool double_size, double* ucm,
                   UcmStages* stages = nullptr);
+
+// The same consumer in three parts, so that the per-pixel parts can run
+// where the maps are (the GPU path's device buffers):
+//   1. pb8: the maximum over orientations scaled to 8 bits (pb8_from_maps on
+//      the host);
+//   2. finest_partition: watershed and contours of pb8 (host), and omask[N],
+//      bit o set where a contour pixel belongs to an edge whose chord has
+//      orientation o;
+//   3. ws_wt[i] = max(0, pb_oriented[o][i] over the bits of omask[i])
+//      (gather_strength on the host), then ucm_from_strength.
+void pb8_from_maps(const double* pb_oriented, int rows, int cols, UcmStages& S);
+void finest_partition(int rows, int cols, UcmStages& S, std::vector<uint8_t>& omask);
+void gather_strength(const double* pb_oriented, const uint8_t* omask, size_t n, std::vector<double>& ws_wt);
+void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmStages& S);
 
 }  // namespace owt
 }  // namespace gpb

@@ -137,7 +137,7 @@ def warm_layers() -> list:
 # orientation (3 - s) mod 8; damascene/src/damascene.cu:598-603,
 # spectralPb/spectralPb.cu:101,706-712).
 GPB_ALL_BLOCK_ORIENTATION = (7, 6, 5, 4, 3, 2, 1, 0)
-KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb", "warm"]
+KERNEL_CLASSES = ["texton", "cues", "nonmax", "affinity", "spmv", "dots", "update", "ritz", "spb", "gpb", "warm", "ucm"]
 
 
 _sig("gpb_frame_size", C.c_size_t, [C.c_int, C.c_void_p])
@@ -212,6 +212,8 @@ def device_count() -> int:
 _sig("gpb_contours2ucm", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
 _sig("gpb_ucm_stages", C.c_int, [C.c_int, C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p])
 _sig("gpb_watershed8", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p])
+_sig("gpb_contours2ucm_device", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
+_sig("gpb_compute_ucm", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p])
 _sig("gpb_fit_contours", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p])
 
@@ -526,6 +528,18 @@ class Context:
                        d_orient: int = 0):
         self._chk(lib.gpb_compute_device(self.h, width, height, d_packed, d_codebook, d_gmax or None, d_gall or None,
                                          d_orient or None))
+
+    def contours2ucm_device(self, d_pb_oriented: int, width: int, height: int, double_size: bool = True) -> np.ndarray:
+        """OWT-UCM of oriented FP32 maps in device memory (pointer to [8, H, W]); the ucm on the host."""
+        out = np.empty((2 * height + 1, 2 * width + 1) if double_size else (height + 1, width + 1))
+        self._chk(lib.gpb_contours2ucm_device(self.h, width, height, d_pb_oriented, int(double_size), out.ctypes.data))
+        return out
+
+    def compute_ucm(self, width: int, height: int, d_packed: int, d_codebook: int, double_size: bool = True):
+        """gPb of a device image, then OWT-UCM of its oriented maps; the ucm on the host."""
+        out = np.empty((2 * height + 1, 2 * width + 1) if double_size else (height + 1, width + 1))
+        self._chk(lib.gpb_compute_ucm(self.h, width, height, d_packed, d_codebook, int(double_size), out.ctypes.data))
+        return out
 
     def last_eigenvectors(self, width: int, height: int, nvec: int = 9, with_degree: bool = True):
         """Eigenvectors [nvec, H, W] and degree [H, W] of the last compute / compute_device call."""
