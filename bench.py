@@ -302,6 +302,14 @@ def bench_ucm(gpb, ctx, S: int, d_rgb, d_cb, gpb_ms: float):
     out["cpu_sample"] = dict(sample=f"{crop}x{crop} crop of the image's oriented gPb maps", oracle_seconds=round(t_orc, 3),
                              host_consumer_seconds=round(t_host, 4), kind="port",
                              oracle_mpixel_s=round(crop * crop / t_orc / 1e6, 5))
+    # the host stages of the consumer on the whole image's maps
+    st = gpb.ucm_stages(orient)
+    out["host_stage_ms"] = st["stage_ms"]
+    out["regions"] = st["regions"]
+    dump = os.environ.get("GPB_UCM_DUMP_MAPS")
+    if dump:  # a 1024 x 1024 crop of the oriented maps, for profiling the host stages elsewhere
+        q0 = S // 2 - 512
+        np.ascontiguousarray(orient[:, q0:q0 + 1024, q0:q0 + 1024], np.float32).tofile(dump)
     return out
 
 

@@ -499,10 +499,13 @@ int gpb_compute_ucm(gpb_ctx* ctx, int width, int height, const uint32_t* d_packe
  * [(2H+1)(2W+1)] (the cleaned doubled grid, the map before calibration),
  * labels[N] (initial regions), *regions, and *contours_failed (1 when the
  * contour subdivision could not be completed and the partition has no
- * contours, as the reference's exception handler leaves it).  No counterpart
- * beyond the functions gpb_contours2ucm replaces. */
+ * contours, as the reference's exception handler leaves it), and stage_ms[7]
+ * (host milliseconds: 8-bit map, watershed, contours, strength, doubled grid
+ * with cleaning and regions, merging, calibration).  No counterpart beyond
+ * the functions gpb_contours2ucm replaces. */
 int gpb_ucm_stages(int width, int height, const double* pb_oriented, uint8_t* pb8, int32_t* ws, double* ws_wt,
-                   double* ws_wt2, int32_t* labels, double* super_ucm, int* regions, int* contours_failed);
+                   double* ws_wt2, int32_t* labels, double* super_ucm, int* regions, int* contours_failed,
+                   double* stage_ms);
 
 /* 8-connected watershed of an 8-bit image[H][W]: levels in increasing order,
  * each level's pixels in raster order; labels[N] > 0 for basins (numbered in

@@ -8,6 +8,7 @@
 #include "owt_ucm.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -19,6 +20,10 @@
 namespace gpb {
 namespace owt {
 namespace {
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 
 // ---------------------------------------------------------------------------
 // Contour geometry
@@ -161,7 +166,8 @@ struct Builder {
           if (a != b) parent[size_t(std::max(a, b))] = std::min(a, b);
         }
       }
-    std::unordered_map<int32_t, int32_t> comp_of_root;
+    // unions keep the smaller root: a component's root is its first pixel
+    std::vector<int32_t> comp(N, -1);
     std::vector<size_t> last;
     std::vector<uint8_t> has_vertex;
     for (long x = 0; x < sx; x++)
@@ -169,15 +175,14 @@ struct Builder {
         const size_t i = size_t(x) * sy + y;
         if (!bw[i]) continue;
         const int32_t r = find(int32_t(i));
-        auto it = comp_of_root.find(r);
         int32_t k;
-        if (it == comp_of_root.end()) {
+        if (size_t(r) == i) {
           k = int32_t(last.size());
-          comp_of_root.emplace(r, k);
+          comp[i] = k;
           last.push_back(i);
           has_vertex.push_back(0);
         } else {
-          k = it->second;
+          k = comp[size_t(r)];
           last[size_t(k)] = i;
         }
         const bool hn = y + 1 < sy, hs = y > 0, he = x + 1 < sx, hw = x > 0;
@@ -640,15 +645,14 @@ std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
         if (a != b) parent[size_t(std::max(a, b))] = std::min(a, b);
       }
     }
+  // unions keep the smaller root, so a component's root is its first pixel
+  // in raster order and is numbered before any other pixel of it
   std::vector<int32_t> lab(N, 0);
-  std::unordered_map<int32_t, int32_t> id;
   int32_t next = 0;
   for (size_t i = 0; i < N; i++) {
     if (!m[i]) continue;
     const int32_t r = find(int32_t(i));
-    auto it = id.find(r);
-    if (it == id.end()) it = id.emplace(r, ++next).first;
-    lab[i] = it->second;
+    lab[i] = (size_t(r) == i) ? ++next : lab[size_t(r)];
   }
   *count = next;
   return lab;
@@ -855,6 +859,7 @@ Contours fit_contours(const uint8_t* bw, int rows, int cols) {
 
 void pb8_from_maps(const double* pbo, int rows, int cols, UcmStages& S) {
   // CreateFinestPartition, src/contours2ucm.cpp:255-278
+  const auto t0 = std::chrono::steady_clock::now();
   const size_t N = size_t(rows) * cols;
   std::vector<double> pb(pbo, pbo + N);
   for (int k = 1; k < 8; k++)
@@ -869,16 +874,21 @@ void pb8_from_maps(const double* pbo, int rows, int cols, UcmStages& S) {
   const double alpha = 255 / (hi - lo);
   const double beta = -255 * lo / (hi - lo);
   S.pb8.resize(N);
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < N; i++) {
     const double r = std::nearbyint(pb[i] * alpha + beta);
     S.pb8[i] = !(r >= 0) ? uint8_t(0) : (r > 255 ? uint8_t(255) : uint8_t(r));
   }
+  S.ms[0] = ms_since(t0);
 }
 
 void finest_partition(int rows, int cols, UcmStages& S, std::vector<uint8_t>& omask) {
   // CreateFinestPartition, src/contours2ucm.cpp:280-404
   const size_t N = size_t(rows) * cols;
+  auto t0 = std::chrono::steady_clock::now();
   S.ws = watershed8(S.pb8.data(), rows, cols);
+  S.ms[1] = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
   std::vector<uint8_t> bw(N);
   for (size_t i = 0; i < N; i++) bw[i] = S.ws[i] == 0;
   S.contours = fit_contours(bw.data(), rows, cols);
@@ -913,11 +923,13 @@ void finest_partition(int rows, int cols, UcmStages& S, std::vector<uint8_t>& om
     omask[size_t(r1) * cols + c1] |= bit;
     omask[size_t(r2) * cols + c2] |= bit;
   }
+  S.ms[2] = ms_since(t0);
 }
 
 void gather_strength(const double* pbo, const uint8_t* omask, size_t n, std::vector<double>& ws_wt) {
   // each contour pixel: the max over its edges' orientation maps, from 0
   ws_wt.assign(n, 0.0);
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; i++) {
     const unsigned m = omask[i];
     if (!m) continue;
@@ -934,24 +946,34 @@ void contours2ucm(const double* pbo, int rows, int cols, bool double_size, doubl
   pb8_from_maps(pbo, rows, cols, S);
   std::vector<uint8_t> omask;
   finest_partition(rows, cols, S, omask);
+  const auto t0 = std::chrono::steady_clock::now();
   gather_strength(pbo, omask.data(), size_t(rows) * cols, S.ws_wt);
+  S.ms[3] = ms_since(t0);
   ucm_from_strength(rows, cols, double_size, ucm, S);
 }
 
 void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmStages& S) {
   const size_t N = size_t(rows) * cols;
+  auto t0 = std::chrono::steady_clock::now();
   // ---- the doubled grid (SuperContour4C, :24-78): pixels at even sites,
   // the min of two neighbours between them, the last row and column copied
   const int R2 = 2 * rows, C2 = 2 * cols;
-  std::vector<double> pb2(size_t(R2) * C2, 0.0);
+  std::vector<double> pb2(size_t(R2) * C2);
   const auto& w = S.ws_wt;
-  for (int i = 0; i < R2; i += 2)
-    for (int j = 0; j < C2; j += 2) {
-      const size_t s = size_t(i / 2) * cols + j / 2;
-      pb2[size_t(i) * C2 + j] = w[s];
-      if (j != C2 - 2) pb2[size_t(i) * C2 + j + 1] = std::min(w[s], w[s + 1]);
-      if (i != R2 - 2) pb2[size_t(i + 1) * C2 + j] = std::min(w[s], w[s + size_t(cols)]);
+  // per pixel row: its even row (pixel, horizontal min) and odd row
+  // (vertical min, zero); every element is written once
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < rows; r++) {
+    double* ev = pb2.data() + size_t(2 * r) * C2;
+    double* od = ev + C2;
+    const double* wr = w.data() + size_t(r) * cols;
+    for (int c = 0; c < cols; c++) {
+      ev[2 * c] = wr[c];
+      ev[2 * c + 1] = (c != cols - 1) ? std::min(wr[c], wr[c + 1]) : 0.0;
+      od[2 * c] = (r != rows - 1) ? std::min(wr[c], wr[c + size_t(cols)]) : 0.0;
+      od[2 * c + 1] = 0.0;
     }
+  }
   for (int j = 0; j < C2; j++) pb2[size_t(R2 - 1) * C2 + j] = pb2[size_t(R2 - 2) * C2 + j];
   for (int i = 0; i < R2; i++)
     pb2[size_t(i) * C2 + C2 - 1] = std::max(pb2[size_t(i) * C2 + C2 - 1], pb2[size_t(i) * C2 + C2 - 2]);
@@ -965,7 +987,9 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
     return f < 0 ? 0.0 : pb2[size_t(f)];
   };
   auto ref = [&](long i, long j) -> double& { return pb2[size_t(i * C2 + j)]; };
-  std::vector<long> art;
+  // isolated zeros, found per row in parallel, then in raster order
+  std::vector<std::vector<long>> art_rows(static_cast<size_t>(R2));
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < R2; i++)
     for (long j = 0; j < C2; j++) {
       if (pb2[size_t(i) * C2 + j] != 0) continue;
@@ -978,8 +1002,10 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
             break;
           }
         }
-      if (isolated) art.push_back(i * C2 + j);
+      if (isolated) art_rows[size_t(i)].push_back(i * C2 + j);
     }
+  std::vector<long> art;
+  for (const auto& r : art_rows) art.insert(art.end(), r.begin(), r.end());
   for (long a : art) {
     const long xc = a / C2, yc = a % C2;
     double vec[4];
@@ -1008,7 +1034,8 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
   // ---- regions: 8-connected zeros of the cleaned grid, sampled at the odd
   // sites (one per pixel), numbered from 0
   std::vector<uint8_t> zero(size_t(R2) * C2);
-  for (size_t i = 0; i < zero.size(); i++) zero[i] = pb2[i] == 0;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < long(zero.size()); i++) zero[size_t(i)] = pb2[size_t(i)] == 0;
   int nlab = 0;
   const std::vector<int32_t> lab2 = label8(zero.data(), R2, C2, &nlab);
   S.labels.resize(N);
@@ -1026,12 +1053,17 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
   // (2 rows + 1) x (2 cols + 1) with the last row and column repeated
   const int RR = R2 + 1, CC = C2 + 1;
   S.ws_wt2.resize(size_t(RR) * CC);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < RR; i++)
     for (int j = 0; j < CC; j++)
       S.ws_wt2[size_t(i) * CC + j] = pb2[size_t(i == R2 ? R2 - 1 : i) * C2 + (j == C2 ? C2 - 1 : j)];
+  S.ms[4] = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
   // ---- merging (UcmMeanPb, src/ucm_mean_pb.cpp:402-449)
   S.super_ucm.resize(size_t(RR) * CC);
   merge_regions(S.ws_wt2.data(), S.labels.data(), rows, cols, totcc, S.super_ucm.data());
+  S.ms[5] = ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
   // ---- calibration (NormalizeOutput, :191-249) and the output size
   auto calib = [](double v) {
     const double comp = 1 * -2.7487 + v * 11.1189;
@@ -1040,13 +1072,16 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
     return std::min(1.0, std::max(0.0, p));
   };
   if (double_size) {
-    for (size_t i = 0; i < size_t(RR) * CC; i++) ucm[i] = calib(S.super_ucm[i]);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(RR) * CC; i++) ucm[i] = calib(S.super_ucm[size_t(i)]);
   } else {
     const int ro = (RR + 1) / 2, co = (CC + 1) / 2;
     std::fill(ucm, ucm + size_t(ro) * co, 0.0);
+#pragma omp parallel for schedule(static)
     for (int i = 2; i < RR; i += 2)
       for (int j = 2; j < CC; j += 2) ucm[size_t((i - 2) / 2) * co + (j - 2) / 2] = calib(S.super_ucm[size_t(i) * CC + j]);
   }
+  S.ms[6] = ms_since(t0);
 }
 
 }  // namespace owt

@@ -46,6 +46,9 @@ struct UcmStages {
   std::vector<double> super_ucm; // (2 rows + 1) x (2 cols + 1), before calibration
   Contours contours;
   int regions = 0;
+  // host milliseconds: 8-bit map, watershed, contours, strength, doubled grid
+  // with cleaning and regions, merging, calibration
+  double ms[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 // The whole consumer: pb_oriented[8][rows][cols] -> ucm, (2 rows + 1) x

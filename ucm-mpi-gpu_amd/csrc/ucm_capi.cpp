@@ -38,7 +38,8 @@ int gpb_contours2ucm(int width, int height, const double* pb_oriented, int doubl
 }
 
 int gpb_ucm_stages(int width, int height, const double* pb_oriented, uint8_t* pb8, int32_t* ws, double* ws_wt,
-                   double* ws_wt2, int32_t* labels, double* super_ucm, int* regions, int* contours_failed) {
+                   double* ws_wt2, int32_t* labels, double* super_ucm, int* regions, int* contours_failed,
+                   double* stage_ms) {
   if (!ucm_dims_ok(width, height) || !pb_oriented) return GPB_ERR_ARG;
   return guarded([&] {
     gpb::owt::UcmStages st;
@@ -53,6 +54,8 @@ int gpb_ucm_stages(int width, int height, const double* pb_oriented, uint8_t* pb
     if (super_ucm) std::memcpy(super_ucm, st.super_ucm.data(), N2 * sizeof(double));
     if (regions) *regions = st.regions;
     if (contours_failed) *contours_failed = st.contours.failed ? 1 : 0;
+    if (stage_ms)
+      for (int k = 0; k < 7; k++) stage_ms[k] = st.ms[k];
     return GPB_OK;
   });
 }

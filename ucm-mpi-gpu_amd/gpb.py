@@ -210,7 +210,7 @@ def device_count() -> int:
 
 # ---- OWT-UCM (host code, no context) ----
 _sig("gpb_contours2ucm", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
-_sig("gpb_ucm_stages", C.c_int, [C.c_int, C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_void_p, C.c_void_p])
+_sig("gpb_ucm_stages", C.c_int, [C.c_int, C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_void_p] * 3)
 _sig("gpb_watershed8", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p])
 _sig("gpb_contours2ucm_device", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
 _sig("gpb_compute_ucm", C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p])
@@ -235,10 +235,13 @@ def ucm_stages(pb_oriented: np.ndarray) -> dict:
              ws_wt2=np.empty((2 * H + 1, 2 * W + 1)), labels=np.empty((H, W), np.int32),
              super_ucm=np.empty((2 * H + 1, 2 * W + 1)))
     reg, fail = C.c_int(), C.c_int()
+    ms = np.zeros(7)
     _check(lib.gpb_ucm_stages(W, H, pbo.ctypes.data, d["pb8"].ctypes.data, d["ws"].ctypes.data, d["ws_wt"].ctypes.data,
                               d["ws_wt2"].ctypes.data, d["labels"].ctypes.data, d["super_ucm"].ctypes.data,
-                              C.byref(reg), C.byref(fail)), what="ucm_stages")
+                              C.byref(reg), C.byref(fail), ms.ctypes.data), what="ucm_stages")
     d["regions"], d["contours_failed"] = reg.value, bool(fail.value)
+    d["stage_ms"] = dict(zip(("pb8", "watershed", "contours", "strength", "grid_clean_regions", "merging",
+                              "calibration"), ms.round(2).tolist()))
     return d
 
 
