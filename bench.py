@@ -136,6 +136,25 @@ def measured_traffic(kernel_class: str, size: int):
     return round(c["bytes_per_launch"]) if c else None
 
 
+def _form_roofline(ctx, run):
+    """One more untimed run with HIP events around every launch: the kernel
+    class with the largest device time, its algorithmic bytes over its event
+    time, against the 8 TB/s HBM peak."""
+    import torch
+
+    ctx.set_profiling(True)
+    run()
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    dom = max(kt, key=lambda k: kt[k]["ms"])
+    d = kt[dom]
+    total = sum(v["ms"] for v in kt.values())
+    gbs = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
+    return dict(kernel=dom, bound="hbm", achieved=round(gbs, 1), peak=8000.0, unit="GB/s", frac=round(gbs / 8000.0, 4),
+                device_ms=round(d["ms"], 2), launches=d["launches"], share_of_device_time=round(d["ms"] / total, 3))
+
+
 def bench_patches(gpb, ctx, tol: float, size: int = 5000, regions: int = 2500, grid: int = 4, overlap: int = 16,
                   seed: int = 4):
     """The batched form: a 5000x5000 seeded Voronoi image (2500 regions, sigma 8,
@@ -172,6 +191,8 @@ def bench_patches(gpb, ctx, tol: float, size: int = 5000, regions: int = 2500, g
                          matvecs_total=int(sum(x["matvecs"] for x in st)),
                          max_residual=max(x["max_residual"] for x in st),
                          stage_ms=[round(x, 1) for x in st[0]["stage_ms"]])
+        if mode == 1:
+            out[name]["roofline"] = _form_roofline(ctx, run)
     out["speedup_batched_vs_one_by_one"] = round(out["one_by_one"]["ms"] / out["batched"]["ms"], 3)
     out["workload"] = (f"{size}x{size} seeded Voronoi ({regions} regions, sigma 8, seed {seed}), {grid}x{grid} patches "
                        f"{int(rects[:, 2].min())}..{int(rects[:, 2].max())} x {int(rects[:, 3].min())}.."
@@ -207,6 +228,8 @@ def bench_warm(gpb, ctx, step, S: int, steps: int, tol: float):
         out[name] = dict(value=round(S * S * steps / dt / 1e6, 4), unit="Mpixel/s", ms=round(dt * 1e3 / steps, 1),
                          matvecs=st["matvecs"], fallback=st["fallback"], max_residual=st["max_residual"],
                          network_ms=round(st["warm_ms"], 2), stage_ms=[round(x, 1) for x in st["stage_ms"]])
+        if mode == 1:
+            out[name]["roofline"] = _form_roofline(ctx, step)
     ctx.set_warm_start(0)
     out["speedup_warm_vs_plain"] = round(out["plain"]["ms"] / out["warm_start"]["ms"], 3)
     out["speedup_random_block_vs_plain"] = round(out["plain"]["ms"] / out["random_block"]["ms"], 3)
