@@ -534,6 +534,110 @@ void launch_project(Context& c, const float* V, size_t ld, int n, float* w, cons
   GPB_LAUNCH_CHECK();
 }
 
+// Classical Gram-Schmidt's update followed by the next pass's projection,
+// in one pass over the basis: w <- fl32(w - sum_i h_i V_i) (the update's
+// arithmetic), then the dots V_i . w and ||w||^2 of the new w with the
+// projection's per-thread accumulation and block reduction, so the partials
+// equal those of update_gs_kernel followed by project_kernel on the same grid
+// (launch it on the projection's grid).
+template <int NV>
+__global__ __launch_bounds__(256) void update_project_kernel(const float* __restrict__ V, size_t ld, int n,
+                                                             const double* __restrict__ h, float* __restrict__ w,
+                                                             size_t Np, double* __restrict__ partial) {
+  __shared__ double sh[4][NV + 1];
+  double hv[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) hv[i] = i < n ? h[i] : 0.0;
+  double acc[NV + 1];
+#pragma unroll
+  for (int i = 0; i <= NV; i++) acc[i] = 0.0;
+  for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < Np; p += size_t(gridDim.x) * blockDim.x) {
+    const float* q = V + p;
+    float vv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) vv[i] = q[size_t(i < n ? i : 0) * ld];
+    double u = w[p];
+#pragma unroll
+    for (int i = 0; i < NV; i++) u -= hv[i] * double(vv[i]);
+    const float f = float(u);
+    w[p] = f;
+    const double wn = f;
+    acc[NV] += wn * wn;
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc[i] += double(vv[i]) * wn;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i <= NV; i++) {
+    if (i < n || i == NV) {
+      double v = acc[i];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) sh[wv][i] = v;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t <= n; t += blockDim.x) {
+    const int i = t < n ? t : NV;
+    partial[size_t(t) * gridDim.x + blockIdx.x] = sh[0][i] + sh[1][i] + sh[2][i] + sh[3][i];
+  }
+}
+
+void launch_update_project(Context& c, const float* V, size_t ld, int n, const double* h, float* w, size_t Np,
+                           double* partial, int G) {
+  if (n < 1 || n > kProjectMaxN) throw ArgError("projection width out of range");
+  const dim3 g(G), b(256);
+#define GPB_UP(NV)                                                                                               \
+  case NV / 4:                                                                                                   \
+    hipLaunchKernelGGL(update_project_kernel<NV>, g, b, 0, c.stream, V, ld, n, h, w, Np, partial);               \
+    break;
+  switch ((n + 3) / 4) {
+    GPB_UP(4)
+    GPB_UP(8)
+    GPB_UP(12)
+    GPB_UP(16)
+    GPB_UP(20)
+    GPB_UP(24)
+    GPB_UP(28)
+    GPB_UP(32)
+    GPB_UP(36)
+    GPB_UP(40)
+    GPB_UP(44)
+    GPB_UP(48)
+    GPB_UP(52)
+    GPB_UP(56)
+    GPB_UP(60)
+    GPB_UP(64)
+  }
+#undef GPB_UP
+  GPB_LAUNCH_CHECK();
+}
+
+// Weighted squared norms v_k . D v_k of the vectors V + sel_k ld (k < m) in
+// one launch: block (chunk, k) with multidot2_kernel's chunking and
+// arithmetic, so each equals that kernel's weighted norm of the same vector.
+struct NormSel {
+  int idx[kMaxNvec + 2];
+};
+__global__ __launch_bounds__(256) void wnorms_kernel(const float* __restrict__ V, size_t ld, NormSel sel,
+                                                     const float* __restrict__ D, size_t Np,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[16];
+  const int k = blockIdx.y;
+  const size_t lo = size_t(blockIdx.x) * kDotChunk;
+  const float* v = V + size_t(sel.idx[k]) * ld;
+  double acc = 0;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < kDotChunk; e += 256) {
+    const size_t p = lo + e;
+    if (p < Np) {
+      const double x = v[p];
+      acc += double(D[p]) * x * x;
+    }
+  }
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[size_t(k) * gridDim.x + blockIdx.x] = r;
+}
+
 // out[i] = sum_blk partial[i][blk]  (one block per i, fixed order); gated.
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const double* __restrict__ partial, int G,
                                                              double* __restrict__ out, const int* __restrict__ gate) {
@@ -1086,6 +1190,30 @@ struct Reducer {
     for (int i0 = 0; i0 < n; i0 += kProjectMaxN)
       project(V + size_t(i0) * ld, ld, std::min(kProjectMaxN, n - i0), w, nullptr, nullptr, 0, gate, dst + i0);
   }
+  // w -= V h, then dst[0..n) = V_i . w, dst[n] = ||w||^2 of the new w (n <=
+  // kProjectMaxN): the update and the next projection of classical
+  // Gram-Schmidt in one pass, with the separate kernels' results
+  void update_project(const float* V, size_t ld, int n, const double* h, float* w, double* dst) {
+    {
+      KScope ks(c, kKUpdate, double(Np) * (4.0 * n + 8));
+      launch_update_project(c, V, ld, n, h, w, Np, partial, GP);
+    }
+    hipLaunchKernelGGL(reduce_kernel, dim3(n + 1), dim3(kRedThreads), 0, c.stream, partial, GP, dst, nullptr);
+    GPB_LAUNCH_CHECK();
+  }
+  // dst[k] = v_k . D v_k for the vectors V + sel[k] ld, k < m, in one launch
+  void wnorms(const float* V, size_t ld, const int* sel, int m, const float* D, double* dst) {
+    if (m < 1 || m > kMaxNvec + 2) throw ArgError("too many norms");
+    NormSel ns{};
+    for (int k = 0; k < m; k++) ns.idx[k] = sel[k];
+    {
+      KScope ks(c, kKDots, double(Np) * (4.0 * m + 4));
+      hipLaunchKernelGGL(wnorms_kernel, dim3(NW, m), dim3(256), 0, c.stream, V, ld, ns, D, Np, partial);
+      GPB_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(reduce_kernel, dim3(m), dim3(kRedThreads), 0, c.stream, partial, NW, dst, nullptr);
+    GPB_LAUNCH_CHECK();
+  }
   // w -= V h; nrm_dst = ||w||^2 afterwards (if non-null); gated when gate is non-null
   void update(const float* V, size_t ld, int n, const double* h, float* w, double* nrm_dst,
               const int* gate = nullptr) {
@@ -1241,13 +1369,10 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         launch_spmv(c, st, vj, w);
         if (n <= kProjectMaxN) {
           red.project(V, Np, n, w, nullptr, nullptr, j, nullptr, h1);
+          red.update_project(V, Np, n, h1, w, h2);
         } else {
           red.dots(V, Np, n, w, nullptr, false, h1);
-        }
-        red.update(V, Np, n, h1, w, nullptr);
-        if (n <= kProjectMaxN) {
-          red.project(V, Np, n, w, nullptr, nullptr, j, nullptr, h2);
-        } else {
+          red.update(V, Np, n, h1, w, nullptr);
           red.dots(V, Np, n, w, nullptr, false, h2);
         }
         red.update(V, Np, n, h2, w, nrm + 1);
@@ -1411,10 +1536,14 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     launch_ritz(c, V, Np, m, nk, Ydev, V2, Np);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(nk) * Np, V + size_t(m) * Np, Np * sizeof(float), hipMemcpyDeviceToDevice,
                            c.stream));
-    // weighted norms ||D^1/2 u_i||, ||D^1/2 v_m||
+    // weighted norms ||D^1/2 u_i||, ||D^1/2 v_m|| in one launch
     std::vector<double> nrm(K + 1);
-    for (int i = 0; i < K; i++) red.dots(V2, Np, 0, V2 + size_t(i) * Np, st.D, true, h + i);
-    red.dots(V2, Np, 0, V2 + size_t(nk) * Np, st.D, true, h + K);
+    {
+      int sel[kMaxNvec + 2];
+      for (int i = 0; i < K; i++) sel[i] = i;
+      sel[K] = nk;
+      red.wnorms(V2, Np, sel, K + 1, st.D, h);
+    }
     GPB_HIP(hipMemcpyAsync(nrm.data(), h, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(gacc_h, gacc, sizeof(gacc_h), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
@@ -1449,9 +1578,15 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       // step of the next cycle (the omega recurrence assumes it is at
       // rounding level): Gram-Schmidt it against them twice, then normalise.
       float* vk = V + size_t(keep) * Np;
-      for (int pass = 0; pass < 2; pass++) {
-        red.project_any(V, Np, keep, vk, nullptr, h1);
-        red.update(V, Np, keep, h1, vk, rnrm);
+      if (keep <= kProjectMaxN) {
+        red.project(V, Np, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+        red.update_project(V, Np, keep, h1, vk, h2);
+        red.update(V, Np, keep, h2, vk, rnrm);
+      } else {
+        for (int pass = 0; pass < 2; pass++) {
+          red.project_any(V, Np, keep, vk, nullptr, h1);
+          red.update(V, Np, keep, h1, vk, rnrm);
+        }
       }
       launch_scale_by(c, vk, rnrm, Np);
     }
