@@ -665,7 +665,9 @@ std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
 // keeps the larger id, and its sums of strength and length with each
 // neighbour are the sums of the two it came from.  After every merge the
 // reference pushes a queue entry for every neighbour of the merged region,
-// stale or not; this keeps that discipline, so the merge order is the
+// changed or not, and the region order inside an entry breaks ties; this
+// pushes the same entries except those an identical or earlier-ordered live
+// entry of the same pair already shadows, so the merge order is the
 // reference's, ties included.
 //
 // The reference stores each region's boundary elements and, at every merge,
@@ -682,37 +684,71 @@ std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
 void merge_regions(const double* local, const int32_t* part, int rows, int cols, int totcc, double* ucm) {
   const int txc = 2 * rows + 1, tyc = 2 * cols + 1;
   std::fill(ucm, ucm + size_t(txc) * tyc, 0.0);
-  // boundary elements: (pair, coord) with coord the reference's column-major
-  // doubled-grid index; the sums run over each pair's elements in coord order
-  struct Elem {
-    uint64_t pair;
-    int32_t coord;
-  };
-  std::vector<Elem> el;
-  el.reserve(size_t(rows) * cols / 2);
-  for (int x = 0; x < rows; x++)
-    for (int y = 0; y < cols; y++) {
-      const int32_t a = part[size_t(x) * cols + y];
-      if (x + 1 < rows) {
-        const int32_t b = part[size_t(x + 1) * cols + y];
-        if (a != b)
-          el.push_back({(uint64_t(std::min(a, b)) << 32) | uint32_t(std::max(a, b)), (2 * x + 2) + (2 * y + 1) * txc});
-      }
-      if (y + 1 < cols) {
-        const int32_t b = part[size_t(x) * cols + y + 1];
-        if (a != b)
-          el.push_back({(uint64_t(std::min(a, b)) << 32) | uint32_t(std::max(a, b)), (2 * x + 1) + (2 * y + 2) * txc});
-      }
-    }
-  std::sort(el.begin(), el.end(), [](const Elem& p, const Elem& q) {
-    return p.pair < q.pair || (p.pair == q.pair && p.coord < q.coord);
-  });
+  // Boundary elements, visited in increasing coordinate (the reference's
+  // column-major doubled-grid index): per pixel column y, the elements between
+  // vertically adjacent pixels (doubled column 2y + 1), then those between
+  // horizontally adjacent ones (column 2y + 2), rows increasing.  Each pair of
+  // regions' sums therefore run over its elements in coordinate order, as
+  // the reference's sorted boundary lists do.  Pairs are found through each
+  // region's short neighbour list.
   struct Pair {
     double energy, total, length;
-    std::vector<std::pair<int32_t, int32_t>> runs;  // [begin, end) ranges of el
+    std::vector<std::pair<int32_t, int32_t>> runs;  // [begin, end) ranges of coord
+    // the queue entries last pushed for this pair: their energy, the pair's
+    // region ids then, and the smallest first region id among them
+    double qe;
+    int32_t qlo, qhi, minr1;
   };
   std::vector<Pair> pairs;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> adj(static_cast<size_t>(totcc));  // (neighbour, pair)
+  std::vector<int32_t> el_pair, el_coord;
+  el_pair.reserve(size_t(rows) * cols / 2);
+  el_coord.reserve(size_t(rows) * cols / 2);
+  auto pair_of = [&](int32_t a, int32_t b) {
+    for (const auto& e : adj[size_t(a)])
+      if (e.first == b) return e.second;
+    const int32_t id = int32_t(pairs.size());
+    pairs.push_back(Pair{0.0, 0.0, 0.0, {}, 0.0, -1, -1, -1});
+    adj[size_t(a)].emplace_back(b, id);
+    adj[size_t(b)].emplace_back(a, id);
+    return id;
+  };
+  auto visit = [&](int32_t a, int32_t b, int32_t coord) {
+    const int32_t id = pair_of(std::min(a, b), std::max(a, b));
+    Pair& q = pairs[size_t(id)];
+    q.length++;
+    q.total += local[size_t(coord % txc) * tyc + coord / txc];
+    el_pair.push_back(id);
+    el_coord.push_back(coord);
+  };
+  for (int y = 0; y < cols; y++) {
+    for (int x = 0; x + 1 < rows; x++) {
+      const int32_t a = part[size_t(x) * cols + y], b = part[size_t(x + 1) * cols + y];
+      if (a != b) visit(a, b, (2 * x + 2) + (2 * y + 1) * txc);
+    }
+    if (y + 1 < cols)
+      for (int x = 0; x < rows; x++) {
+        const int32_t a = part[size_t(x) * cols + y], b = part[size_t(x) * cols + y + 1];
+        if (a != b) visit(a, b, (2 * x + 1) + (2 * y + 2) * txc);
+      }
+  }
+  // each pair's elements contiguous (a stable counting sort by pair)
+  std::vector<int32_t> coord(el_coord.size());
+  {
+    std::vector<int32_t> start(pairs.size() + 1, 0);
+    for (int32_t id : el_pair) start[size_t(id) + 1]++;
+    for (size_t k = 0; k < pairs.size(); k++) start[k + 1] += start[k];
+    for (size_t k = 0; k < pairs.size(); k++) pairs[k].runs.emplace_back(start[k], start[k + 1]);
+    for (size_t e = 0; e < el_pair.size(); e++) coord[size_t(start[size_t(el_pair[e])]++)] = el_coord[e];
+  }
+  std::vector<int32_t>().swap(el_pair);
+  std::vector<int32_t>().swap(el_coord);
   std::vector<std::unordered_map<int32_t, int32_t>> nb(static_cast<size_t>(totcc));
+  for (size_t r = 0; r < adj.size(); r++) {
+    nb[r].reserve(adj[r].size());
+    for (const auto& e : adj[r]) nb[r].emplace(e.first, e.second);
+    std::vector<std::pair<int32_t, int32_t>>().swap(adj[r]);
+  }
   struct Node {
     double energy;
     int r1, r2;
@@ -722,24 +758,31 @@ void merge_regions(const double* local, const int32_t* part, int rows, int cols,
            (a.energy == b.energy && a.r1 == b.r1 && a.r2 > b.r2);
   };
   std::vector<Node> heap_store;
-  heap_store.reserve(el.size() + size_t(totcc) * 4);
+  heap_store.reserve(coord.size() + size_t(totcc) * 4);
   std::priority_queue<Node, std::vector<Node>, decltype(later)> queue(later, std::move(heap_store));
-  for (size_t i = 0; i < el.size();) {
-    const uint64_t key = el[i].pair;
-    const size_t i0 = i;
-    double len = 0, tot = 0;
-    for (; i < el.size() && el[i].pair == key; i++) {
-      len++;
-      const int32_t c = el[i].coord;
-      tot += local[size_t(c % txc) * tyc + c / txc];
-    }
-    const int32_t a = int32_t(key >> 32), b = int32_t(key & 0xffffffffu);
-    const int32_t id = int32_t(pairs.size());
-    pairs.push_back(Pair{tot / len, tot, len, {{int32_t(i0), int32_t(i)}}});
-    nb[size_t(a)].emplace(b, id);
-    nb[size_t(b)].emplace(a, id);
-    queue.push(Node{tot / len, a, b});
+  for (size_t k = 0; k < pairs.size(); k++) {
+    Pair& q = pairs[k];
+    q.energy = q.total / q.length;
   }
+  // Pushing (d, r1, r2) for a pair that already has an entry with the same
+  // energy and regions and a first id <= r1 changes nothing: that entry
+  // pops first, and if it is stale so is this one.  Such pushes are skipped.
+  auto push = [&](Pair& q, double d, int32_t r1, int32_t r2) {
+    const int32_t lo = std::min(r1, r2), hi = std::max(r1, r2);
+    if (q.qe == d && q.qlo == lo && q.qhi == hi) {
+      if (q.minr1 <= r1) return;
+      q.minr1 = r1;
+    } else {
+      q.qe = d;
+      q.qlo = lo;
+      q.qhi = hi;
+      q.minr1 = r1;
+    }
+    queue.push(Node{d, r1, r2});
+  };
+  for (size_t r = 0; r < nb.size(); r++)
+    for (const auto& e : nb[r])
+      if (e.first > int32_t(r)) push(pairs[size_t(e.second)], pairs[size_t(e.second)].energy, int32_t(r), e.first);
   std::vector<int32_t> parent(static_cast<size_t>(totcc));
   std::iota(parent.begin(), parent.end(), 0);
   auto find = [&](int32_t r) {
@@ -759,18 +802,18 @@ void merge_regions(const double* local, const int32_t* part, int rows, int cols,
     const double saliency = joined.total / joined.length;
     for (const auto& r : joined.runs)
       for (int32_t k = r.first; k < r.second; k++) {
-        const int32_t c = el[size_t(k)].coord;
+        const int32_t c = coord[size_t(k)];
         double& u = ucm[size_t(c % txc) * tyc + c / txc];
         if (u < saliency) u = saliency;
       }
     joined.runs.clear();
-    joined.runs.shrink_to_fit();
     const int32_t son = std::min(top.r1, top.r2), father = std::max(top.r1, top.r2);
     parent[size_t(son)] = father;
     auto& fmap = nb[size_t(father)];
     auto& smap = nb[size_t(son)];
+    // every map loses an absorbed region's id when it is absorbed, so the
+    // father's map holds active regions only once the son is gone
     fmap.erase(son);
-    for (auto it = fmap.begin(); it != fmap.end();) it = (find(it->first) != it->first) ? fmap.erase(it) : std::next(it);
     for (const auto& kv : smap) {
       const int32_t n = kv.first;
       if (n == father || find(n) != n) continue;
@@ -791,15 +834,13 @@ void merge_regions(const double* local, const int32_t* part, int rows, int cols,
         if (f.runs.size() < q.runs.size()) f.runs.swap(q.runs);
         f.runs.insert(f.runs.end(), q.runs.begin(), q.runs.end());
         q.runs.clear();
-        q.runs.shrink_to_fit();
       }
     }
     smap.clear();
-    std::unordered_map<int32_t, int32_t>().swap(smap);
     for (const auto& kv : fmap) {
       Pair& q = pairs[size_t(kv.second)];
       const double d = q.total / q.length;
-      queue.push(Node{d, kv.first, father});
+      push(q, d, kv.first, father);
       q.energy = d;
     }
   }
