@@ -618,12 +618,34 @@ struct Builder {
 // Regions
 // ---------------------------------------------------------------------------
 
-// 8-connected labels of the nonzero pixels, numbered from 1 by first pixel in
-// raster order (the numbering of the breadth-first bwlabel,
-// src/MATLAB_functions.cpp:182-217).
-std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
-  const size_t N = size_t(rows) * cols;
-  std::vector<int32_t> parent(N, -1);
+// Region ids of the pixels from the cleaned doubled grid g (R2 x C2): the
+// 8-connected components of its zero sites, numbered from 0 in the order of
+// their first site in raster order (bwlabel's numbering, less one), read at
+// the odd sites (2i + 1, 2j + 1).  Components are built from runs of zeros:
+// two runs in adjacent rows touch when their column ranges, one widened by a
+// site on each side, overlap.  labels[i][j] = -1 where the site is not zero.
+// Returns the number of components.
+int region_labels(const double* g, int R2, int C2, int rows, int cols, int32_t* labels) {
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> row_runs(static_cast<size_t>(R2));
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < R2; r++) {
+    const double* gr = g + size_t(r) * C2;
+    auto& rr = row_runs[size_t(r)];
+    for (int c = 0; c < C2;) {
+      if (gr[c] != 0) {
+        c++;
+        continue;
+      }
+      const int s0 = c;
+      while (c < C2 && gr[c] == 0) c++;
+      rr.emplace_back(s0, c - 1);
+    }
+  }
+  std::vector<int32_t> first(static_cast<size_t>(R2) + 1, 0);
+  for (int r = 0; r < R2; r++) first[size_t(r) + 1] = first[size_t(r)] + int32_t(row_runs[size_t(r)].size());
+  const int32_t nrun = first[size_t(R2)];
+  std::vector<int32_t> parent(static_cast<size_t>(nrun));
+  std::iota(parent.begin(), parent.end(), 0);
   auto find = [&](int32_t a) {
     while (parent[size_t(a)] != a) {
       parent[size_t(a)] = parent[size_t(parent[size_t(a)])];
@@ -631,31 +653,38 @@ std::vector<int32_t> label8(const uint8_t* m, int rows, int cols, int* count) {
     }
     return a;
   };
-  for (long x = 0; x < rows; x++)
-    for (long y = 0; y < cols; y++) {
-      const size_t i = size_t(x) * cols + y;
-      if (!m[i]) continue;
-      parent[i] = int32_t(i);
-      const long nb[4][2] = {{x - 1, y - 1}, {x - 1, y}, {x - 1, y + 1}, {x, y - 1}};
-      for (const auto& q : nb) {
-        if (q[0] < 0 || q[1] < 0 || q[1] >= cols) continue;
-        const size_t j = size_t(q[0]) * cols + q[1];
-        if (!m[j]) continue;
-        const int32_t a = find(int32_t(i)), b = find(int32_t(j));
+  for (int r = 1; r < R2; r++) {
+    const auto& up = row_runs[size_t(r - 1)];
+    const auto& cur = row_runs[size_t(r)];
+    size_t u = 0;
+    for (size_t k = 0; k < cur.size(); k++) {
+      const int32_t s0 = cur[k].first, e0 = cur[k].second;
+      while (u < up.size() && up[u].second + 1 < s0) u++;
+      for (size_t v = u; v < up.size() && up[v].first <= e0 + 1; v++) {
+        const int32_t a = find(first[size_t(r) - 1] + int32_t(v)), b = find(first[size_t(r)] + int32_t(k));
         if (a != b) parent[size_t(std::max(a, b))] = std::min(a, b);
       }
     }
-  // unions keep the smaller root, so a component's root is its first pixel
-  // in raster order and is numbered before any other pixel of it
-  std::vector<int32_t> lab(N, 0);
-  int32_t next = 0;
-  for (size_t i = 0; i < N; i++) {
-    if (!m[i]) continue;
-    const int32_t r = find(int32_t(i));
-    lab[i] = (size_t(r) == i) ? ++next : lab[size_t(r)];
   }
-  *count = next;
-  return lab;
+  // unions keep the smaller root: a component's root is its first run
+  std::vector<int32_t> lab(static_cast<size_t>(nrun));
+  int32_t next = 0;
+  for (int32_t k = 0; k < nrun; k++) {
+    const int32_t rt = find(k);
+    lab[size_t(k)] = (rt == k) ? next++ : lab[size_t(rt)];
+  }
+  std::fill(labels, labels + size_t(rows) * cols, -1);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < rows; i++) {
+    const int r = 2 * i + 1;
+    const auto& rr = row_runs[size_t(r)];
+    for (size_t k = 0; k < rr.size(); k++) {
+      const int32_t l = lab[size_t(first[size_t(r)]) + k];
+      for (int c = rr[k].first | 1; c <= rr[k].second; c += 2)
+        if ((c - 1) / 2 < cols) labels[size_t(i) * cols + (c - 1) / 2] = l;
+    }
+  }
+  return next;
 }
 
 // Greedy merging of the initial regions by mean boundary strength
@@ -1074,19 +1103,11 @@ void ucm_from_strength(int rows, int cols, bool double_size, double* ucm, UcmSta
   }
   // ---- regions: 8-connected zeros of the cleaned grid, sampled at the odd
   // sites (one per pixel), numbered from 0
-  std::vector<uint8_t> zero(size_t(R2) * C2);
-#pragma omp parallel for schedule(static)
-  for (long i = 0; i < long(zero.size()); i++) zero[size_t(i)] = pb2[size_t(i)] == 0;
-  int nlab = 0;
-  const std::vector<int32_t> lab2 = label8(zero.data(), R2, C2, &nlab);
   S.labels.resize(N);
+  const int ncomp = region_labels(pb2.data(), R2, C2, rows, cols, S.labels.data());
   int totcc = -1;
-  for (int i = 0; i < rows; i++)
-    for (int j = 0; j < cols; j++) {
-      const int32_t l = lab2[size_t(2 * i + 1) * C2 + 2 * j + 1] - 1;
-      S.labels[size_t(i) * cols + j] = l;
-      totcc = std::max(totcc, int(l));
-    }
+  for (size_t i = 0; i < N; i++) totcc = std::max(totcc, int(S.labels[i]));
+  (void)ncomp;
   if (*std::min_element(S.labels.begin(), S.labels.end()) < 0)
     throw std::runtime_error("a pixel site of the cleaned grid is a boundary (the reference's region ids go negative)");
   totcc++;
