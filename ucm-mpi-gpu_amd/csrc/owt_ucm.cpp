@@ -713,13 +713,12 @@ int region_labels(const double* g, int R2, int C2, int rows, int cols, int32_t* 
 void merge_regions(const double* local, const int32_t* part, int rows, int cols, int totcc, double* ucm) {
   const int txc = 2 * rows + 1, tyc = 2 * cols + 1;
   std::fill(ucm, ucm + size_t(txc) * tyc, 0.0);
-  // Boundary elements, visited in increasing coordinate (the reference's
-  // column-major doubled-grid index): per pixel column y, the elements between
-  // vertically adjacent pixels (doubled column 2y + 1), then those between
-  // horizontally adjacent ones (column 2y + 2), rows increasing.  Each pair of
-  // regions' sums therefore run over its elements in coordinate order, as
-  // the reference's sorted boundary lists do.  Pairs are found through each
-  // region's short neighbour list.
+  // Boundary elements, gathered in row-major order (sequential reads of the
+  // region ids and the strengths), grouped by pair with a stable counting
+  // sort, then sorted within each pair by coordinate (the reference's
+  // column-major doubled-grid index), so that each pair's sums run over its
+  // elements in coordinate order as the reference's sorted boundary lists
+  // do.  Pairs are found through each region's short neighbour list.
   struct Pair {
     double energy, total, length;
     std::vector<std::pair<int32_t, int32_t>> runs;  // [begin, end) ranges of coord
@@ -731,8 +730,10 @@ void merge_regions(const double* local, const int32_t* part, int rows, int cols,
   std::vector<Pair> pairs;
   std::vector<std::vector<std::pair<int32_t, int32_t>>> adj(static_cast<size_t>(totcc));  // (neighbour, pair)
   std::vector<int32_t> el_pair, el_coord;
+  std::vector<double> el_val;
   el_pair.reserve(size_t(rows) * cols / 2);
   el_coord.reserve(size_t(rows) * cols / 2);
+  el_val.reserve(size_t(rows) * cols / 2);
   auto pair_of = [&](int32_t a, int32_t b) {
     for (const auto& e : adj[size_t(a)])
       if (e.first == b) return e.second;
@@ -742,36 +743,59 @@ void merge_regions(const double* local, const int32_t* part, int rows, int cols,
     adj[size_t(b)].emplace_back(a, id);
     return id;
   };
-  auto visit = [&](int32_t a, int32_t b, int32_t coord) {
-    const int32_t id = pair_of(std::min(a, b), std::max(a, b));
-    Pair& q = pairs[size_t(id)];
-    q.length++;
-    q.total += local[size_t(coord % txc) * tyc + coord / txc];
-    el_pair.push_back(id);
-    el_coord.push_back(coord);
-  };
-  for (int y = 0; y < cols; y++) {
-    for (int x = 0; x + 1 < rows; x++) {
-      const int32_t a = part[size_t(x) * cols + y], b = part[size_t(x + 1) * cols + y];
-      if (a != b) visit(a, b, (2 * x + 2) + (2 * y + 1) * txc);
-    }
-    if (y + 1 < cols)
-      for (int x = 0; x < rows; x++) {
-        const int32_t a = part[size_t(x) * cols + y], b = part[size_t(x) * cols + y + 1];
-        if (a != b) visit(a, b, (2 * x + 1) + (2 * y + 2) * txc);
+  for (int x = 0; x < rows; x++) {
+    const int32_t* pr = part + size_t(x) * cols;
+    const double* l1 = local + size_t(2 * x + 1) * tyc;  // doubled row 2x + 1
+    const double* l2 = l1 + tyc;                         // doubled row 2x + 2
+    for (int y = 0; y < cols; y++) {
+      const int32_t a = pr[y];
+      if (x + 1 < rows) {
+        const int32_t b = pr[size_t(cols) + y];
+        if (a != b) {
+          el_pair.push_back(pair_of(std::min(a, b), std::max(a, b)));
+          el_coord.push_back((2 * x + 2) + (2 * y + 1) * txc);
+          el_val.push_back(l2[2 * y + 1]);
+        }
       }
+      if (y + 1 < cols) {
+        const int32_t b = pr[y + 1];
+        if (a != b) {
+          el_pair.push_back(pair_of(std::min(a, b), std::max(a, b)));
+          el_coord.push_back((2 * x + 1) + (2 * y + 2) * txc);
+          el_val.push_back(l1[2 * y + 2]);
+        }
+      }
+    }
   }
-  // each pair's elements contiguous (a stable counting sort by pair)
   std::vector<int32_t> coord(el_coord.size());
   {
     std::vector<int32_t> start(pairs.size() + 1, 0);
     for (int32_t id : el_pair) start[size_t(id) + 1]++;
     for (size_t k = 0; k < pairs.size(); k++) start[k + 1] += start[k];
-    for (size_t k = 0; k < pairs.size(); k++) pairs[k].runs.emplace_back(start[k], start[k + 1]);
-    for (size_t e = 0; e < el_pair.size(); e++) coord[size_t(start[size_t(el_pair[e])]++)] = el_coord[e];
+    std::vector<std::pair<int32_t, double>> cv(el_coord.size());
+    std::vector<int32_t> at(start.begin(), start.end() - 1);
+    for (size_t e = 0; e < el_pair.size(); e++) cv[size_t(at[size_t(el_pair[e])]++)] = {el_coord[e], el_val[e]};
+    std::vector<int32_t>().swap(el_pair);
+    std::vector<int32_t>().swap(el_coord);
+    std::vector<double>().swap(el_val);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long k = 0; k < long(pairs.size()); k++) {
+      auto b0 = cv.begin() + start[size_t(k)], b1 = cv.begin() + start[size_t(k) + 1];
+      std::sort(b0, b1, [](const std::pair<int32_t, double>& p, const std::pair<int32_t, double>& q) {
+        return p.first < q.first;
+      });
+      double len = 0, tot = 0;
+      for (auto it = b0; it != b1; ++it) {
+        len++;
+        tot += it->second;
+        coord[size_t(it - cv.begin())] = it->first;
+      }
+      Pair& q = pairs[size_t(k)];
+      q.length = len;
+      q.total = tot;
+      q.runs.emplace_back(start[size_t(k)], start[size_t(k) + 1]);
+    }
   }
-  std::vector<int32_t>().swap(el_pair);
-  std::vector<int32_t>().swap(el_coord);
   std::vector<std::unordered_map<int32_t, int32_t>> nb(static_cast<size_t>(totcc));
   for (size_t r = 0; r < adj.size(); r++) {
     nb[r].reserve(adj[r].size());
