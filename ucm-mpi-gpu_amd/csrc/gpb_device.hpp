@@ -97,6 +97,29 @@ enum KernelClass {
   kKTexton = 0, kKCues, kKNonmax, kKAffinity, kKSpmv, kKDots, kKUpdate, kKRitz, kKSpb, kKGpb, kKWarm, kKUcm, kKOther,
   kNumKClasses
 };
+// Pinned host buffer (hipHostMalloc): copies to and from it are DMA
+// transfers that stay asynchronous; a pageable buffer makes every small
+// copy a staged, blocking one.
+template <typename T>
+struct HostBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    GPB_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+};
+
 struct KernelTimes {
   bool on = false;
   unsigned mask = ~0u;  // kernel classes that get events while on
@@ -193,6 +216,18 @@ struct Context {
   int warm_mode = 0;
   int warm_basis = 80, warm_keep = 24;  // block-solver basis and restart size
   double warm_ms = 0;  // network inference time of the last warm-started solve
+
+  // pinned host workspace (grow-only), keyed like the device workspace
+  std::map<std::string, std::shared_ptr<void>> host_scratch;
+  template <typename T>
+  T* host_ws(const std::string& name, size_t count) {
+    auto it = host_scratch.find(name);
+    if (it == host_scratch.end())
+      it = host_scratch.emplace(name, std::static_pointer_cast<void>(std::make_shared<HostBuf<T>>())).first;
+    auto* b = static_cast<HostBuf<T>*>(it->second.get());
+    b->reserve(count);
+    return b->p;
+  }
 
   template <typename T>
   T* ws(const std::string& name, size_t count) {

@@ -18,6 +18,7 @@
 // the device in FP64, and convergence is judged by the relative generalised
 // residual ||(D - W) v - lambda D v|| / ||D v||, computed exactly from the
 // Krylov relation (the residual of every Ritz vector is parallel to v_m).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1341,8 +1342,17 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
   stats.basis = m;
   stats.block = 1;
   int matvecs = 1;
-  std::vector<double> Th(size_t(m) * m), th, Y;
-  std::vector<double> betah(kMaxBasis);
+  std::vector<double> th, Y;
+  // restart exchanges go through pinned host buffers (asynchronous DMA; the
+  // host reads them after a stream synchronisation and rewrites the upload
+  // buffers only after the next one, by which time their copies are done)
+  double* Th = c.host_ws<double>("eig.h.T", size_t(m) * m);
+  double* betah = c.host_ws<double>("eig.h.beta", kMaxBasis);
+  int* viol_h = c.host_ws<int>("eig.h.viol", 1);
+  double* Yk = c.host_ws<double>("eig.h.Y", size_t(m) * keep);
+  double* wnrm_h = c.host_ws<double>("eig.h.nrm", K + 1);
+  double* gacc_p = c.host_ws<double>("eig.h.gacc", 3);
+  double* T0 = c.host_ws<double>("eig.h.T0", size_t(m) * m);
   bool converged = false;
   std::vector<double> resid(K, 0.0);
   // One restart cycle's device work, steps k0 .. m-1 on the basis Vc.  From
@@ -1492,12 +1502,12 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     }
     matvecs += m - k;
     // Rayleigh-Ritz on the host
-    int viol_h = kMaxBasis + 1;
-    GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
-    if (optimistic) GPB_HIP(hipMemcpyAsync(&viol_h, first_viol, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    *viol_h = kMaxBasis + 1;
+    GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(betah, beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+    if (optimistic) GPB_HIP(hipMemcpyAsync(viol_h, first_viol, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
-    if (optimistic && viol_h < m) {
+    if (optimistic && *viol_h < m) {
       // the omega recurrence triggered at step viol_h: every step before it
       // matches the gated form, so redo the cycle from there with the gated
       // pass.  The steps after it are discarded; of what they leave behind,
@@ -1509,14 +1519,15 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       hipLaunchKernelGGL(fill_int1_kernel, dim3(1), dim3(1), 0, c.stream, pend, 0);
       GPB_LAUNCH_CHECK();
       launch_fill(c, w, Np, 0.f);
-      cycle(k, V, viol_h, false);
-      matvecs += m - viol_h;
-      stats.redone_steps += m - viol_h;
-      GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
-      GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+      const int redo = *viol_h;
+      cycle(k, V, redo, false);
+      matvecs += m - redo;
+      stats.redone_steps += m - redo;
+      GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * m * m, hipMemcpyDeviceToHost, c.stream));
+      GPB_HIP(hipMemcpyAsync(betah, beta, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
       GPB_HIP(hipStreamSynchronize(c.stream));
     }
-    std::vector<double> A = Th;
+    std::vector<double> A(Th, Th + size_t(m) * m);
     {
       // smallest Ritz values first
       std::vector<double> thd, Yd;
@@ -1529,24 +1540,23 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
       }
     }
     const int nk = keep;
-    std::vector<double> Yk(size_t(m) * nk);
     for (int j = 0; j < m; j++)
       for (int i = 0; i < nk; i++) Yk[size_t(j) * nk + i] = Y[size_t(j) * m + i];
-    GPB_HIP(hipMemcpyAsync(Ydev, Yk.data(), sizeof(double) * m * nk, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(Ydev, Yk, sizeof(double) * m * nk, hipMemcpyHostToDevice, c.stream));
     launch_ritz(c, V, Np, m, nk, Ydev, V2, Np);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(nk) * Np, V + size_t(m) * Np, Np * sizeof(float), hipMemcpyDeviceToDevice,
                            c.stream));
     // weighted norms ||D^1/2 u_i||, ||D^1/2 v_m|| in one launch
-    std::vector<double> nrm(K + 1);
     {
       int sel[kMaxNvec + 2];
       for (int i = 0; i < K; i++) sel[i] = i;
       sel[K] = nk;
       red.wnorms(V2, Np, sel, K + 1, st.D, h);
     }
-    GPB_HIP(hipMemcpyAsync(nrm.data(), h, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipMemcpyAsync(gacc_h, gacc, sizeof(gacc_h), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(wnrm_h, h, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(gacc_p, gacc, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
+    std::copy(gacc_p, gacc_p + 3, gacc_h);
     c.kt.fold();
     if (c.kt.timing(kKDots)) c.kt.bytes[kKDots] += gacc_h[0] - stats.pass2_bytes[0];
     if (c.kt.timing(kKUpdate)) c.kt.bytes[kKUpdate] += gacc_h[1] - stats.pass2_bytes[1];
@@ -1558,7 +1568,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     const double bm = betah[m - 1];
     double worst = 0;
     for (int i = 0; i < K; i++) {
-      resid[i] = std::fabs(bm * Y[size_t(m - 1) * m + i]) * std::sqrt(nrm[K]) / std::sqrt(std::max(nrm[i], 1e-300));
+      resid[i] = std::fabs(bm * Y[size_t(m - 1) * m + i]) * std::sqrt(wnrm_h[K]) / std::sqrt(std::max(wnrm_h[i], 1e-300));
       worst = std::max(worst, resid[i]);
     }
     stats.max_residual = worst;
@@ -1567,9 +1577,9 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     converged = conv_count(opt, th.data(), resid.data()) == K;
     if (converged || matvecs >= opt.max_matvecs) break;
     // thick restart: basis = Ritz vectors [0, keep) and v_m; T = diag(theta)
-    std::vector<double> T0(size_t(m) * m, 0.0);
+    std::fill(T0, T0 + size_t(m) * m, 0.0);
     for (int i = 0; i < keep; i++) T0[size_t(i) * m + i] = th[size_t(i)];
-    GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * m * m, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
     if (pro) {
