@@ -818,8 +818,14 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     s.block = 1;
   }
   evals.assign(size_t(np) * K, 0.0);
-  std::vector<double> Th(size_t(np) * m * m), betah(size_t(np) * 2 * kMaxBasis), nrmh(size_t(np) * kPS);
-  std::vector<double> Yall(size_t(np) * m * keep);
+  // restart exchanges through pinned host buffers (asynchronous DMA; read
+  // after a stream synchronisation, upload buffers rewritten only after the
+  // next one)
+  double* Th = c.host_ws<double>("bat.h.T", size_t(np) * m * m);
+  double* betah = c.host_ws<double>("bat.h.beta", size_t(np) * 2 * kMaxBasis);
+  double* nrmh = c.host_ws<double>("bat.h.nrm", size_t(np) * kPS);
+  double* Yall = c.host_ws<double>("bat.h.Y", size_t(np) * m * keep);
+  double* T0 = c.host_ws<double>("bat.h.T0", size_t(np) * m * m);
   std::vector<std::vector<double>> th(static_cast<size_t>(np)), Y(static_cast<size_t>(np));
   std::vector<int> rows(size_t(K + 1));
   for (int i = 0; i < K; i++) rows[size_t(i)] = i;
@@ -911,13 +917,13 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       if (act[size_t(p)]) mv[size_t(p)] += m - k;
     cycles++;
     // Rayleigh-Ritz for every active patch on the host
-    GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * np * m * m, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipMemcpyAsync(betah.data(), beta, sizeof(double) * np * 2 * kMaxBasis, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * np * m * m, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(betah, beta, sizeof(double) * np * 2 * kMaxBasis, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
 #pragma omp parallel for schedule(dynamic, 1)
     for (int p = 0; p < np; p++) {
       if (!act[size_t(p)]) continue;
-      std::vector<double> A(Th.begin() + long(size_t(p) * m * m), Th.begin() + long(size_t(p + 1) * m * m)), thd, Yd;
+      std::vector<double> A(Th + size_t(p) * m * m, Th + size_t(p + 1) * m * m), thd, Yd;
       symeig_desc(m, A, thd, Yd);
       th[size_t(p)].assign(size_t(m), 0.0);
       Y[size_t(p)].assign(size_t(m) * m, 0.0);
@@ -928,7 +934,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       for (int j = 0; j < m; j++)
         for (int i = 0; i < keep; i++) Yall[(size_t(p) * m + j) * keep + i] = Y[size_t(p)][size_t(j) * m + i];
     }
-    GPB_HIP(hipMemcpyAsync(Ydev, Yall.data(), sizeof(double) * np * m * keep, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(Ydev, Yall, sizeof(double) * np * m * keep, hipMemcpyHostToDevice, c.stream));
     {
       KScope ks(c, kKRitz, double(Nt) * 4.0 * (m + keep));
       hipLaunchKernelGGL(britz_kernel, dim3(bs.GW), dim3(256), 0, c.stream, bs.P, np, V, Nt, m, keep, Ydev, V2,
@@ -945,7 +951,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     hipLaunchKernelGGL(breduce_kernel, dim3(K + 1, np), dim3(kRedThreads), 0, c.stream, bs.partial, bs.G, bs.brange,
                        kPS, 0, wn, kPS, nullptr, bs.active);
     GPB_LAUNCH_CHECK();
-    GPB_HIP(hipMemcpyAsync(nrmh.data(), wn, sizeof(double) * np * kPS, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(nrmh, wn, sizeof(double) * np * kPS, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     c.kt.fold();
     std::swap(V, V2);
@@ -980,11 +986,11 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     }
     if (nactive == 0) break;
     // thick restart for the rest: basis = Ritz vectors [0, keep) and v_m; T = diag(theta)
-    std::vector<double> T0(size_t(np) * m * m, 0.0);
+    std::fill(T0, T0 + size_t(np) * m * m, 0.0);
     for (int p = 0; p < np; p++)
       if (act[size_t(p)])
         for (int i = 0; i < keep; i++) T0[size_t(p) * m * m + size_t(i) * m + i] = th[size_t(p)][size_t(i)];
-    GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * np * m * m, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * np * m * m, hipMemcpyHostToDevice, c.stream));
     if (newly_done) GPB_HIP(hipMemcpyAsync(bs.active, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
     k = keep;
     if (pro) {

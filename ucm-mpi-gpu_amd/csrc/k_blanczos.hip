@@ -11,6 +11,7 @@
 // cycle needs no host synchronisation.  The vector D^1/2 1 (lambda = 0) is
 // locked as basis vector 0.  The starting block is random or supplied by the
 // caller (warm start).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -868,7 +869,17 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   stats.basis = M;
   stats.block = B;
   int matvecs = 1;
-  std::vector<double> Th(size_t(M) * M), th, Y, Rh(B * B);
+  std::vector<double> th, Y;
+  // restart exchanges through pinned host buffers (asynchronous DMA; read
+  // after a stream synchronisation, upload buffers rewritten only after the
+  // next one)
+  double* Th = c.host_ws<double>("beig.h.T", size_t(M) * M);
+  double* Rh = c.host_ws<double>("beig.h.R", size_t(B) * B);
+  double* Gd = c.host_ws<double>("beig.h.G", size_t(B) * B);
+  int* brk_h = c.host_ws<int>("beig.h.brk", 1);
+  double* Yk = c.host_ws<double>("beig.h.Y", size_t(M) * keep);
+  double* nh = c.host_ws<double>("beig.h.nrm", size_t(K));
+  double* T0 = c.host_ws<double>("beig.h.T0", size_t(M) * M);
   std::vector<double> resid(K, 0.0);
   bool converged = false;
   int kcur = 0;
@@ -938,19 +949,18 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     }
     kcur = k;  // projected size
     // Rayleigh-Ritz on the host (smallest first)
-    GPB_HIP(hipMemcpyAsync(Th.data(), T, sizeof(double) * M * M, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipMemcpyAsync(Rh.data(), Rtot, sizeof(double) * B * B, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * M * M, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(Rh, Rtot, sizeof(double) * B * B, hipMemcpyDeviceToHost, c.stream));
     // weighted Gram of the residual panel
     ops.dots(V + size_t(kcur) * Np, B, V + size_t(kcur) * Np, st.D, G);
-    std::vector<double> Gd(B * B);
-    int brk_h = 0;
-    GPB_HIP(hipMemcpyAsync(Gd.data(), G, sizeof(double) * B * B, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipMemcpyAsync(&brk_h, brk, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    *brk_h = 0;
+    GPB_HIP(hipMemcpyAsync(Gd, G, sizeof(double) * B * B, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(brk_h, brk, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     c.kt.fold();
     // A rank-deficient panel means the block Krylov space became (nearly)
     // invariant; the caller falls back to the single-vector solver.
-    if (brk_h > 0) {
+    if (*brk_h > 0) {
       stats.iterations = matvecs;
       throw BlockBreakdown("block Lanczos: rank-deficient panel");
     }
@@ -968,17 +978,15 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
       }
     }
     const int nk = keep;
-    std::vector<double> Yk(size_t(kcur) * nk);
     for (int j = 0; j < kcur; j++)
       for (int i = 0; i < nk; i++) Yk[size_t(j) * nk + i] = Y[size_t(j) * kcur + i];
-    GPB_HIP(hipMemcpyAsync(Ydev, Yk.data(), sizeof(double) * kcur * nk, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(Ydev, Yk, sizeof(double) * kcur * nk, hipMemcpyHostToDevice, c.stream));
     ops.ritz(V, kcur, nk, Ydev, V2);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(nk) * Np, V + size_t(kcur) * Np, size_t(B) * Np * sizeof(float),
                            hipMemcpyDeviceToDevice, c.stream));
     // ||D^1/2 u_i|| for the wanted Ritz vectors
     ops.wnorms(V2, K, nrm);
-    std::vector<double> nh(static_cast<size_t>(K));
-    GPB_HIP(hipMemcpyAsync(nh.data(), nrm, sizeof(double) * K, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipMemcpyAsync(nh, nrm, sizeof(double) * K, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
     std::swap(V, V2);
     double worst = 0;
@@ -1003,9 +1011,9 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
                    st.H, B, matvecs, worst);
     converged = conv_count(opt, th.data(), resid.data()) == K;
     if (converged || matvecs >= opt.max_matvecs) break;
-    std::vector<double> T0(size_t(M) * M, 0.0);
+    std::fill(T0, T0 + size_t(M) * M, 0.0);
     for (int i = 0; i < keep; i++) T0[size_t(i) * M + i] = th[size_t(i)];
-    GPB_HIP(hipMemcpyAsync(T, T0.data(), sizeof(double) * M * M, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * M * M, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
     if (semi) {
