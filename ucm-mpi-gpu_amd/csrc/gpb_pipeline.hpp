@@ -16,6 +16,14 @@ struct BlockBreakdown : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// Host <-> device copies of caller buffers (gpb_run.hip): through two pinned
+// staging chunks, the DMA of one overlapping the host-side copy of the other
+// (parallel_copy, hostcopy.cpp); small copies go directly.  stage_d2h
+// returns with the data in host; stage_h2d returns once host may be reused.
+void parallel_copy(void* dst, const void* src, size_t bytes);
+void stage_d2h(Context& c, void* host, const void* dev, size_t bytes);
+void stage_h2d(Context& c, void* dev, const void* host, size_t bytes);
+
 // stage (a)
 void upload_schmid(Context& c);
 void launch_texton(Context& c, const uint32_t* d_rgb, int W, int H, const float* d_codebook, uint8_t* d_labels);

@@ -4,6 +4,7 @@
 // Reference call chain: computeGPb (damascene/src/damascene.cu:249-661) on the
 // GPU side; GlobalPb -> multiscalePb -> spectralPb (src/globalPb.cpp:15-126,
 // src/multiscalePb.cpp:16-283, src/spectralPb.cpp:33-343) on the CPU side.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -316,6 +317,71 @@ static void run_patches(Context& c, int W, int H, const uint32_t* d_packed, cons
     sum.second_passes = st.second_passes;
   }
   c.last_lanczos = sum;
+}
+
+// Pinned staging of host-buffer transfers.  A copy to or from pageable
+// memory is staged by the runtime and blocks; here two pinned chunks
+// alternate, so the DMA of one chunk overlaps the spread host-side copy of
+// the other.
+namespace {
+constexpr size_t kStageChunk = size_t(16) << 20;
+constexpr size_t kStageMin = size_t(1) << 20;  // smaller copies go directly
+
+struct XferEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  XferEvents() {
+    for (auto& x : e) GPB_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  }
+  ~XferEvents() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+}  // namespace
+
+void stage_d2h(Context& c, void* host, const void* dev, size_t bytes) {
+  if (bytes < kStageMin) {
+    GPB_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  char* buf[2] = {c.host_ws<char>("io.stage0", kStageChunk), c.host_ws<char>("io.stage1", kStageChunk)};
+  XferEvents ev;
+  const size_t n = (bytes + kStageChunk - 1) / kStageChunk;
+  auto len = [&](size_t i) { return std::min(kStageChunk, bytes - i * kStageChunk); };
+  for (size_t i = 0; i < n + 2; i++) {
+    const int b = int(i & 1);
+    if (i >= 2) {  // chunk i - 2 has landed in buf[b]: move it out, then reuse buf[b]
+      GPB_HIP(hipEventSynchronize(ev.e[b]));
+      parallel_copy(static_cast<char*>(host) + (i - 2) * kStageChunk, buf[b], len(i - 2));
+    }
+    if (i < n) {
+      GPB_HIP(hipMemcpyAsync(buf[b], static_cast<const char*>(dev) + i * kStageChunk, len(i), hipMemcpyDeviceToHost,
+                             c.stream));
+      GPB_HIP(hipEventRecord(ev.e[b], c.stream));
+    }
+  }
+}
+
+void stage_h2d(Context& c, void* dev, const void* host, size_t bytes) {
+  if (bytes < kStageMin) {
+    GPB_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  char* buf[2] = {c.host_ws<char>("io.stage0", kStageChunk), c.host_ws<char>("io.stage1", kStageChunk)};
+  XferEvents ev;
+  const size_t n = (bytes + kStageChunk - 1) / kStageChunk;
+  for (size_t i = 0; i < n; i++) {
+    const int b = int(i & 1);
+    if (i >= 2) GPB_HIP(hipEventSynchronize(ev.e[b]));  // chunk i - 2's DMA out of buf[b] is done
+    const size_t l = std::min(kStageChunk, bytes - i * kStageChunk);
+    parallel_copy(buf[b], static_cast<const char*>(host) + i * kStageChunk, l);
+    GPB_HIP(hipMemcpyAsync(static_cast<char*>(dev) + i * kStageChunk, buf[b], l, hipMemcpyHostToDevice, c.stream));
+    GPB_HIP(hipEventRecord(ev.e[b], c.stream));
+  }
+  // the staging chunks are free (and host reusable) once the last DMAs ran
+  for (size_t i = n > 2 ? n - 2 : 0; i < n; i++) GPB_HIP(hipEventSynchronize(ev.e[i & 1]));
 }
 
 }  // namespace gpb
@@ -808,16 +874,16 @@ int gpb_compute(gpb_ctx* ctx, int width, int height, const uint32_t* packed, con
     float* d_gmax = gpb_max ? c.ws<float>("io.gmax", N) : nullptr;
     float* d_gall = gpb_all ? c.ws<float>("io.gall", 8 * N) : nullptr;
     float* d_or = gpb_orient ? c.ws<float>("io.gorient", 8 * N) : nullptr;
-    GPB_HIP(hipMemcpyAsync(d_rgb, packed, N * 4, hipMemcpyHostToDevice, c.stream));
+    stage_h2d(c, d_rgb, packed, N * 4);
     GPB_HIP(hipMemcpyAsync(d_cb, codebook, kSchmidNF * kTextonK * 4, hipMemcpyHostToDevice, c.stream));
     double ev[kMaxNvec];
     PipelineBuffers b;
     run_pipeline(c, DeviceImage{width, height, d_rgb, d_cb}, eo, false, d_gmax, d_gall, d_or, ev, &b);
     for (int i = 0; i < eo.K; i++) c.last_evals[i] = ev[i];
-    if (gpb_max) GPB_HIP(hipMemcpyAsync(gpb_max, d_gmax, N * 4, hipMemcpyDeviceToHost, c.stream));
-    if (gpb_all) GPB_HIP(hipMemcpyAsync(gpb_all, d_gall, 8 * N * 4, hipMemcpyDeviceToHost, c.stream));
-    if (gpb_orient) GPB_HIP(hipMemcpyAsync(gpb_orient, d_or, 8 * N * 4, hipMemcpyDeviceToHost, c.stream));
-    if (mpb_nmax) GPB_HIP(hipMemcpyAsync(mpb_nmax, b.nmax, N * 4, hipMemcpyDeviceToHost, c.stream));
+    if (gpb_max) stage_d2h(c, gpb_max, d_gmax, N * 4);
+    if (gpb_all) stage_d2h(c, gpb_all, d_gall, 8 * N * 4);
+    if (gpb_orient) stage_d2h(c, gpb_orient, d_or, 8 * N * 4);
+    if (mpb_nmax) stage_d2h(c, mpb_nmax, b.nmax, N * 4);
     GPB_HIP(hipStreamSynchronize(c.stream));
   });
 }

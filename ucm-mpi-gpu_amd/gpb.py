@@ -497,17 +497,22 @@ class Context:
                                          spb.reshape(-1)))
         return spb
 
-    def compute(self, packed: np.ndarray, codebook: np.ndarray):
-        """computeGPb semantics; returns dict of gpb_max [H,W], gpb_all [8,W,H], gpb_orient [8,H,W], mpb_nmax."""
+    def compute(self, packed: np.ndarray, codebook: np.ndarray, orient: bool = True):
+        """computeGPb semantics; returns dict of gpb_max [H,W], gpb_all [8,W,H], gpb_orient [8,H,W] (unless
+        orient is False), mpb_nmax."""
         packed = np.ascontiguousarray(packed, np.uint32)
         H, W = packed.shape
         gmax = np.empty((H, W), np.float32)
         gall = np.empty((8, W, H), np.float32)
-        gor = np.empty((8, H, W), np.float32)
+        gor = np.empty((8, H, W), np.float32) if orient else None
         nm = np.empty((H, W), np.float32)
         self._chk(lib.gpb_compute(self.h, W, H, packed.reshape(-1), np.ascontiguousarray(codebook, np.float32).reshape(-1),
-                                  gmax.ctypes.data, gall.ctypes.data, gor.ctypes.data, nm.ctypes.data))
-        return dict(gpb_max=gmax, gpb_all=gall, gpb_orient=gor, mpb_nmax=nm)
+                                  gmax.ctypes.data, gall.ctypes.data, gor.ctypes.data if orient else None,
+                                  nm.ctypes.data))
+        out = dict(gpb_max=gmax, gpb_all=gall, mpb_nmax=nm)
+        if orient:
+            out["gpb_orient"] = gor
+        return out
 
     def set_profiling(self, on: bool):
         self._chk(lib.gpb_set_profiling(self.h, int(on)))
