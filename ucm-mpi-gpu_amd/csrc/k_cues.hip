@@ -109,6 +109,7 @@ struct CueParams {
   const int* off;           // disc offsets (LDS tile offset) sorted by radius
   const uint8_t* slice;     // slice index per offset
   int end_by_r[21];         // number of offsets with d^2 <= r^2
+  int nslice[4][kNumSlices];  // disc pixels per slice at the snapshot radii 3, 5, 10, 20
   const int* ring_off;      // rings 5 < d <= 10 and 10 < d <= 20, offsets sorted by slice, padded
   int ring_seg[kCueRings * kNumSlices + 1];  // start of (ring, slice) list i in ring_off
   float smooth[3][9];       // half kernels: [0] bg (sigma 2.5, 17 taps), [1] cg (sigma 1.25, 9 taps), [2] none
@@ -154,22 +155,45 @@ __device__ __forceinline__ CT recip(CT x) {
   if constexpr (sizeof(CT) == 4) return __builtin_amdgcn_rcpf(x);  // 1 ulp
   else return CT(1) / x;
 }
-template <int NB, int KH, typename CT>
-__device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
-                                           const CT* __restrict__ g, const CT* __restrict__ csum,
-                                           CT out[kNumOri]) {
-  // pass A: per-slice mass after smoothing, a_s = 1 / mass
-  CT alpha[kNumSlices];
+// Pass A's mass correction over KH consecutive bins from b0: alpha[s] -=
+// (1 - c_b) h_s[b], c_b = csum[b].
+template <int KH, typename CT>
+__device__ __forceinline__ void chi2_mass_edge(const uint32_t* __restrict__ hist, int lane, int b0,
+                                               const CT* __restrict__ csum, CT alpha[kNumSlices]) {
 #pragma unroll
-  for (int s = 0; s < kNumSlices; s++) alpha[s] = CT(0);
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    const CT cb = csum[b];
+  for (int i = 0; i < KH; i++) {
+    const int b = b0 + i;
+    const CT e = CT(1) - csum[b];
 #pragma unroll
     for (int grp = 0; grp < 4; grp++) {
       const uint32_t w = hist[(grp * kNbStride + b) * kCueLanes + lane];
 #pragma unroll
-      for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(cb, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
+      for (int q = 0; q < 4; q++) alpha[4 * grp + q] = fma_t(-e, CT((w >> (8 * q)) & 0xffu), alpha[4 * grp + q]);
+    }
+  }
+}
+
+// ns: the disc pixels of each slice at this radius (every label of a channel
+// is below its nb, so the counts of a slice sum to ns[s]).
+template <int NB, int KH, typename CT>
+__device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist, int lane, int nb,
+                                           const CT* __restrict__ g, const CT* __restrict__ csum,
+                                           const int* __restrict__ ns, CT out[kNumOri]) {
+  // pass A: per-slice mass after smoothing, a_s = 1 / mass.  The smoothing
+  // kernel sums to 1, so a bin whose taps all fall inside [0, nb) keeps its
+  // whole count: mass_s = ns[s] - sum_b (1 - c_b) h_s[b] over the bins within
+  // KH of either end only (all bins when the two ends overlap; none without
+  // smoothing, where the mass is the pixel count itself)
+  CT alpha[kNumSlices];
+#pragma unroll
+  for (int s = 0; s < kNumSlices; s++) alpha[s] = CT(ns[s]);
+  if (KH > 0) {
+    if (nb >= 2 * KH) {  // uniform
+      chi2_mass_edge<KH, CT>(hist, lane, 0, csum, alpha);
+      chi2_mass_edge<KH, CT>(hist, lane, nb - KH, csum, alpha);
+    } else {
+#pragma unroll 1
+      for (int b = 0; b < nb; b++) chi2_mass_edge<1, CT>(hist, lane, b, csum, alpha);
     }
   }
 #pragma unroll
@@ -181,7 +205,7 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
   CT T[NB], V[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    CT u = CT(0), v = CT(0);
+    CT l = CT(0), r = CT(0);  // the halves of orientation 0: slices 0..7 and 8..15
 #pragma unroll
     for (int grp = 0; grp < 4; grp++) {
       const uint32_t w = hist[(grp * kNbStride + b) * kCueLanes + lane];
@@ -189,12 +213,14 @@ __device__ __forceinline__ void chi2_snapshot(const uint32_t* __restrict__ hist,
       for (int q = 0; q < 4; q++) {
         const int s = 4 * grp + q;
         const CT c = CT((w >> (8 * q)) & 0xffu);
-        u = fma_t(alpha[s], c, u);
-        v = fma_t(s < kNumOri ? -alpha[s] : alpha[s], c, v);
+        if (s < kNumOri)
+          l = fma_t(alpha[s], c, l);
+        else
+          r = fma_t(alpha[s], c, r);
       }
     }
-    T[b] = u;  // U_T for now, smoothed below
-    V[b] = v;
+    T[b] = l + r;  // U_T for now, smoothed below
+    V[b] = r - l;
   }
   if (KH > 0) {
     CT U[NB];
@@ -350,7 +376,8 @@ __device__ void cue_channel(const CueParams& P, int ch, int ri0, const int* radi
     }
     }
     CT chi[kNumOri];
-    chi2_snapshot<NB, KH, CT>(hist, lane, nb, g, csum, chi);
+    const int rsel = radii[ri] == 3 ? 0 : (radii[ri] == 5 ? 1 : (radii[ri] == 10 ? 2 : 3));
+    chi2_snapshot<NB, KH, CT>(hist, lane, nb, g, csum, P.nslice[rsel], chi);
     const int cue = ri0 + ri;
     if (valid) {
       const size_t N = size_t(W) * H, p = size_t(py) * W + px;
@@ -502,6 +529,7 @@ struct CueTables {
   DevBuf<uint8_t> slice;
   DevBuf<double> gamma;
   int end_by_r[21];
+  int nslice[4][kNumSlices];
   float smooth[3][9];
   double smooth_d[3][9];
   NmsTable nms;
@@ -530,6 +558,14 @@ static CueTables& cue_tables(Context& c) {
     int e = 0;
     while (e < int(offs.size()) && offs[size_t(e)][0] <= r * r) e++;
     t->end_by_r[r] = e;
+  }
+  {
+    const int rs[4] = {3, 5, 10, 20};
+    for (int k = 0; k < 4; k++) {
+      for (int q = 0; q < kNumSlices; q++) t->nslice[k][q] = 0;
+      for (size_t i = 0; i < offs.size(); i++)
+        if (offs[i][0] <= rs[k] * rs[k]) t->nslice[k][sl[i]]++;
+    }
   }
   // the two outer rings, by slice, each slice's list padded to kCueBatch
   // entries with the sentinel offset
@@ -603,6 +639,8 @@ static void local_cues_t(Context& c, const uint32_t* d_rgb, const uint8_t* d_tex
   P.ring_off = t.ring_off.p;
   for (int q = 0; q <= kCueRings * kNumSlices; q++) P.ring_seg[q] = t.ring_seg[q];
   for (int r = 0; r <= kCueRmax; r++) P.end_by_r[r] = t.end_by_r[r];
+  for (int k = 0; k < 4; k++)
+    for (int q = 0; q < kNumSlices; q++) P.nslice[k][q] = t.nslice[k][q];
   for (int a = 0; a < 3; a++)
     for (int k = 0; k < 9; k++) {
       P.smooth[a][k] = t.smooth[a][k];
