@@ -285,23 +285,16 @@ __global__ __launch_bounds__(256) void bthree_term_kernel(const BPatchDev* __res
 // (estimate above sqrt(eps), or the step after one); T[j][j] = alpha and,
 // without a pass, beta_j and the coupling.  nrm[2 p]: ||w||^2 after the
 // three-term step.
-__global__ __launch_bounds__(256) void bomega_kernel(double* __restrict__ T, int m, int j,
-                                                     const double* __restrict__ alpha, double* __restrict__ beta,
-                                                     const double* __restrict__ nrm, double* __restrict__ om,
-                                                     int* __restrict__ flag, int* __restrict__ pending,
-                                                     const int* __restrict__ active, double* __restrict__ npasses) {
-  const int p = blockIdx.x, t = threadIdx.x;
-  if (!active[p]) {
-    if (t == 0) flag[p] = 0;
-    return;
-  }
-  __shared__ double mx[4];
+__device__ __forceinline__ void bomega_body(int p, double* __restrict__ T, int m, int j, double al, double bn,
+                                            double* __restrict__ beta, double* __restrict__ om,
+                                            int* __restrict__ flag, int* __restrict__ pending,
+                                            double* __restrict__ npasses, double* __restrict__ aeff, double* mx) {
+  const int t = threadIdx.x;
   double* Tp = T + size_t(p) * m * m;
   double* bp = beta + size_t(p) * 2 * kMaxBasis;
   const double* om0 = om + (size_t(p) * 2 + (j & 1)) * kOmS;
   double* om_1 = om + (size_t(p) * 2 + ((j + 1) & 1)) * kOmS;
-  const double bn = sqrt(nrm[2 * p]);
-  const double al = alpha[p], bc = bp[j - 1];
+  const double bc = bp[j - 1];
   double o = 0;
   if (t < j) {
     double tw = Tp[size_t(t) * m + j];
@@ -330,6 +323,7 @@ __global__ __launch_bounds__(256) void bomega_kernel(double* __restrict__ T, int
     } else {
       bp[j] = bn;
       bp[kMaxBasis + j] = bn > 1e-300 ? 1.0 / bn : 0.0;
+      if (aeff) aeff[p] = al;
       if (j + 1 < m) {
         Tp[size_t(j) * m + j + 1] = bn;
         Tp[size_t(j + 1) * m + j] = bn;
@@ -338,13 +332,54 @@ __global__ __launch_bounds__(256) void bomega_kernel(double* __restrict__ T, int
   }
 }
 
+__global__ __launch_bounds__(256) void bomega_kernel(double* __restrict__ T, int m, int j,
+                                                     const double* __restrict__ alpha, double* __restrict__ beta,
+                                                     const double* __restrict__ nrm, double* __restrict__ om,
+                                                     int* __restrict__ flag, int* __restrict__ pending,
+                                                     const int* __restrict__ active, double* __restrict__ npasses) {
+  const int p = blockIdx.x;
+  if (!active[p]) {
+    if (threadIdx.x == 0) flag[p] = 0;
+    return;
+  }
+  __shared__ double mx[4];
+  bomega_body(p, T, m, j, alpha[p], sqrt(nrm[2 * p]), beta, om, flag, pending, npasses, nullptr, mx);
+}
+
+// Step j of the fused form, per patch: alpha_j = x . w and beta_j^2 =
+// w . w - 2 alpha (x . w) + alpha^2 (x . x) from the patch's sums of the fused
+// kernel's partials (s3[3 p .. 3 p + 2]), then the omega recurrence and the
+// gate as bomega_kernel; alpha_j also to alpha_f for a gated pass, and to
+// aeff (the alpha the next step's staging applies) when no pass is needed.
+__global__ __launch_bounds__(256) void bomega_fused_kernel(double* __restrict__ T, int m, int j,
+                                                           const double* __restrict__ s3, double* __restrict__ beta,
+                                                           double* __restrict__ aeff, double* __restrict__ alpha_f,
+                                                           double* __restrict__ om, int* __restrict__ flag,
+                                                           int* __restrict__ pending, const int* __restrict__ active,
+                                                           double* __restrict__ npasses) {
+  const int p = blockIdx.x;
+  if (!active[p]) {
+    if (threadIdx.x == 0) flag[p] = 0;
+    return;
+  }
+  __shared__ double mx[4];
+  const double xw = s3[3 * p], ww = s3[3 * p + 1], xx = s3[3 * p + 2];
+  const double al = xw;
+  const double b2 = ww - 2.0 * al * xw + al * al * xx;
+  if (threadIdx.x == 0) alpha_f[p] = al;
+  bomega_body(p, T, m, j, al, sqrt(b2 > 0 ? b2 : 0.0), beta, om, flag, pending, npasses, aeff, mx);
+}
+
 // After a gated whole-basis pass: beta_j from ||w||^2 (nrm[2 p + 1]) and the
 // coupling; the removed components stay out of T.
+// aeff (fused form): 0 after the pass, whose w is fully orthogonalised, so
+// that the next step's staging only scales.
 __global__ void bpro_finish_kernel(double* __restrict__ T, int m, int j, const double* __restrict__ nrm,
                                    double* __restrict__ beta, const int* __restrict__ flag,
-                                   const int* __restrict__ active, int np) {
+                                   const int* __restrict__ active, int np, double* __restrict__ aeff) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np || !active[p] || !flag[p]) return;
+  if (aeff) aeff[p] = 0.0;
   const double bn = sqrt(nrm[2 * p + 1]);
   double* bp = beta + size_t(p) * 2 * kMaxBasis;
   bp[j] = bn;
@@ -357,13 +392,95 @@ __global__ void bpro_finish_kernel(double* __restrict__ T, int m, int j, const d
 }
 
 // omega_{j+1} = omega_j = eps after the whole-basis step j of a cycle; pending cleared.
-__global__ void bomega_reset_kernel(double* __restrict__ om, int j, int* __restrict__ pending) {
+__global__ void bomega_reset_kernel(double* __restrict__ om, int j, int* __restrict__ pending,
+                                    double* __restrict__ aeff) {
   const int p = blockIdx.x;
+  if (aeff && threadIdx.x == 0) aeff[p] = 0.0;  // fused form: that step's w is fully orthogonalised
   for (int k = threadIdx.x; k <= j; k += blockDim.x) {
     om[(size_t(p) * 2 + 0) * kOmS + k] = kBProEps;
     om[(size_t(p) * 2 + 1) * kOmS + k] = kBProEps;
   }
   if (threadIdx.x == 0) pending[p] = 0;
+}
+
+// Fused form, flagged patches: w -= alpha_f v_j (the update the fused step
+// leaves to the next step's staging) before their gated whole-basis pass.
+__global__ __launch_bounds__(256) void bmaterialise_kernel(const BPatchDev* __restrict__ P, int np,
+                                                           float* __restrict__ w, const float* __restrict__ vj,
+                                                           const double* __restrict__ alpha_f,
+                                                           const int* __restrict__ flag,
+                                                           const int* __restrict__ active, size_t Ntot, size_t chunk) {
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p] || !flag[p]) continue;
+    const double a = alpha_f[p];
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) w[e] = float(double(w[e]) - a * double(vj[e]));
+  }
+}
+
+// Fused form, end of a cycle: v_m = (w - aeff_p v_{m-1}) / beta_p,m-1 per
+// patch, the scaling a next step would have done.
+__global__ __launch_bounds__(256) void bfused_last_kernel(const BPatchDev* __restrict__ P, int np,
+                                                          const float* __restrict__ w,
+                                                          const float* __restrict__ vprev,
+                                                          const double* __restrict__ aeff,
+                                                          const double* __restrict__ beta, int j,
+                                                          float* __restrict__ out, const int* __restrict__ active,
+                                                          size_t Ntot, size_t chunk) {
+  const Range r = my_range(Ntot, chunk);
+  for (int p = 0; p < np; p++) {
+    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
+    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
+    if (lo >= hi || !active[p]) continue;
+    const double a = aeff[p], ib = beta[size_t(p) * 2 * kMaxBasis + kMaxBasis + j];
+    for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x)
+      out[e] = float((double(w[e]) - a * double(vprev[e])) * ib);
+  }
+}
+
+// One fused Lanczos step for every active patch (lanczos_step_tile on the
+// combined tile list, XCD-aware as bspmv_kernel): v_j formed from the
+// previous step's w while the tile is staged and written to the basis, w_out =
+// L v_j - beta_j-1 v_j-1, and per tile the partial sums x . w, w . w, x . x
+// (part3[k * ntiles + tile]).  a, 1/beta and beta per patch from aeff and
+// beta at j - 1.  w_in and w_out are distinct buffers.
+template <int ROWS, typename AT>
+__global__ __launch_bounds__(kSpmvTX * 4) void bstep_kernel(const BPatchDev* __restrict__ P, int np, int ntiles,
+                                                            const float* __restrict__ w_in,
+                                                            const float* __restrict__ v_prev,
+                                                            float* __restrict__ v_out, float* __restrict__ w_out,
+                                                            const double* __restrict__ aeff,
+                                                            const double* __restrict__ beta, int j,
+                                                            const int* __restrict__ active,
+                                                            double* __restrict__ part3) {
+  constexpr int kTY = 4 * ROWS, kTileH = kTY + 2 * kICRadius;
+  __shared__ AT u[kTileH * kSpmvTileW];
+  __shared__ float xs[kTileH * kSpmvTileW];
+  __shared__ double red[3][4];
+  const int per_xcd = (ntiles + kNumXcd - 1) / kNumXcd;
+  const int tile = int(blockIdx.x % kNumXcd) * per_xcd + int(blockIdx.x / kNumXcd);
+  if (tile >= ntiles) return;
+  const int p = patch_of_tile(P, np, tile);
+  if (!active[p]) return;  // whole workgroup; its partials are never summed
+  const BPatchDev q = P[p];
+  const double* bp = beta + size_t(p) * 2 * kMaxBasis;
+  const double a = aeff[p], ib = bp[kMaxBasis + j - 1], b = bp[j - 1];
+  const int t = tile - q.tile0;
+  const int x0 = (t % q.tiles_x) * kSpmvTX, y0 = (t / q.tiles_x) * kTY;
+  double acc[3];
+  lanczos_step_tile<ROWS, AT, 4, 1>(q.wup, q.s, w_in + q.voff, v_prev + q.voff, v_out + q.voff, w_out + q.voff, a,
+                                    ib, b, q.W, q.H, q.Wp, q.Hp, q.Np, x0, y0, u, xs, acc);
+  const int tid = threadIdx.y * kSpmvTX + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double v = acc[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((tid & 63) == 0) red[k][tid >> 6] = v;
+  }
+  __syncthreads();
+  if (tid < 3) part3[size_t(tid) * ntiles + tile] = red[tid][0] + red[tid][1] + red[tid][2] + red[tid][3];
 }
 
 // out = w / beta_p,j per patch (or * 1/sqrt(nrm2[p * nstride]) when nrm2).
@@ -507,7 +624,34 @@ struct BatchState {
   double* xy = nullptr;
   BSpmvArgs sa{};
   double bytes_spmv = 0;   // algorithmic bytes of one batched SpMV
+  double bytes_step = 0;   // algorithmic bytes of one batched fused step
   explicit BatchState(Context& cc) : c(cc) {}
+
+  // One fused Lanczos step of every active patch (bstep_kernel).
+  void step(const float* w_in, const float* v_prev, float* v_out, float* w_out, const double* aeff,
+            const double* beta, int j, double* part3) {
+    KScope ks(c, kKSpmv, bytes_step);
+    const dim3 g(unsigned(div_up(ntiles, kNumXcd) * kNumXcd)), b(kSpmvTX, 4);
+#define GPB_BSTEP(R, AT)                                                                                       \
+  hipLaunchKernelGGL((bstep_kernel<R, AT>), g, b, 0, c.stream, P, np, ntiles, w_in, v_prev, v_out, w_out, aeff, \
+                     beta, j, active, part3);
+    const bool f64 = c.spmv_fp64;
+    switch (rows) {
+      case 2:
+        if (f64) GPB_BSTEP(2, double) else GPB_BSTEP(2, float)
+        break;
+      case 4:
+        if (f64) GPB_BSTEP(4, double) else GPB_BSTEP(4, float)
+        break;
+      case 8:
+        if (f64) GPB_BSTEP(8, double) else GPB_BSTEP(8, float)
+        break;
+      default:
+        throw ArgError("unsupported SpMV tile");
+    }
+#undef GPB_BSTEP
+    GPB_LAUNCH_CHECK();
+  }
 
   void spmv(const float* x, float* y, double* xy_out) {
     KScope ks(c, kKSpmv, bytes_spmv);
@@ -659,6 +803,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     d.tiles_x = int(div_up(st.W, kSpmvTX));
     off += st.Np;
     bs.bytes_spmv += double(st.W) * st.H * (4 * kStencilHalf + 12);
+    bs.bytes_step += double(st.W) * st.H * (4 * kStencilHalf + 20);
   }
   bs.Ntot = off;
   constexpr int kMaxG = 2048;
@@ -790,6 +935,21 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
   const bool pro = c.reorth_mode >= 1 && eig_partial_reorth(opt) && m <= kProjectMaxN;
   double* om = c.ws<double>("bat.omega", size_t(np) * 2 * kOmS);
   int* pend = c.ws<int>("bat.pending", size_t(np));
+  // fused steps (gpb_set_lanczos_fusion): one pass per three-term step, w
+  // alternating between two buffers (neighbouring tiles read the previous w in
+  // their halos while a tile writes the new one); only interior pixels are
+  // written, so the basis and w buffers start zero-filled, halos included
+  const bool fusedb = pro && c.lanczos_fused;
+  float* wb[2] = {w, fusedb ? c.ws<float>("bat.w2", Nt) : w};
+  double* aeff = c.ws<double>("bat.aeff", size_t(np));
+  double* alpha_f = c.ws<double>("bat.alphaf", size_t(np));
+  double* s3 = c.ws<double>("bat.s3", size_t(3) * np);
+  double* part3 = c.ws<double>("bat.part3", size_t(3) * bs.ntiles);
+  if (fusedb) {
+    launch_fill(c, V, size_t(m + 1) * Nt, 0.f);
+    launch_fill(c, V2, size_t(m + 1) * Nt, 0.f);
+    launch_fill(c, wb[1], Nt, 0.f);
+  }
   int* rows_dev = c.ws<int>("bat.rows", size_t(K + 1));
   GPB_HIP(hipMemcpyAsync(ones, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
   GPB_HIP(hipMemsetAsync(T, 0, sizeof(double) * np * m * m, c.stream));
@@ -840,6 +1000,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     for (int j = k0; j < m; j++) {
       float* vj = Vc + size_t(j) * Nt;
       const int n = j + 1;
+      float* const w = wb[j & 1];
       if (j == k0) {
         bs.spmv(vj, w, nullptr);
         bs.project(Vc, Nt, n, w, nullptr, nullptr, j, nullptr, h1);
@@ -850,9 +1011,31 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
                            nrm + 1, nrm + 1, beta, bs.active);
         GPB_LAUNCH_CHECK();
         if (pro) {
-          hipLaunchKernelGGL(bomega_reset_kernel, dim3(np), dim3(256), 0, c.stream, om, j, pend);
+          hipLaunchKernelGGL(bomega_reset_kernel, dim3(np), dim3(256), 0, c.stream, om, j, pend,
+                             fusedb ? aeff : nullptr);
           GPB_LAUNCH_CHECK();
         }
+        if (fusedb) continue;  // the next step's staging scales w into v_j+1
+      } else if (fusedb) {
+        // v_j formed from the previous step's w while the tiles are staged,
+        // then w = L v_j - beta_j-1 v_j-1; alpha_j, beta_j and omega per patch
+        // from the partial sums; the gated whole-basis pass for the flagged
+        bs.step(wb[(j - 1) & 1], Vc + size_t(j - 1) * Nt, vj, w, aeff, beta, j, part3);
+        hipLaunchKernelGGL(breduce_kernel, dim3(3, np), dim3(kRedThreads), 0, c.stream, part3, bs.ntiles, bs.trange, 1,
+                           1, s3, 3, nullptr, bs.active);
+        GPB_LAUNCH_CHECK();
+        hipLaunchKernelGGL(bomega_fused_kernel, dim3(np), dim3(256), 0, c.stream, T, m, j, s3, beta, aeff, alpha_f, om,
+                           flag, pend, bs.active, nsecond);
+        GPB_LAUNCH_CHECK();
+        hipLaunchKernelGGL(bmaterialise_kernel, dim3(bs.G), dim3(256), 0, c.stream, bs.P, np, w, vj, alpha_f, flag,
+                           bs.active, Nt, bs.chunk);
+        GPB_LAUNCH_CHECK();
+        bs.project(Vc, Nt, n, w, nullptr, nullptr, j, flag, h2);
+        bs.update(Vc, Nt, n, h2, w, nrm + 1, flag);
+        hipLaunchKernelGGL(bpro_finish_kernel, dim3(div_up(np, 64)), dim3(64), 0, c.stream, T, m, j, nrm, beta, flag,
+                           bs.active, np, aeff);
+        GPB_LAUNCH_CHECK();
+        continue;
       } else if (pro) {
         bs.spmv(vj, w, alpha);
         {
@@ -870,7 +1053,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
         bs.project(Vc, Nt, n, w, nullptr, nullptr, j, flag, h2);
         bs.update(Vc, Nt, n, h2, w, nrm + 1, flag);
         hipLaunchKernelGGL(bpro_finish_kernel, dim3(div_up(np, 64)), dim3(64), 0, c.stream, T, m, j, nrm, beta, flag,
-                           bs.active, np);
+                           bs.active, np, nullptr);
         GPB_LAUNCH_CHECK();
       } else {
         bs.spmv(vj, w, alpha);
@@ -886,6 +1069,12 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
         GPB_LAUNCH_CHECK();
       }
       bs.scale(w, beta, j, nullptr, 0, Vc + size_t(j + 1) * Nt);
+    }
+    if (fusedb) {
+      // the residual direction v_m: the scaling a next step would have done
+      hipLaunchKernelGGL(bfused_last_kernel, dim3(bs.GW), dim3(256), 0, c.stream, bs.P, np, wb[(m - 1) & 1],
+                         Vc + size_t(m - 1) * Nt, aeff, beta, m - 1, Vc + size_t(m) * Nt, bs.active, Nt, bs.chunk_w);
+      GPB_LAUNCH_CHECK();
     }
   };
 
