@@ -305,6 +305,50 @@ __device__ __forceinline__ double block_sum_waves(double v, double* red) {
   return t;
 }
 
+// init + a[0] b[0] + ... + a[n-1] b[n-1], accumulated in order of the index
+// (the same rounding as the plain loop), with the loads of 32 terms issued
+// before their FMAs: the plain loop waits one memory latency per term, which
+// made the one-block recurrence kernels latency-bound.
+__device__ __forceinline__ double dot_in_order(const double* __restrict__ a, const double* __restrict__ b, int n,
+                                               double init) {
+  constexpr int U = 32;
+  double s = init;
+  for (int l0 = 0; l0 < n; l0 += U) {
+    double x[U], y[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      x[u] = l0 + u < n ? a[l0 + u] : 0.0;
+      y[u] = l0 + u < n ? b[l0 + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (l0 + u < n) s += x[u] * y[u];
+  }
+  return s;
+}
+
+// Per-thread strided sums s[k] = sum of P[k * G + b] over b = tid, tid + NT, ...
+// (in that order) for K arrays of G partials, with eight strides' loads issued
+// together.
+template <int K, int NT>
+__device__ __forceinline__ void strided_sums(const double* __restrict__ P, int G, int tid, double* s) {
+  constexpr int U = 8;
+#pragma unroll
+  for (int k = 0; k < K; k++) s[k] = 0.0;
+  for (int b0 = tid; b0 < G; b0 += U * NT) {
+    double x[U][K];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < K; k++) x[u][k] = b0 + u * NT < G ? P[size_t(k) * G + b0 + u * NT] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int k = 0; k < K; k++)
+        if (b0 + u * NT < G) s[k] += x[u][k];
+  }
+}
+
 // Block sum of one double per thread (256 threads); result in thread 0.
 __device__ __forceinline__ double block_sum256(double v, double* red4) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -191,8 +191,8 @@ __global__ __launch_bounds__(kRedThreads) void breduce_kernel(const double* __re
   if (!active[p] || (gate && !gate[p])) return;
   __shared__ double red[kRedThreads / 64];
   const double* row = flat ? partial + size_t(i) * G : partial + (size_t(p) * pstride + i) * G;
-  double a = 0;
-  for (int b = brange[2 * p] + int(threadIdx.x); b <= brange[2 * p + 1]; b += kRedThreads) a += row[b];
+  double a;
+  strided_sums<1, kRedThreads>(row + brange[2 * p], brange[2 * p + 1] - brange[2 * p] + 1, threadIdx.x, &a);
   for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
   __syncthreads();
@@ -297,8 +297,7 @@ __device__ __forceinline__ void bomega_body(int p, double* __restrict__ T, int m
   const double bc = bp[j - 1];
   double o = 0;
   if (t < j) {
-    double tw = Tp[size_t(t) * m + j];
-    for (int l = 0; l < j; l++) tw += Tp[size_t(t) * m + l] * om0[l];
+    const double tw = dot_in_order(Tp + size_t(t) * m, om0, j, Tp[size_t(t) * m + j]);
     const double prev = (t == j - 1) ? 1.0 : om_1[t];
     const double e = tw - al * om0[t] - bc * prev;
     const double noise = kBProEps * (bn + fabs(Tp[size_t(t) * m + t + 1]));

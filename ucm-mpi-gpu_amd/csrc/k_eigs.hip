@@ -644,8 +644,8 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const double* __res
                                                              double* __restrict__ out, const int* __restrict__ gate) {
   if (gate && *gate == 0) return;
   __shared__ double sh[16];
-  double a = 0;
-  for (int b = threadIdx.x; b < G; b += kRedThreads) a += partial[size_t(blockIdx.x) * G + b];
+  double a;
+  strided_sums<1, kRedThreads>(partial + size_t(blockIdx.x) * G, G, threadIdx.x, &a);
   const double r = block_sum(a, sh);
   if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
@@ -751,8 +751,7 @@ __device__ void omega_body(double* __restrict__ T, int m, int j, double al, doub
   const double bc = beta[j - 1];
   double om = 0;
   if (t < j) {
-    double tw = T[size_t(t) * m + j];  // om0[j] = 1
-    for (int l = 0; l < j; l++) tw += T[size_t(t) * m + l] * om0[l];
+    const double tw = dot_in_order(T + size_t(t) * m, om0, j, T[size_t(t) * m + j]);  // om0[j] = 1
     const double prev = (t == j - 1) ? 1.0 : omp[t];
     const double e = tw - al * om0[t] - bc * prev;
     const double noise = kProEps * (bn + fabs(T[size_t(t) * m + t + 1]));
@@ -798,8 +797,8 @@ __global__ __launch_bounds__(256) void omega_kernel(double* __restrict__ T, int 
                                                     double* __restrict__ acc, double thr, int* __restrict__ first_viol) {
   __shared__ double red4[4];
   __shared__ double mx[4];
-  double a = 0;
-  for (int b = threadIdx.x; b < G; b += 256) a += nrm_partial[b];
+  double a;
+  strided_sums<1, 256>(nrm_partial, G, threadIdx.x, &a);
   const double nrm = block_sum_waves<4>(a, red4);
   omega_body(T, m, j, *alpha, sqrt(nrm), beta, om0, omp, om_1, flag, pending, np, acc, thr, first_viol, nullptr, mx);
 }
@@ -819,12 +818,8 @@ __global__ __launch_bounds__(256) void omega_fused_kernel(double* __restrict__ T
                                                           int* __restrict__ first_viol) {
   __shared__ double red4[4];
   __shared__ double mx[4];
-  double s3[3] = {0, 0, 0};
-  for (int b = threadIdx.x; b < G; b += 256) {
-    s3[0] += partial[b];
-    s3[1] += partial[size_t(G) + b];
-    s3[2] += partial[size_t(2) * G + b];
-  }
+  double s3[3];
+  strided_sums<3, 256>(partial, G, threadIdx.x, s3);
   const double xw = block_sum_waves<4>(s3[0], red4);
   __syncthreads();
   const double ww = block_sum_waves<4>(s3[1], red4);
