@@ -1471,6 +1471,38 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
         if (g[i]) (void)hipGraphExecDestroy(g[i]);
     }
   } graph_guard{graphs};
+  // The kept Ritz vectors inherit the basis' loss of orthogonality, and the
+  // residual vector's overlap with them would feed every three-term step of
+  // the next cycle (the omega recurrence assumes it is at rounding level):
+  // Gram-Schmidt it against them twice, then normalise.
+  auto clean_residual = [&](float* Vn) {
+    float* vk = Vn + size_t(keep) * Np;
+    if (keep <= kProjectMaxN) {
+      red.project(Vn, Np, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+      red.update_project(Vn, Np, keep, h1, vk, h2);
+      red.update(Vn, Np, keep, h2, vk, rnrm);
+    } else {
+      for (int pass = 0; pass < 2; pass++) {
+        red.project_any(Vn, Np, keep, vk, nullptr, h1);
+        red.update(Vn, Np, keep, h1, vk, rnrm);
+      }
+    }
+    launch_scale_by(c, vk, rnrm, Np);
+  };
+  // With partial reorthogonalisation the clean-up depends only on the new
+  // basis, so it is queued before the host reads the residual norms and runs
+  // while the host decides (after the last restart it is work on a vector no
+  // output uses).  Not under per-kernel timing, whose events are read at that
+  // point.
+  const bool early_clean = pro && !c.kt.on;
+  hipEvent_t norms_ready = nullptr;
+  if (early_clean) GPB_HIP(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } norms_guard{norms_ready};
   while (true) {
     if (use_graph && k == keep) {
       hipGraphExec_t& ge = graphs[parity];
@@ -1550,7 +1582,13 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     }
     GPB_HIP(hipMemcpyAsync(wnrm_h, h, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(gacc_p, gacc, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipStreamSynchronize(c.stream));
+    if (early_clean) {
+      GPB_HIP(hipEventRecord(norms_ready, c.stream));
+      clean_residual(V2);
+      GPB_HIP(hipEventSynchronize(norms_ready));
+    } else {
+      GPB_HIP(hipStreamSynchronize(c.stream));
+    }
     std::copy(gacc_p, gacc_p + 3, gacc_h);
     c.kt.fold();
     if (c.kt.timing(kKDots)) c.kt.bytes[kKDots] += gacc_h[0] - stats.pass2_bytes[0];
@@ -1577,24 +1615,7 @@ void lanczos_solve(Context& c, const Stencil& st, const EigOptions& opt_in, doub
     GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * m * m, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
-    if (pro) {
-      // The kept Ritz vectors inherit the basis' loss of orthogonality, and
-      // the residual vector's overlap with them would feed every three-term
-      // step of the next cycle (the omega recurrence assumes it is at
-      // rounding level): Gram-Schmidt it against them twice, then normalise.
-      float* vk = V + size_t(keep) * Np;
-      if (keep <= kProjectMaxN) {
-        red.project(V, Np, keep, vk, nullptr, nullptr, 0, nullptr, h1);
-        red.update_project(V, Np, keep, h1, vk, h2);
-        red.update(V, Np, keep, h2, vk, rnrm);
-      } else {
-        for (int pass = 0; pass < 2; pass++) {
-          red.project_any(V, Np, keep, vk, nullptr, h1);
-          red.update(V, Np, keep, h1, vk, rnrm);
-        }
-      }
-      launch_scale_by(c, vk, rnrm, Np);
-    }
+    if (pro && !early_clean) clean_residual(V);
   }
   stats.iterations = matvecs;
   stats.converged = 0;

@@ -16,8 +16,11 @@ namespace gpb {
 // columns.
 void symeig_desc(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& z) {
   std::vector<double> d(n), e(n);
-  z = a;
-  auto Z = [&](int i, int j) -> double& { return z[size_t(i) * n + j]; };
+  z = a;  // symmetric: the same array read column-major
+  // Z is held column-major: the inner loops below run down columns (the
+  // Householder updates and the QL rotations of two columns), which are then
+  // contiguous.
+  auto Z = [&](int i, int j) -> double& { return z[size_t(j) * n + i]; };
   // Householder tridiagonalisation (accumulating the transformation in Z)
   for (int j = 0; j < n; j++) d[j] = Z(n - 1, j);
   for (int i = n - 1; i > 0; i--) {
