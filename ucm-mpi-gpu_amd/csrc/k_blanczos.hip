@@ -893,6 +893,30 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   GPB_LAUNCH_CHECK();
   hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, gacc, size_t(1), 0.0);
   GPB_LAUNCH_CHECK();
+  // The residual panel's overlap with the kept Ritz vectors (inherited from
+  // the basis' semi-orthogonality) would feed every local step of the next
+  // cycle: Gram-Schmidt it against them twice, re-orthonormalise.
+  auto clean_panel = [&](float* Vn) {
+    float* P = Vn + size_t(keep) * Np;
+    for (int pass = 0; pass < 2; pass++) {
+      ops.dots(Vn, keep, P, nullptr, H1);
+      ops.update(Vn, keep, H1, P);
+    }
+    ops.cholqr2(P, G, Rinv, R1, Rtot, brk);
+  };
+  // The clean-up depends only on the new basis: it is queued before the host
+  // reads the Ritz norms and runs while the host decides (after the last
+  // restart it is work on a panel no output uses).  Not under per-kernel
+  // timing, which reads its events at the restart's first sync.
+  const bool early_clean = semi && !c.kt.on;
+  hipEvent_t norms_ready = nullptr;
+  if (early_clean) GPB_HIP(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } norms_guard{norms_ready};
   while (true) {
     const int cycle_start = k;
     while (k + B <= M) {
@@ -987,7 +1011,13 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     // ||D^1/2 u_i|| for the wanted Ritz vectors
     ops.wnorms(V2, K, nrm);
     GPB_HIP(hipMemcpyAsync(nh, nrm, sizeof(double) * K, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipStreamSynchronize(c.stream));
+    if (early_clean) {
+      GPB_HIP(hipEventRecord(norms_ready, c.stream));
+      clean_panel(V2);
+      GPB_HIP(hipEventSynchronize(norms_ready));
+    } else {
+      GPB_HIP(hipStreamSynchronize(c.stream));
+    }
     std::swap(V, V2);
     double worst = 0;
     for (int i = 0; i < K; i++) {
@@ -1016,17 +1046,7 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
     GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * M * M, hipMemcpyHostToDevice, c.stream));
     k = keep;
     stats.restarts++;
-    if (semi) {
-      // the residual panel's overlap with the kept Ritz vectors (inherited
-      // from the basis' semi-orthogonality) would feed every local step of
-      // the next cycle: Gram-Schmidt it against them twice, re-orthonormalise
-      float* P = V + size_t(keep) * Np;
-      for (int pass = 0; pass < 2; pass++) {
-        ops.dots(V, keep, P, nullptr, H1);
-        ops.update(V, keep, H1, P);
-      }
-      ops.cholqr2(P, G, Rinv, R1, Rtot, brk);
-    }
+    if (semi && !early_clean) clean_panel(V);
   }
   if (semi) {
     double g = 0, d = 0;
