@@ -1076,6 +1076,29 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       GPB_LAUNCH_CHECK();
     }
   };
+  // The residual vector's overlap with the kept Ritz vectors would feed every
+  // three-term step of the next cycle: Gram-Schmidt it against them twice,
+  // then normalise (per active patch).  With partial reorthogonalisation it
+  // is queued before the host reads the Ritz norms and runs while the host
+  // decides (for a patch that has just converged it is work on a vector no
+  // output uses); not under per-kernel timing, whose events are read there.
+  auto clean_residual = [&](float* Vn) {
+    float* vk = Vn + size_t(keep) * Nt;
+    for (int pass = 0; pass < 2; pass++) {
+      bs.project(Vn, Nt, keep, vk, nullptr, nullptr, 0, nullptr, h1);
+      bs.update(Vn, Nt, keep, h1, vk, nrm + 1);
+    }
+    bs.scale(vk, nullptr, 0, nrm + 1, 2, vk);
+  };
+  const bool early_clean = pro && !c.kt.on;
+  hipEvent_t norms_ready = nullptr;
+  if (early_clean) GPB_HIP(hipEventCreateWithFlags(&norms_ready, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } norms_guard{norms_ready};
 
   while (true) {
     if (use_graph && k == keep) {
@@ -1140,7 +1163,13 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
                        kPS, 0, wn, kPS, nullptr, bs.active);
     GPB_LAUNCH_CHECK();
     GPB_HIP(hipMemcpyAsync(nrmh, wn, sizeof(double) * np * kPS, hipMemcpyDeviceToHost, c.stream));
-    GPB_HIP(hipStreamSynchronize(c.stream));
+    if (early_clean) {
+      GPB_HIP(hipEventRecord(norms_ready, c.stream));
+      clean_residual(V2);
+      GPB_HIP(hipEventSynchronize(norms_ready));
+    } else {
+      GPB_HIP(hipStreamSynchronize(c.stream));
+    }
     c.kt.fold();
     std::swap(V, V2);
     parity ^= 1;
@@ -1181,17 +1210,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     GPB_HIP(hipMemcpyAsync(T, T0, sizeof(double) * np * m * m, hipMemcpyHostToDevice, c.stream));
     if (newly_done) GPB_HIP(hipMemcpyAsync(bs.active, act.data(), sizeof(int) * np, hipMemcpyHostToDevice, c.stream));
     k = keep;
-    if (pro) {
-      // the residual vector's overlap with the kept Ritz vectors would feed
-      // every three-term step of the next cycle: Gram-Schmidt it against them
-      // twice, then normalise (per patch)
-      float* vk = V + size_t(keep) * Nt;
-      for (int pass = 0; pass < 2; pass++) {
-        bs.project(V, Nt, keep, vk, nullptr, nullptr, 0, nullptr, h1);
-        bs.update(V, Nt, keep, h1, vk, nrm + 1);
-      }
-      bs.scale(vk, nullptr, 0, nrm + 1, 2, vk);
-    }
+    if (pro && !early_clean) clean_residual(V);
   }
   for (int g = 0; g < 2; g++)
     if (graphs[g]) (void)hipGraphExecDestroy(graphs[g]);
