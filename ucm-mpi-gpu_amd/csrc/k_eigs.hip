@@ -68,8 +68,13 @@ __global__ __launch_bounds__(kSpmvTX * NY) void spmv_kernel(const float* __restr
 // One fused Lanczos step over the whole image (lanczos_step_tile per tile, the
 // SpMV's XCD-aware tile order): partial[k * ntiles + tile] for k = x . w,
 // w . w, x . x.  a = aeff[j - 1], 1/beta and beta from beta[] at j - 1.
+// For the 64x32 tile of 8 wavefronts, a register budget of 7 waves per SIMD
+// (70 VGPRs, against the 80 the compiler takes unconstrained, which leaves 6)
+// gives the weight stream more loads in flight: 180 -> 178 us per step at
+// 2048^2.  The other shapes keep the compiler's choice.
 template <int ROWS, typename AT, int NY, int XMODE>
-__global__ __launch_bounds__(kSpmvTX * NY) void lanczos_step_kernel(
+__global__ __launch_bounds__(kSpmvTX * NY) __attribute__((amdgpu_waves_per_eu(NY == 8 && ROWS == 4 ? 7 : 1)))
+void lanczos_step_kernel(
     const float* __restrict__ wup, const float* __restrict__ s, const float* __restrict__ w_in,
     const float* __restrict__ v_prev, float* __restrict__ v_out, float* __restrict__ w_out,
     const double* __restrict__ aeff, const double* __restrict__ beta, int j, int W, int H, int Wp, int Hp, size_t Np,
