@@ -256,16 +256,19 @@ int gpb_set_spmv_precision(gpb_ctx* ctx, int fp64);
  * step.  With 1 the gated pass is launched on every step and exits at once
  * when not needed.  With 2 (default) a restart cycle runs without those
  * launches and is redone from the first step whose recurrence triggered,
- * with them; the results equal those of 1.  The block and batched solvers
- * treat 1 and 2 alike.  No counterpart in the reference (Spectra
+ * with them; the results equal those of 1.  The block solver's
+ * semi-orthogonal steps measure each new panel against the whole basis with
+ * 1, and with 2 (default) estimate those overlaps by the block form of the
+ * recurrence, measuring them only on the steps it flags; the batched solver
+ * treats 1 and 2 alike.  No counterpart in the reference (Spectra
  * reorthogonalises fully, SymEigsSolver.h:200-280; damascene not at all,
  * lanczos.cu:192-282). */
 int gpb_set_reorth(gpb_ctx* ctx, int mode);
 
-/* Trigger level of the partial reorthogonalisation's omega recurrence for
- * later solves on ctx (0 = sqrt(2^-24), the default).  Lower levels make the
- * gated pass run more often; for verification.  No counterpart in the
- * reference. */
+/* Trigger level of the partial reorthogonalisation's omega recurrence (and
+ * of the block solver's semi-orthogonality gate) for later solves on ctx
+ * (0 = sqrt(2^-24), the default).  Lower levels make the gated pass run more
+ * often; for verification.  No counterpart in the reference. */
 int gpb_set_reorth_threshold(gpb_ctx* ctx, double level);
 
 /* Single-vector Lanczos for later solves on ctx: 1 (default) replays each

@@ -155,8 +155,9 @@ def _apply_w(orc, Wd, v):
                                                 (256, 224, 4, 1e-4, 100)])
 def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
     """Semi-orthogonal steps (the default for residual targets >= 1e-4: the
-    omega recurrence for the single-vector solver, measured overlaps for the
-    block solver) against the whole-basis pass on every step: with either,
+    omega recurrence for the single-vector solver; for the block solver the
+    block form of the recurrence, mode 2, and measured overlaps, mode 1)
+    against the whole-basis pass on every step: with each,
     every returned pair meets the target when its residual is recomputed in
     FP64 from the oracle's W, and every Ritz value lies within its FP64
     residual in the normalised-Laplacian norm, ||L u - theta u|| / ||u|| with
@@ -175,7 +176,7 @@ def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
         ctx.set_eig_options(nvec=9, basis=40, keep=20, tol=tol)
     else:
         ctx.set_eig_options(nvec=9, basis=64, keep=24, tol=tol, max_matvecs=40000)
-    for partial in (False, True):
+    for partial in ((False, 1, True) if block > 1 else (False, True)):
         ctx.set_block_size(block)
         ctx.set_reorth(partial)
         try:
@@ -198,6 +199,46 @@ def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
             assert np.min(np.abs(ev_ref - ev[i])) <= bound * (1 + 1e-6) + 1e-9
         if partial:
             assert st["second_passes"] < 0.5 * steps
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [1e-9, 1e-6])
+def test_block_estimate_forced_passes_gpu(gpb, orc, ctx, level):
+    """The block solver's estimated semi-orthogonality with the gate lowered
+    so that the estimate flags steps: the flagged steps measure the panel
+    against the whole basis and remove it (the gated measuring pass and
+    update), the step after each is flagged too, and every returned pair
+    still meets the target against the FP64 oracle's W and lies within its
+    residual bound of an oracle eigenvalue."""
+    print(f"block estimate forced passes level {level}", flush=True)
+    tol = 1e-3
+    nm = _tiny_w(orc, gpb, 256, 224, 4)
+    Wd, D = orc.build_w(nm)
+    ev_ref, _, _ = orc.eigs(Wd, D, K=20, tol=1e-12)
+    sD = np.sqrt(D)
+    ctx.set_eig_options(nvec=9, basis=64, keep=24, tol=tol, max_matvecs=40000)
+    ctx.set_block_size(8)
+    ctx.set_reorth(2)
+    ctx.set_reorth_threshold(level)
+    try:
+        ev, vec = ctx.eigs(nm.astype(np.float32))
+        st = ctx.stats()
+    finally:
+        ctx.set_reorth_threshold(0)
+        ctx.set_block_size(1)
+    steps = (st["matvecs"] - 1) // 8 - st["restarts"] - 1
+    print(f"  matvecs {st['matvecs']} restarts {st['restarts']} semi steps {steps} whole-basis passes "
+          f"{st['second_passes']} residual {st['max_residual']:.2e}", flush=True)
+    assert st["fallback"] == 0 and st["max_residual"] <= tol
+    if level <= 1e-9:
+        # below the recurrence's rounding term: every semi-orthogonal step flagged
+        assert st["second_passes"] >= 0.9 * steps
+    for i in range(9):
+        v = vec[i].astype(np.float64)
+        assert orc.gen_residual(Wd, D, ev[i], v) <= 2 * tol + 1e-6
+        r = ((D * v - _apply_w(orc, Wd, v)) - ev[i] * D * v) / sD
+        bound = np.linalg.norm(r) / np.linalg.norm(sD * v)
+        assert np.min(np.abs(ev_ref - ev[i])) <= bound * (1 + 1e-6) + 1e-9
 
 
 @pytest.mark.gpu

@@ -181,6 +181,8 @@ double measured_max_residual(Context& c, const Stencil& st, const float* d_evecs
                              double* resid = nullptr);
 // helpers shared by the solvers
 void launch_reduce(Context& c, const double* partial, int G, int n, double* dst);
+// launch_reduce only when *gate != 0 (dst untouched otherwise)
+void launch_reduce_gated(Context& c, const double* partial, int G, int n, double* dst, const int* gate);
 void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np);
 void launch_locked_start(Context& c, const Stencil& st, float* v0, float* v1_block, int B, size_t ld, uint64_t seed);
 void launch_to_general(Context& c, const float* u, const float* s, float* v, size_t Np);

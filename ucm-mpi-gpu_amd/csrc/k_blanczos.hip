@@ -202,7 +202,8 @@ template <int B, int NV, int PX>
 __global__ __launch_bounds__(256) void bdots_kernel(const float* __restrict__ V, size_t ld, int n,
                                                     const float* __restrict__ Wm, size_t ldw,
                                                     const float* __restrict__ D, size_t Np,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;  // whole grid (the gated reduce skips too)
   __shared__ double red[4][DotsAcc<B, NV, PX>::K];
   __shared__ double tot[kBMaxBasis * B];
   const int tid = threadIdx.x;
@@ -572,13 +573,14 @@ __global__ void fill_f64_kernel(double* p, size_t n, double v) {
 // blocks [lo, n) -- the block three-term recurrence -- unless some overlap
 // with an older vector, relative to the norm the local update leaves
 // (||Pn_r||^2 - sum_{lo <= i < n} H[i][r]^2), exceeds kBSemiOrthTol, in which
-// case the whole basis is removed (*flag = 1).  A second Gram-Schmidt pass
+// case the whole basis is removed (*flag = 1).  The level is kBSemiOrthTol
+// unless set for verification (gpb_set_reorth_threshold).  A second Gram-Schmidt pass
 // against the two blocks follows either way; T keeps the block-tridiagonal
 // entries only.  acc[0] counts the whole-basis updates.
 constexpr double kBSemiOrthTol = 2.44140625e-04;  // sqrt of the FP32 unit roundoff 2^-24
 
 __global__ __launch_bounds__(256) void bgate_kernel(const double* __restrict__ H, int n, int lo, int B,
-                                                    int* __restrict__ flag, double* __restrict__ acc) {
+                                                    int* __restrict__ flag, double* __restrict__ acc, double thr) {
   __shared__ double est[kBMax];
   __shared__ double mx[4];
   const int t = threadIdx.x;
@@ -594,9 +596,102 @@ __global__ __launch_bounds__(256) void bgate_kernel(const double* __restrict__ H
   if ((t & 63) == 0) mx[t >> 6] = m;
   __syncthreads();
   if (t == 0) {
-    const bool f = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3])) > kBSemiOrthTol;
+    const bool f = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3])) > thr;
     *flag = f ? 1 : 0;
     if (f) acc[0] += 1.0;
+  }
+}
+
+// Estimated semi-orthogonality (the block form of the omega recurrence of
+// Simon's partial reorthogonalisation, as in Grimes, Lewis and Simon's
+// shifted block Lanczos), in place of measuring each new panel against the
+// whole basis.  Om[l][c] estimates V_l . P_c for the panel P in question.
+// From A P_k = sum_{l<n} V_l T[l][P_k] + P_n R (R = Rtot of the new panel),
+// for an old row r < lo (below the two local panels [lo, n)):
+//   (V_r . P_n) R = sum_{l<n} T[r][l] Om_k[l] - sum_{lo<=i<n} Om[r][i] H[i]
+// with Om_k rows lo .. k-1 at rounding level and rows k .. n-1 the identity,
+// H = the measured local coefficients (rows lo .. n-1 of this step's first
+// pass), Om[r][i] = Om_l (i < k) or Om_k (i >= k).  X = the right-hand side
+// plus a rounding term eps (|T[r]|_1 + |H|_1) of the sign of each entry (the
+// single-vector recurrence's noise model).  The gate then compares |X[r][c]|
+// with the norm the local update leaves, as bgate_kernel does with measured
+// overlaps; the step after a flagged one is flagged too (pending).
+constexpr double kBOmEps = 5.9604644775390625e-08;  // 2^-24, unit roundoff of the FP32 basis
+__global__ __launch_bounds__(256) void bomega_est_kernel(const double* __restrict__ T, int M, int k, int n, int lo,
+                                                         int B, const double* __restrict__ H,
+                                                         const double* __restrict__ OmK,
+                                                         const double* __restrict__ OmL, double* __restrict__ X,
+                                                         int* __restrict__ flag, int* __restrict__ pending,
+                                                         double* __restrict__ acc, double thr) {
+  __shared__ double est[kBMax];
+  __shared__ double hn;
+  __shared__ double mx[4];
+  const int t = threadIdx.x;
+  if (t < B) {
+    double e = H[size_t(n + t) * B + t];
+    for (int i = lo; i < n; i++) e -= H[size_t(i) * B + t] * H[size_t(i) * B + t];
+    est[t] = fmax(e, 1e-300);
+  }
+  if (t == 0) {
+    double a = 0;
+    for (int q = lo * B; q < n * B; q++) a = fmax(a, fabs(H[q]));
+    hn = a;
+  }
+  __syncthreads();
+  double m = 0;
+  for (int q = t; q < lo * B; q += 256) {
+    const int r = q / B, c = q - r * B;
+    const double* Tr = T + size_t(r) * M;
+    double x = 0, tn = 0;
+    for (int l = 0; l < lo; l++) {
+      x += Tr[l] * OmK[size_t(l) * B + c];
+      tn += fabs(Tr[l]);
+    }
+    x += Tr[k + c];  // rows k .. n-1 of Om_k: the identity
+    for (int l = k; l < n; l++) tn += fabs(Tr[l]);
+    for (int i = lo; i < n; i++) {
+      const double om = i < k ? OmL[size_t(r) * B + (i - lo)] : OmK[size_t(r) * B + (i - k)];
+      x -= om * H[size_t(i) * B + c];
+    }
+    const double noise = kBOmEps * (tn + B * hn);
+    x += x >= 0 ? noise : -noise;
+    X[q] = x;
+    m = fmax(m, fabs(x) / sqrt(est[c]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((t & 63) == 0) mx[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    const bool viol = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3])) > thr;
+    const int pend = *pending;
+    const bool f = viol || pend;
+    *flag = f ? 1 : 0;
+    *pending = (viol && !pend) ? 1 : 0;
+    if (f) acc[0] += 1.0;
+  }
+}
+
+// After CholQR of the new panel: Om_new[r] = X[r] R^-1 (R upper triangular)
+// for the old rows r < lo, rounding level for the rows lo .. n-1 the step
+// orthogonalised against; rounding level for every row when the
+// whole-basis pass ran (flag, or flag == nullptr after a full
+// Gram-Schmidt step).
+__global__ __launch_bounds__(256) void bomega_finish_kernel(const double* __restrict__ X, const double* __restrict__ R,
+                                                            double* __restrict__ Om, int lo, int n, int B,
+                                                            const int* __restrict__ flag) {
+  const bool whole = !flag || *flag;
+  for (int r = threadIdx.x; r < n; r += 256) {
+    double* o = Om + size_t(r) * B;
+    if (whole || r >= lo) {
+      for (int c = 0; c < B; c++) o[c] = kBOmEps;
+      continue;
+    }
+    const double* x = X + size_t(r) * B;
+    for (int c = 0; c < B; c++) {
+      double v = x[c];
+      for (int a = 0; a < c; a++) v -= o[a] * R[size_t(a) * B + c];
+      o[c] = v / R[size_t(c) * B + c];
+    }
   }
 }
 
@@ -707,10 +802,22 @@ struct BlockOps {
       KScope ks(c, kKDots, double(Np) * (4.0 * n + 4.0 * B + (D ? 4 : 0)));
       if (n > kBMaxBasis) throw ArgError("block Lanczos: basis above the dots kernel's capacity");
       hipLaunchKernelGGL((bdots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD>), dim3(NC), dim3(256), 0, c.stream, V,
-                         Np, n, Wm, Np, D, Np, partial);
+                         Np, n, Wm, Np, D, Np, partial, nullptr);
       GPB_LAUNCH_CHECK();
     }
     launch_reduce(c, partial, NC, n * B, dst);
+  }
+  // dots(V, n, Wm, nullptr, dst) only when *gate != 0 (dst untouched otherwise)
+  void dots_gated(const float* V, int n, const float* Wm, double* dst, const int* gate) {
+    if (n <= 0) return;
+    {
+      KScope ks(c, kKDots, 0.0);
+      if (n > kBMaxBasis) throw ArgError("block Lanczos: basis above the dots kernel's capacity");
+      hipLaunchKernelGGL((bdots_kernel<B, BlockShape<B>::NV, BlockShape<B>::PXD>), dim3(NC), dim3(256), 0, c.stream, V,
+                         Np, n, Wm, Np, nullptr, Np, partial, gate);
+      GPB_LAUNCH_CHECK();
+    }
+    launch_reduce_gated(c, partial, NC, n * B, dst, gate);
   }
   // W_r -= sum_i Hm[i][r] V_i, i < n; with gate, only when *gate == want
   void update(const float* V, int n, const double* Hm, float* Wm, const int* gate = nullptr, int want = 1) {
@@ -889,6 +996,20 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   double* gacc = c.ws<double>("beig.gacc", 1);
   int* dflag = c.ws<int>("beig.dflag", 1);
   double* dacc = c.ws<double>("beig.dacc", 1);
+  // With reorthogonalisation mode 2 (the default) the semi-orthogonal steps
+  // estimate the new panel's overlaps with the older basis by the block omega
+  // recurrence (bomega_est_kernel) and measure them only when it triggers;
+  // mode 1 measures them on every step.
+  const bool om_est = semi && c.reorth_mode == 2;
+  // the gate's level: sqrt of the FP32 unit roundoff unless set for verification
+  const double semi_thr = c.pro_threshold > 0 ? c.pro_threshold : kBSemiOrthTol;
+  double* om_buf = om_est ? c.ws<double>("beig.om", size_t(3) * kBMaxBasis * kBMax) : nullptr;
+  double* om_x = om_est ? c.ws<double>("beig.omx", size_t(kBMaxBasis) * kBMax) : nullptr;
+  int* gpend = om_est ? c.ws<int>("beig.gpend", 1) : nullptr;
+  // om slots: the estimates for the previous panel, the current one, and the
+  // one being built
+  int om_l = 0, om_k = 1, om_n = 2;
+  auto om_slot = [&](int i) { return om_buf + size_t(i) * kBMaxBasis * kBMax; };
   hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, dacc, size_t(1), 0.0);
   GPB_LAUNCH_CHECK();
   hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(1), 0, c.stream, gacc, size_t(1), 0.0);
@@ -919,6 +1040,14 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
   } norms_guard{norms_ready};
   while (true) {
     const int cycle_start = k;
+    if (om_est) {
+      // the cycle's first panel was cleaned against the kept vectors
+      hipLaunchKernelGGL(bomega_finish_kernel, dim3(1), dim3(256), 0, c.stream, om_x, Rtot, om_slot(om_k), 0, k, B,
+                         nullptr);
+      GPB_LAUNCH_CHECK();
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, c.stream, gpend, 0);
+      GPB_LAUNCH_CHECK();
+    }
     while (k + B <= M) {
       float* Pk = V + size_t(k) * Np;
       const int n = k + B;
@@ -940,9 +1069,20 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
         GPB_LAUNCH_CHECK();
       } else {
         const int lo = k - B;
-        ops.dots(V, n + B, Pn, nullptr, H1);  // rows n .. n+B-1: the panel's own Gram
-        hipLaunchKernelGGL(bgate_kernel, dim3(1), dim3(256), 0, c.stream, H1, n, lo, B, gflag, gacc);
-        GPB_LAUNCH_CHECK();
+        if (om_est) {
+          // measure the two local panels and the panel's own Gram (rows lo ..
+          // n+B-1), estimate the rest, measure it only when the estimate
+          // triggers the whole-basis pass
+          ops.dots(V + size_t(lo) * Np, n + B - lo, Pn, nullptr, H1 + size_t(lo) * B);
+          hipLaunchKernelGGL(bomega_est_kernel, dim3(1), dim3(256), 0, c.stream, T, M, k, n, lo, B, H1, om_slot(om_k),
+                             om_slot(om_l), om_x, gflag, gpend, gacc, semi_thr);
+          GPB_LAUNCH_CHECK();
+          ops.dots_gated(V, lo, Pn, H1, gflag);
+        } else {
+          ops.dots(V, n + B, Pn, nullptr, H1);  // rows n .. n+B-1: the panel's own Gram
+          hipLaunchKernelGGL(bgate_kernel, dim3(1), dim3(256), 0, c.stream, H1, n, lo, B, gflag, gacc, semi_thr);
+          GPB_LAUNCH_CHECK();
+        }
         ops.update(V, n, H1, Pn, gflag, 1);
         if (ops.fused && ops.dgks) {
           // local update with the re-measure of the two local panels and
@@ -969,6 +1109,16 @@ void block_solve_impl(Context& c, const Stencil& st, const EigOptions& opt_in, c
         GPB_LAUNCH_CHECK();
       }
       ops.cholqr2(Pn, G, Rinv, R1, Rtot, brk, ops.fused);
+      if (om_est) {
+        const bool full = !semi || k == cycle_start;
+        hipLaunchKernelGGL(bomega_finish_kernel, dim3(1), dim3(256), 0, c.stream, om_x, Rtot, om_slot(om_n),
+                           full ? n : k - B, n, B, full ? nullptr : gflag);
+        GPB_LAUNCH_CHECK();
+        const int t = om_l;
+        om_l = om_k;
+        om_k = om_n;
+        om_n = t;
+      }
       k += B;
     }
     kcur = k;  // projected size

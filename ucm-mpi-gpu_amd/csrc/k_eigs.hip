@@ -1101,6 +1101,11 @@ void launch_reduce(Context& c, const double* partial, int G, int n, double* dst)
   GPB_LAUNCH_CHECK();
 }
 
+void launch_reduce_gated(Context& c, const double* partial, int G, int n, double* dst, const int* gate) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(n), dim3(kRedThreads), 0, c.stream, partial, G, dst, gate);
+  GPB_LAUNCH_CHECK();
+}
+
 void launch_ritz(Context& c, const float* V, size_t ld, int m, int nk, const double* Ydev, float* out, size_t Np) {
   if (m > kRitzMaxM) throw ArgError("basis too large for the Ritz recombination");
   const dim3 grid(std::min<unsigned>(2048, div_up(Np, 256)));

@@ -680,7 +680,8 @@ class Context:
         self._chk(lib.gpb_set_reorth(self.h, mode))
 
     def set_reorth_threshold(self, level: float):
-        """Trigger level of the omega recurrence (0 = sqrt(2^-24), the default)."""
+        """Trigger level of the omega recurrence and of the block solver's semi-orthogonality gate
+        (0 = sqrt(2^-24), the default)."""
         self._chk(lib.gpb_set_reorth_threshold(self.h, C.c_double(level)))
 
     def spmv_tile(self, width: int, height: int) -> int:
