@@ -624,7 +624,6 @@ __global__ __launch_bounds__(256) void bomega_est_kernel(const double* __restric
                                                          int* __restrict__ flag, int* __restrict__ pending,
                                                          double* __restrict__ acc, double thr) {
   __shared__ double est[kBMax];
-  __shared__ double hn;
   __shared__ double mx[4];
   const int t = threadIdx.x;
   if (t < B) {
@@ -632,26 +631,52 @@ __global__ __launch_bounds__(256) void bomega_est_kernel(const double* __restric
     for (int i = lo; i < n; i++) e -= H[size_t(i) * B + t] * H[size_t(i) * B + t];
     est[t] = fmax(e, 1e-300);
   }
-  if (t == 0) {
-    double a = 0;
-    for (int q = lo * B; q < n * B; q++) a = fmax(a, fabs(H[q]));
-    hn = a;
-  }
+  // |H|_max over the local rows, for the rounding term
+  double a = 0;
+  for (int q = lo * B + t; q < n * B; q += 256) a = fmax(a, fabs(H[q]));
+  for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+  if ((t & 63) == 0) mx[t >> 6] = a;
+  __syncthreads();
+  const double hn = fmax(fmax(mx[0], mx[1]), fmax(mx[2], mx[3]));
   __syncthreads();
   double m = 0;
   for (int q = t; q < lo * B; q += 256) {
     const int r = q / B, c = q - r * B;
     const double* Tr = T + size_t(r) * M;
     double x = 0, tn = 0;
-    for (int l = 0; l < lo; l++) {
-      x += Tr[l] * OmK[size_t(l) * B + c];
-      tn += fabs(Tr[l]);
+    // sum over l in order, the loads of 16 terms issued before their FMAs
+    for (int l0 = 0; l0 < lo; l0 += 16) {
+      double tv[16], ov[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        tv[u] = l0 + u < lo ? Tr[l0 + u] : 0.0;
+        ov[u] = l0 + u < lo ? OmK[size_t(l0 + u) * B + c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        if (l0 + u < lo) {
+          x += tv[u] * ov[u];
+          tn += fabs(tv[u]);
+        }
     }
     x += Tr[k + c];  // rows k .. n-1 of Om_k: the identity
-    for (int l = k; l < n; l++) tn += fabs(Tr[l]);
-    for (int i = lo; i < n; i++) {
-      const double om = i < k ? OmL[size_t(r) * B + (i - lo)] : OmK[size_t(r) * B + (i - k)];
-      x -= om * H[size_t(i) * B + c];
+    // the two local panels, eight terms' loads at a time
+    for (int i0 = lo; i0 < n; i0 += 8) {
+      double ov[8], hv[8], tv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u;
+        const bool in = i < n;
+        ov[u] = in ? (i < k ? OmL[size_t(r) * B + (i - lo)] : OmK[size_t(r) * B + (i - k)]) : 0.0;
+        hv[u] = in ? H[size_t(i) * B + c] : 0.0;
+        tv[u] = in ? Tr[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (i0 + u < n) {
+          x -= ov[u] * hv[u];
+          tn += fabs(tv[u]);
+        }
     }
     const double noise = kBOmEps * (tn + B * hn);
     x += x >= 0 ? noise : -noise;
