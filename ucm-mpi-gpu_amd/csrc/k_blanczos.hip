@@ -609,11 +609,12 @@ __global__ __launch_bounds__(256) void bgate_kernel(const double* __restrict__ H
 // From A P_k = sum_{l<n} V_l T[l][P_k] + P_n R (R = Rtot of the new panel),
 // for an old row r < lo (below the two local panels [lo, n)):
 //   (V_r . P_n) R = sum_{l<n} T[r][l] Om_k[l] - sum_{lo<=i<n} Om[r][i] H[i]
-// with Om_k rows lo .. k-1 at rounding level and rows k .. n-1 the identity,
-// H = the measured local coefficients (rows lo .. n-1 of this step's first
-// pass), Om[r][i] = Om_l (i < k) or Om_k (i >= k).  X = the right-hand side
-// plus a rounding term eps (|T[r]|_1 + |H|_1) of the sign of each entry (the
-// single-vector recurrence's noise model).  The gate then compares |X[r][c]|
+// with Om_k rows k .. n-1 the identity and rows lo .. k-1 (the previous
+// panel, orthogonalised against P_k) at rounding level, which the sum leaves
+// out, H = the measured local coefficients (rows lo .. n-1 of this step's
+// first pass), Om[r][i] = Om_l (i < k) or Om_k (i >= k).  X = the right-hand
+// side plus a rounding term eps (sum_{l<n} |T[r][l]| + B max |H|) of the sign
+// of each entry (the single-vector recurrence's noise model).  The gate then compares |X[r][c]|
 // with the norm the local update leaves, as bgate_kernel does with measured
 // overlaps; the step after a flagged one is flagged too (pending).
 constexpr double kBOmEps = 5.9604644775390625e-08;  // 2^-24, unit roundoff of the FP32 basis
