@@ -498,47 +498,6 @@ __global__ __launch_bounds__(256) void bscale_kernel(const BPatchDev* __restrict
   }
 }
 
-// out_i = sum_j Y_p[j][i] V_j per patch, i < nk (24 at a time), as ritz_kernel.
-__global__ __launch_bounds__(256) void britz_kernel(const BPatchDev* __restrict__ P, int np,
-                                                    const float* __restrict__ V, size_t ld, int m, int nk,
-                                                    const double* __restrict__ Y /* [np][m][nk] */,
-                                                    float* __restrict__ out, const int* __restrict__ active,
-                                                    size_t Ntot, size_t chunk) {
-  constexpr int kC = 24;
-  const Range r = my_range(Ntot, chunk);
-  for (int p = 0; p < np; p++) {
-    const size_t lo = r.e0 > P[p].voff ? r.e0 : P[p].voff;
-    const size_t hi = r.e1 < P[p].voff + P[p].Np ? r.e1 : P[p].voff + P[p].Np;
-    if (lo >= hi || !active[p]) continue;
-    const double* Yp = Y + size_t(p) * m * nk;
-    for (int i0 = 0; i0 < nk; i0 += kC) {
-      const int cnt = nk - i0 < kC ? nk - i0 : kC;
-      for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-        double acc[kC];
-#pragma unroll
-        for (int i = 0; i < kC; i++) acc[i] = 0;
-        for (int j0 = 0; j0 < m; j0 += 8) {
-          float v[8];
-#pragma unroll
-          for (int t = 0; t < 8; t++) v[t] = V[size_t(j0 + t < m ? j0 + t : 0) * ld + e];
-#pragma unroll
-          for (int t = 0; t < 8; t++) {
-            if (j0 + t >= m) break;
-            const double* y = Yp + size_t(j0 + t) * nk + i0;
-            const double vt = double(v[t]);
-#pragma unroll
-            for (int i = 0; i < kC; i++)
-              if (i < cnt) acc[i] += y[i] * vt;
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < kC; i++)
-          if (i < cnt) out[size_t(i0 + i) * ld + e] = float(acc[i]);
-      }
-    }
-  }
-}
-
 // Weighted squared norms sum_e D_e x_e^2 of rows rows[0..nr) of V, per patch.
 __global__ __launch_bounds__(256) void bwnorm_kernel(const BPatchDev* __restrict__ P, int np,
                                                      const float* __restrict__ V, size_t ld, int nr,
@@ -1146,12 +1105,13 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
         for (int i = 0; i < keep; i++) Yall[(size_t(p) * m + j) * keep + i] = Y[size_t(p)][size_t(j) * m + i];
     }
     GPB_HIP(hipMemcpyAsync(Ydev, Yall, sizeof(double) * np * m * keep, hipMemcpyHostToDevice, c.stream));
-    {
-      KScope ks(c, kKRitz, double(Nt) * 4.0 * (m + keep));
-      hipLaunchKernelGGL(britz_kernel, dim3(bs.GW), dim3(256), 0, c.stream, bs.P, np, V, Nt, m, keep, Ydev, V2,
-                         bs.active, Nt, bs.chunk_w);
-      GPB_LAUNCH_CHECK();
-    }
+    // the single-image Ritz recombination on each active patch's range
+    // (Y staged in LDS, the next basis rows in flight): one combined launch
+    // reading Y through the scalar cache per pixel ran at a third of the rate
+    for (int p = 0; p < np; p++)
+      if (act[size_t(p)])
+        launch_ritz(c, V + ph[size_t(p)].voff, Nt, m, keep, Ydev + size_t(p) * m * keep, V2 + ph[size_t(p)].voff,
+                    ph[size_t(p)].Np);
     GPB_HIP(hipMemcpyAsync(V2 + size_t(keep) * Nt, V + size_t(m) * Nt, Nt * sizeof(float), hipMemcpyDeviceToDevice,
                            c.stream));
     // weighted norms ||D^1/2 u_i|| (i < K) and ||D^1/2 v_m|| per patch
