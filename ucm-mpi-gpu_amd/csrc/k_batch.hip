@@ -850,10 +850,25 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       launch_fill(c, w, Nt, 1.0f);
       set_groups(1024);
       if (!best_rows) {
+        // the tile of the kernel the solve spends its time in: the fused step
+        // when the steps are fused (its best tile differs from the plain
+        // SpMV's), timed with zero scalars on scratch rows of V
+        const bool step_tuned = c.reorth_mode >= 1 && eig_partial_reorth(opt) && m <= kProjectMaxN &&
+                                c.lanczos_fused && m >= 3;
+        double* zs = nullptr;
+        double* tpart = nullptr;
+        if (step_tuned) {
+          zs = c.ws<double>("bat.tune.zero", size_t(np) * 2 * kMaxBasis);
+          launch_fill(c, reinterpret_cast<float*>(zs), size_t(np) * 4 * kMaxBasis, 0.f);
+          size_t tmax = 0;
+          for (int p = 0; p < np; p++) tmax += size_t(ph[size_t(p)].tiles_x) * div_up(ph[size_t(p)].H, 8);
+          tpart = c.ws<double>("bat.tune.part3", size_t(3) * tmax);
+        }
         float bt = 1e30f;
         for (int rows : {2, 4, 8}) {
           set_tiles(rows);
-          const float t = timed([&] { bs.spmv(V, w, nullptr); });
+          const float t = step_tuned ? timed([&] { bs.step(w, V, V + Nt, V + 2 * Nt, zs, zs, 1, tpart); })
+                                     : timed([&] { bs.spmv(V, w, nullptr); });
           if (t < bt) {
             bt = t;
             best_rows = rows;
