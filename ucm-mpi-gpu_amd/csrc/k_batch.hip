@@ -511,18 +511,35 @@ __global__ __launch_bounds__(256) void bwnorm_kernel(const BPatchDev* __restrict
     if (lo >= hi || !active[p]) continue;
     const float* D = P[p].D;
     const size_t voff = P[p].voff;
-    for (int k = 0; k < nr; k++) {
-      const float* v = V + size_t(rows[k]) * ld;
-      double a = 0;
+    // up to 8 rows per pass over the pixels (D read once per pass); each
+    // row's per-thread sum runs over the same pixels in the same order
+    constexpr int kG = 8;
+    for (int k0 = 0; k0 < nr; k0 += kG) {
+      const int cnt = nr - k0 < kG ? nr - k0 : kG;
+      const float* v[kG];
+#pragma unroll
+      for (int t = 0; t < kG; t++) v[t] = V + size_t(rows[k0 + (t < cnt ? t : 0)]) * ld;
+      double a[kG];
+#pragma unroll
+      for (int t = 0; t < kG; t++) a[t] = 0;
       for (size_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-        const double x = v[e];
-        a += double(D[e - voff]) * x * x;
+        const double d = D[e - voff];
+#pragma unroll
+        for (int t = 0; t < kG; t++)
+          if (t < cnt) {
+            const double x = v[t][e];
+            a[t] += d * x * x;
+          }
       }
-      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
-      __syncthreads();
-      if (threadIdx.x == 0) partial[(size_t(p) * kPS + k) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-      __syncthreads();
+      for (int t = 0; t < cnt; t++) {
+        double b = a[t];
+        for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+        __syncthreads();
+        if (threadIdx.x == 0)
+          partial[(size_t(p) * kPS + k0 + t) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+      }
     }
   }
 }
