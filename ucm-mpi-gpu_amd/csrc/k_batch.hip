@@ -1122,7 +1122,10 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * np * m * m, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(betah, beta, sizeof(double) * np * 2 * kMaxBasis, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
-#pragma omp parallel for schedule(dynamic, 1)
+    // one host thread: spreading the patches over an OpenMP team cut this
+    // step from about 1.9 to 0.3 ms per restart at C4, but the team's
+    // spin-waiting afterwards delayed the launches that follow (C4 batched
+    // 814 -> 823-864 ms)
     for (int p = 0; p < np; p++) {
       if (!act[size_t(p)]) continue;
       std::vector<double> A(Th + size_t(p) * m * m, Th + size_t(p + 1) * m * m), thd, Yd;
