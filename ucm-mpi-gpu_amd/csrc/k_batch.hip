@@ -17,6 +17,8 @@
 //     replayed, so a cycle of m - keep steps for all patches is one launch.
 // The host sees T only at restarts (Rayleigh-Ritz for all active patches).
 #include <algorithm>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1122,22 +1124,46 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
     GPB_HIP(hipMemcpyAsync(Th, T, sizeof(double) * np * m * m, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipMemcpyAsync(betah, beta, sizeof(double) * np * 2 * kMaxBasis, hipMemcpyDeviceToHost, c.stream));
     GPB_HIP(hipStreamSynchronize(c.stream));
-    // one host thread: spreading the patches over an OpenMP team cut this
-    // step from about 1.9 to 0.3 ms per restart at C4, but the team's
-    // spin-waiting afterwards delayed the launches that follow (C4 batched
-    // 814 -> 823-864 ms)
-    for (int p = 0; p < np; p++) {
-      if (!act[size_t(p)]) continue;
-      std::vector<double> A(Th + size_t(p) * m * m, Th + size_t(p + 1) * m * m), thd, Yd;
-      symeig_desc(m, A, thd, Yd);
-      th[size_t(p)].assign(size_t(m), 0.0);
-      Y[size_t(p)].assign(size_t(m) * m, 0.0);
-      for (int i = 0; i < m; i++) {
-        th[size_t(p)][size_t(i)] = thd[size_t(m - 1 - i)];
-        for (int j = 0; j < m; j++) Y[size_t(p)][size_t(j) * m + i] = Yd[size_t(j) * m + (m - 1 - i)];
+    // the patches' eigenproblems on up to 8 short-lived host threads, joined
+    // before the next launch (an OpenMP team was slower overall: its
+    // spin-waiting after the loop delayed the launches that follow)
+    {
+      auto rr = [&](int p) {
+        std::vector<double> A(Th + size_t(p) * m * m, Th + size_t(p + 1) * m * m), thd, Yd;
+        symeig_desc(m, A, thd, Yd);
+        th[size_t(p)].assign(size_t(m), 0.0);
+        Y[size_t(p)].assign(size_t(m) * m, 0.0);
+        for (int i = 0; i < m; i++) {
+          th[size_t(p)][size_t(i)] = thd[size_t(m - 1 - i)];
+          for (int j = 0; j < m; j++) Y[size_t(p)][size_t(j) * m + i] = Yd[size_t(j) * m + (m - 1 - i)];
+        }
+        for (int j = 0; j < m; j++)
+          for (int i = 0; i < keep; i++) Yall[(size_t(p) * m + j) * keep + i] = Y[size_t(p)][size_t(j) * m + i];
+      };
+      std::vector<int> todo;
+      for (int p = 0; p < np; p++)
+        if (act[size_t(p)]) todo.push_back(p);
+      const int nt = std::min<int>(8, int(todo.size()));
+      if (nt <= 1) {
+        for (int p : todo) rr(p);
+      } else {
+        // an exception (a QL iteration that does not converge) is carried
+        // back to this thread and rethrown after the join
+        std::vector<std::exception_ptr> err(size_t(nt));
+        auto part = [&](int t) {
+          try {
+            for (size_t q = size_t(t); q < todo.size(); q += size_t(nt)) rr(todo[q]);
+          } catch (...) {
+            err[size_t(t)] = std::current_exception();
+          }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(part, t);
+        part(0);
+        for (auto& th_ : pool) th_.join();
+        for (auto& e : err)
+          if (e) std::rethrow_exception(e);
       }
-      for (int j = 0; j < m; j++)
-        for (int i = 0; i < keep; i++) Yall[(size_t(p) * m + j) * keep + i] = Y[size_t(p)][size_t(j) * m + i];
     }
     GPB_HIP(hipMemcpyAsync(Ydev, Yall, sizeof(double) * np * m * keep, hipMemcpyHostToDevice, c.stream));
     // the single-image Ritz recombination on each active patch's range
