@@ -1149,7 +1149,7 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
       } else {
         // an exception (a QL iteration that does not converge) is carried
         // back to this thread and rethrown after the join
-        std::vector<std::exception_ptr> err(size_t(nt));
+        std::vector<std::exception_ptr> err(static_cast<size_t>(nt));
         auto part = [&](int t) {
           try {
             for (size_t q = size_t(t); q < todo.size(); q += size_t(nt)) rr(todo[q]);
