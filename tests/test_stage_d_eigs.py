@@ -152,6 +152,7 @@ def _apply_w(orc, Wd, v):
 @pytest.mark.parametrize("W,H,seed,tol,block", [(96, 80, 6, 1e-3, 1), (96, 80, 6, 1e-4, 1), (256, 224, 4, 1e-4, 1),
                                                 (256, 224, 4, 1e-3, 1), (256, 224, 4, 1e-3, 8),
                                                 (256, 224, 4, 1e-4, 8), (160, 120, 5, 1e-3, 4),
+                                                (160, 120, 5, 1e-3, 12), (256, 224, 4, 1e-3, 16),
                                                 (256, 224, 4, 1e-4, 100)])
 def test_partial_reorth_gpu(gpb, orc, ctx, W, H, seed, tol, block):
     """Semi-orthogonal steps (the default for residual targets >= 1e-4: the
