@@ -18,6 +18,7 @@
 // The host sees T only at restarts (Rayleigh-Ritz for all active patches).
 #include <algorithm>
 #include <exception>
+#include <system_error>
 #include <thread>
 #include <cmath>
 #include <cstring>
@@ -1158,7 +1159,13 @@ void lanczos_solve_batched(Context& c, const std::vector<Stencil>& sts, const Ei
           }
         };
         std::vector<std::thread> pool;
-        for (int t = 1; t < nt; t++) pool.emplace_back(part, t);
+        int started = 1;
+        try {
+          for (int t = 1; t < nt; t++, started++) pool.emplace_back(part, t);
+        } catch (const std::system_error&) {
+          // no more threads: this thread takes the parts not started
+        }
+        for (int t = started; t < nt; t++) part(t);
         part(0);
         for (auto& th_ : pool) th_.join();
         for (auto& e : err)
