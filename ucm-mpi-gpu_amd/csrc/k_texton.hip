@@ -50,8 +50,16 @@ __global__ __launch_bounds__(256) void texton_kernel(const uint32_t* __restrict_
   int m = 0;
   for (int c = 0; c < ncls; c++) {
     const int end = cls_start[c + 1];
-    float s = 0.f;
-    for (; m < end; m++) s += centre[cls_off[m]];
+    // eight members in flight: grey levels are integers and a class sums at
+    // most 2401 x 255 < 2^24, so the FP32 partial sums are exact and their
+    // order does not change s
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (; m + 8 <= end; m += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] += centre[cls_off[m + u]];
+    }
+    for (; m < end; m++) a[0] += centre[cls_off[m]];
+    const float s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 #pragma unroll
     for (int f = 0; f < kSchmidNF; f++) r[f] = __builtin_fmaf(cls_w[f * ncls + c], s, r[f]);
   }
